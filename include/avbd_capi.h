@@ -1,0 +1,271 @@
+/*
+ * avbd_capi.h - C ABI of the MI355X-native AVBD (augmented vertex block
+ * descent) step.
+ *
+ * The library keeps a world of one or more independent scenes resident in
+ * HBM and advances it with HIP kernels.  The records below are the exchange
+ * format across the boundary (plain C structs, array-of-records); the device
+ * keeps its own structure-of-arrays layout and converts on upload/download.
+ *
+ * Every entry point returns 0 on success and a negative AVBD_E_* code on
+ * failure; avbd_last_error() returns a message for the calling thread.  No
+ * exception crosses the boundary.
+ *
+ * Reference interface each entry point replaces (SkyePhyX, src/physics):
+ *   avbd_create/avbd_destroy  -> Solver::Solver/~Solver, Solver::Start/Clear
+ *                                (solver.hpp:24-41, solver.cpp:18-47)
+ *   avbd_upload               -> Solver::AddBody/AddParticle/AddForce/AddEnergy
+ *                                (solver.hpp:43-46, solver.cpp:50-164): the host
+ *                                facade builds the world, this call moves it
+ *                                into device memory
+ *   avbd_step                 -> Solver::Step (solver.hpp:41, solver.cpp:199-627)
+ *   avbd_download             -> reads of bodyPtrs[i]->transform/velocity,
+ *                                Force::constraintPoints, Manifold contacts
+ *                                (solver.hpp:32-34, geometry.hpp:148-256,
+ *                                force.hpp:9-24,167-198)
+ *   avbd_get_status           -> Solver::emergencyStop (solver.hpp:86),
+ *                                CheckExplosion (solver.cpp:182-196)
+ *   avbd_get_timings          -> Solver::timings / SolverTimings
+ *                                (solver.hpp:10-19, solver.cpp:599-626)
+ *   avbd_step_scenes          -> (no counterpart: batched/served stepping of a
+ *                                subset of resident scenes)
+ */
+#ifndef AVBD_CAPI_H
+#define AVBD_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVBD_ABI_VERSION 1
+
+/* status codes */
+#define AVBD_OK 0
+#define AVBD_E_ARG (-1)       /* bad argument / inconsistent world */
+#define AVBD_E_HIP (-2)       /* HIP runtime error */
+#define AVBD_E_CAPACITY (-3)  /* a device capacity was exceeded */
+#define AVBD_E_STATE (-4)     /* call not valid in the current state */
+#define AVBD_E_NOGPU (-5)     /* no usable gfx950 device */
+#define AVBD_E_UNSUPPORTED (-6)
+
+/* body flags */
+#define AVBD_BODY_STATIC 1u
+#define AVBD_BODY_PARTICLE 2u
+
+/* force flags */
+#define AVBD_FORCE_DISABLED 1u
+
+/* eigenvalue projection modes (math.hpp:99-104) */
+#define AVBD_PROJ_CLAMP 0
+#define AVBD_PROJ_ABSOLUTE 1
+#define AVBD_PROJ_ADAPTIVE 2
+
+/* Per-scene solver parameters and the scene's sticky status word.
+ * Mirrors Solver's public tunables (solver.hpp:69-86); gravity and world
+ * bounds are constants in the reference (solver.cpp:7, constants.hpp:11-12)
+ * and are per-scene parameters here (defaults (0,-9.81,0) and +-50). */
+typedef struct avbd_scene {
+    float dt;                 /* stepValue */
+    int32_t iterations;       /* numIterations */
+    float alpha;
+    float beta;
+    float gamma;
+    float on_penetration_penalty;
+    int32_t post_stabilization; /* bool */
+    int32_t projection_mode;    /* AVBD_PROJ_* */
+    float gravity[3];
+    float bounds_min[3];
+    float bounds_max[3];
+    int32_t emergency_stop;   /* state: nonzero once CheckExplosion fired */
+    int32_t n_removed;        /* state: bodies removed out of bounds (cumulative) */
+} avbd_scene;
+
+/* One rigid body or particle (Mesh, geometry.hpp:148-256).
+ * Quaternions are stored (w, x, y, z).  `inertia` is the body-frame diagonal
+ * inertia; `radius` the broad-phase detection radius (|scale|/2,
+ * solver.cpp:65).  Bodies of one scene are contiguous and in solverIndex
+ * order. */
+typedef struct avbd_body {
+    float pos[3];
+    float rot[4];
+    float scale[3];
+    float vel[3];
+    float ang_vel[3];
+    float prev_vel[3];
+    float prev_ang_vel[3];
+    float mass;
+    float inertia[3];
+    float friction;
+    float radius;
+    uint32_t flags;   /* AVBD_BODY_* */
+    int32_t scene;
+} avbd_body;
+
+/* Joint (force.hpp:116-143, joint.cpp).  body_a == -1 is the world anchor;
+ * body indices are global (into the body array).  `serial` orders forces by
+ * creation: a body visits its forces in increasing serial order
+ * (Mesh::forces append order, force.cpp:5-9). */
+typedef struct avbd_joint {
+    int32_t body_a;
+    int32_t body_b;
+    int64_t serial;
+    uint32_t flags;           /* AVBD_FORCE_DISABLED */
+    float r_a[3];
+    float r_b[3];
+    float rest_rot[4];        /* (w, x, y, z) */
+    float torque_arm;
+    float stiffness[6];
+    float fracture[6];
+    float fmin[6];
+    float fmax[6];
+    float lambda[6];
+    float penalty[6];
+} avbd_joint;
+
+/* Spring (force.hpp:92-113, spring.cpp). */
+typedef struct avbd_spring {
+    int32_t body_a;
+    int32_t body_b;
+    int64_t serial;
+    uint32_t flags;
+    float r_a[3];
+    float r_b[3];
+    float rest_length;
+    float stiffness;
+    float fracture;
+    float fmin;
+    float fmax;
+    float lambda;
+    float penalty;
+} avbd_spring;
+
+/* One contact of a manifold (collision.hpp:8-18 plus the warm-start row
+ * state of its three constraint rows: normal, tangent1, tangent2). */
+typedef struct avbd_contact {
+    uint32_t id;
+    int32_t stick;
+    float r_a[3];             /* body-A local, unscaled */
+    float r_b[3];             /* body-B local, unscaled */
+    float normal[3];          /* A -> B */
+    float penetration;
+    float lambda[3];
+    float penalty[3];
+} avbd_contact;
+
+/* Contact manifold between two boxes (force.hpp:149-198, manifold.cpp). */
+typedef struct avbd_manifold {
+    int32_t body_a;
+    int32_t body_b;
+    int64_t serial;
+    int32_t n_contacts;
+    float friction;
+    avbd_contact contacts[8];
+} avbd_manifold;
+
+/* Linear tetrahedron with Neo-Hookean energy (energy.hpp:48-85).
+ * dm_inv is row-major; the per-vertex shape gradients follow from it
+ * (neohookeanfem.cpp:26-36).  `serial` orders a vertex's energies. */
+typedef struct avbd_tet {
+    int32_t v[4];
+    int32_t serial;
+    float dm_inv[9];
+    float rest_volume;
+    float mu;
+    float lambda;
+} avbd_tet;
+
+/* Triangle membrane with StVK energy (energy.hpp:88-121). */
+typedef struct avbd_tri {
+    int32_t v[3];
+    int32_t serial;
+    float dm_inv[4];          /* row-major 2x2 */
+    float rest_area;
+    float mu;
+    float lambda;
+} avbd_tri;
+
+/* A whole world: pointers to caller-owned arrays.  For avbd_upload they are
+ * read; for avbd_download they are written (sized from avbd_get_counts). */
+typedef struct avbd_world {
+    int32_t n_scenes;
+    int32_t n_bodies;
+    int32_t n_joints;
+    int32_t n_springs;
+    int32_t n_manifolds;
+    int32_t n_tets;
+    int32_t n_tris;
+    avbd_scene* scenes;
+    avbd_body* bodies;
+    avbd_joint* joints;
+    avbd_spring* springs;
+    avbd_manifold* manifolds;
+    avbd_tet* tets;
+    avbd_tri* tris;
+} avbd_world;
+
+/* Per-phase wall time of the last step, milliseconds (SolverTimings). */
+typedef struct avbd_timings {
+    float broad_phase_ms;
+    float warmstart_ms;
+    float prediction_ms;
+    float primal_dual_ms;
+    float velocity_update_ms;
+    float post_stab_ms;
+    float total_ms;
+    int32_t n_colours;        /* max colour count over scenes, last step */
+} avbd_timings;
+
+typedef struct avbd_ctx avbd_ctx;
+
+/* Library identification; returns AVBD_ABI_VERSION. */
+int avbd_abi_version(void);
+/* sizeof of each record, for bindings to check their layouts. */
+int avbd_record_sizes(int32_t* out, int32_t n);
+
+/* Message describing the last failure on the calling thread. */
+const char* avbd_last_error(void);
+
+/* Create a context on HIP device `device`. */
+int avbd_create(int32_t device, avbd_ctx** out);
+int avbd_destroy(avbd_ctx* ctx);
+
+/* Replace the context's world with `w` (all counts may be zero except
+ * n_scenes).  Allocates device storage sized for the world plus headroom for
+ * contact manifolds (`manifold_capacity` manifolds; 0 picks a default). */
+int avbd_upload(avbd_ctx* ctx, const avbd_world* w, int64_t manifold_capacity);
+
+/* Advance every scene by `n_steps` steps (Solver::Step, n times). */
+int avbd_step(avbd_ctx* ctx, int32_t n_steps);
+
+/* Advance only the listed scenes by `n_steps`; others keep their state. */
+int avbd_step_scenes(avbd_ctx* ctx, const int32_t* scene_ids, int32_t n_ids, int32_t n_steps);
+
+/* Block until all work queued on the context's stream has finished. */
+int avbd_synchronize(avbd_ctx* ctx);
+
+/* Current entity counts (n_* fields of `out`; pointers untouched). */
+int avbd_get_counts(avbd_ctx* ctx, avbd_world* out);
+
+/* Copy the world back into caller arrays sized by avbd_get_counts. */
+int avbd_download(avbd_ctx* ctx, avbd_world* out);
+
+/* Per-scene status words: emergency_stop and n_removed for `n` scenes. */
+int avbd_get_status(avbd_ctx* ctx, int32_t* emergency_stop, int32_t* n_removed, int32_t n);
+
+int avbd_get_timings(avbd_ctx* ctx, avbd_timings* out);
+
+/* Per-body colour of the last step (-1 for static bodies). */
+int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
+
+/* Tuning knobs (no reference counterpart):
+ *   "lanes"  lanes cooperating on one body's block solve: 1, 4, 8, 16 or 32
+ *            (1 = the reference's sequential accumulation order). */
+int avbd_set_option(avbd_ctx* ctx, const char* name, int64_t value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AVBD_CAPI_H */

@@ -1,0 +1,135 @@
+"""ctypes binding of libavbd_hip.so (the C ABI in include/avbd_capi.h).
+
+There is no fallback: if the HIP library is missing or no gfx950 device is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import AvbdTimings, AvbdWorld, alloc_world, world_to_c
+from .world import World
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libavbd_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "avbd_capi.h")
+
+_lib = None
+
+
+class AvbdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"avbd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build the HIP extension first (__graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.avbd_abi_version.restype = C.c_int
+        L.avbd_record_sizes.argtypes = [C.POINTER(C.c_int32), C.c_int32]
+        L.avbd_last_error.restype = C.c_char_p
+        L.avbd_create.argtypes = [C.c_int32, C.POINTER(vp)]
+        L.avbd_destroy.argtypes = [vp]
+        L.avbd_upload.argtypes = [vp, C.POINTER(AvbdWorld), C.c_int64]
+        L.avbd_step.argtypes = [vp, C.c_int32]
+        L.avbd_step_scenes.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, C.c_int32]
+        L.avbd_synchronize.argtypes = [vp]
+        L.avbd_get_counts.argtypes = [vp, C.POINTER(AvbdWorld)]
+        L.avbd_download.argtypes = [vp, C.POINTER(AvbdWorld)]
+        L.avbd_get_status.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32]
+        L.avbd_get_timings.argtypes = [vp, C.POINTER(AvbdTimings)]
+        L.avbd_get_colours.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
+        L.avbd_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Function names declared in include/avbd_capi.h."""
+    import re
+
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(avbd_\w+)\s*\(", text, re.M)))
+
+
+def _check(rc):
+    if rc != 0:
+        raise AvbdError(rc, lib().avbd_last_error().decode(errors="replace"))
+
+
+class Context:
+    """A device-resident world on one GPU."""
+
+    def __init__(self, device: int = 0, lanes: int | None = None):
+        self.L = lib()
+        self.h = C.c_void_p()
+        _check(self.L.avbd_create(device, C.byref(self.h)))
+        if lanes is not None:
+            self.set_option("lanes", lanes)
+
+    def close(self):
+        if self.h:
+            self.L.avbd_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_option(self, name: str, value: int):
+        _check(self.L.avbd_set_option(self.h, name.encode(), int(value)))
+
+    def upload(self, world: World, manifold_capacity: int = 0):
+        keep, cw = world_to_c(world)
+        _check(self.L.avbd_upload(self.h, C.byref(cw), int(manifold_capacity)))
+
+    def step(self, n: int = 1):
+        _check(self.L.avbd_step(self.h, int(n)))
+
+    def step_scenes(self, scene_ids, n: int = 1):
+        ids = np.ascontiguousarray(scene_ids, np.int32)
+        _check(self.L.avbd_step_scenes(self.h, ids.ctypes.data_as(C.POINTER(C.c_int32)), len(ids), int(n)))
+
+    def synchronize(self):
+        _check(self.L.avbd_synchronize(self.h))
+
+    def download(self) -> World:
+        cnt = AvbdWorld()
+        _check(self.L.avbd_get_counts(self.h, C.byref(cnt)))
+        w, cw, keep = alloc_world(cnt)
+        _check(self.L.avbd_download(self.h, C.byref(cw)))
+        return w
+
+    def status(self, n_scenes: int):
+        em = np.zeros(n_scenes, np.int32)
+        rem = np.zeros(n_scenes, np.int32)
+        _check(self.L.avbd_get_status(self.h, em.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      rem.ctypes.data_as(C.POINTER(C.c_int32)), n_scenes))
+        return em, rem
+
+    def timings(self) -> dict:
+        t = AvbdTimings()
+        _check(self.L.avbd_get_timings(self.h, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in AvbdTimings._fields_}
+
+    def colours(self, n: int) -> np.ndarray:
+        out = np.zeros(n, np.int32)
+        _check(self.L.avbd_get_colours(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), n))
+        return out

@@ -1,0 +1,845 @@
+// avbd_host.hip -- context, world upload/download and step orchestration of
+// the device-resident AVBD solver; implements the C ABI of avbd_capi.h.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "avbd_kernels.h"
+#include "avbd_world.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(AVBD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+int grid_for(long long n, int block = 256, int max_blocks = 4096) {
+    long long b = (n + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > max_blocks) b = max_blocks;
+    return int(b);
+}
+
+}  // namespace
+
+struct avbd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    DevWorld w{};
+    ManifoldBuf mb[2]{};
+    int cur = 0;
+    int4* items = nullptr;
+    int n_items = 0;
+    void* cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+    int lanes = 8;
+    bool have_world = false;
+    // host mirrors of the static parts
+    std::vector<avbd_scene> scenes;
+    std::vector<int> sc_off;
+    std::vector<int32_t> host_colours;
+    hipEvent_t ev[8]{};
+    avbd_timings timings{};
+    int last_ncol = 0;
+
+    template <class T>
+    int alloc(T** p, size_t count) {
+        void* q = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return fail(AVBD_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        e = hipMemsetAsync(q, 0, bytes, stream);
+        if (e != hipSuccess) return fail(AVBD_E_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = static_cast<T*>(q);
+        return 0;
+    }
+    void release() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* p : allocs) hipFree(p);
+        allocs.clear();
+        w = DevWorld{};
+        mb[0] = mb[1] = ManifoldBuf{};
+        items = nullptr;
+        cub_tmp = nullptr;
+        have_world = false;
+    }
+};
+
+namespace {
+
+int alloc_manifolds(avbd_ctx* c, ManifoldBuf& m, int cap) {
+    int rc;
+    if ((rc = c->alloc(&m.a, cap))) return rc;
+    if ((rc = c->alloc(&m.b, cap))) return rc;
+    if ((rc = c->alloc(&m.serial, cap))) return rc;
+    if ((rc = c->alloc(&m.n, cap))) return rc;
+    if ((rc = c->alloc(&m.fric, cap))) return rc;
+    if ((rc = c->alloc(&m.hot, (size_t)cap * 8 * CONTACT_HOT_F4))) return rc;
+    if ((rc = c->alloc(&m.cold, (size_t)cap * 8 * CONTACT_COLD_F4))) return rc;
+    return 0;
+}
+
+template <class T>
+int upload_vec(avbd_ctx* c, T* dst, const std::vector<T>& v) {
+    if (v.empty()) return 0;
+    HIP_TRY(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+template <class T>
+int download_vec(avbd_ctx* c, std::vector<T>& v, const T* src, size_t n) {
+    v.resize(n);
+    if (n == 0) return 0;
+    HIP_TRY(hipMemcpyAsync(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+float4 f4(const float* p, float w) { return make_float4(p[0], p[1], p[2], w); }
+
+int scan_ints(avbd_ctx* c, const int* in, int* out, int n) {
+    size_t bytes = c->cub_bytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, in, out, n, c->stream));
+    return 0;
+}
+
+// Build the force incidence lists of the current manifold buffer, joints and
+// springs: count, scan, fill, order by creation.
+int build_incidence(avbd_ctx* c) {
+    DevWorld& w = c->w;
+    const int N = w.N;
+    const int Fcap = c->w.Mcap + w.J + w.SP;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(N + 1)), dim3(256), 0, c->stream, w.cnt_i, N + 1, 0);
+    hipLaunchKernelGGL(k_inc_count, dim3(grid_for(Fcap)), dim3(256), 0, c->stream, w);
+    int rc = scan_ints(c, w.cnt_i, w.f_off, N + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(N + 1)), dim3(256), 0, c->stream, w.cur_i, N + 1, 0);
+    hipLaunchKernelGGL(k_inc_fill, dim3(grid_for(Fcap)), dim3(256), 0, c->stream, w);
+    hipLaunchKernelGGL(k_inc_sort, dim3(grid_for(N)), dim3(256), 0, c->stream, w);
+    return 0;
+}
+
+int launch_primal(avbd_ctx* c, int colour, int iter, int poststab, int grid) {
+    const DevWorld& w = c->w;
+    switch (c->lanes) {
+        case 1: hipLaunchKernelGGL(k_primal<1>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
+        case 4: hipLaunchKernelGGL(k_primal<4>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
+        case 8: hipLaunchKernelGGL(k_primal<8>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
+        case 16: hipLaunchKernelGGL(k_primal<16>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
+        default: hipLaunchKernelGGL(k_primal<32>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
+    }
+    return 0;
+}
+
+int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
+    const int S = in->n_scenes, N = in->n_bodies, J = in->n_joints, SP = in->n_springs;
+    const int M = in->n_manifolds, T = in->n_tets, TR = in->n_tris;
+    if (S < 1) return fail(AVBD_E_ARG, "world needs at least one scene");
+    if (N < 0 || J < 0 || SP < 0 || M < 0 || T < 0 || TR < 0) return fail(AVBD_E_ARG, "negative count");
+    // bodies of a scene are contiguous, scenes in order
+    std::vector<int> off(S + 1, 0);
+    for (int i = 0; i < N; ++i) {
+        int s = in->bodies[i].scene;
+        if (s < 0 || s >= S) return fail(AVBD_E_ARG, "body scene out of range");
+        if (i > 0 && s < in->bodies[i - 1].scene) return fail(AVBD_E_ARG, "bodies must be grouped by scene in order");
+        off[s + 1]++;
+    }
+    for (int s = 0; s < S; ++s) off[s + 1] += off[s];
+    auto body_ok = [&](int b) { return b >= 0 && b < N; };
+    for (int i = 0; i < J; ++i) {
+        const avbd_joint& j = in->joints[i];
+        if (!body_ok(j.body_b) || (j.body_a != -1 && !body_ok(j.body_a))) return fail(AVBD_E_ARG, "joint body index");
+        if (j.body_a >= 0 && in->bodies[j.body_a].scene != in->bodies[j.body_b].scene)
+            return fail(AVBD_E_ARG, "joint links two scenes");
+    }
+    for (int i = 0; i < SP; ++i) {
+        const avbd_spring& s = in->springs[i];
+        if (!body_ok(s.body_b) || (s.body_a != -1 && !body_ok(s.body_a))) return fail(AVBD_E_ARG, "spring body index");
+        if (s.body_a >= 0 && in->bodies[s.body_a].scene != in->bodies[s.body_b].scene)
+            return fail(AVBD_E_ARG, "spring links two scenes");
+    }
+    for (int i = 0; i < M; ++i) {
+        const avbd_manifold& m = in->manifolds[i];
+        if (!body_ok(m.body_a) || !body_ok(m.body_b)) return fail(AVBD_E_ARG, "manifold body index");
+        if (in->bodies[m.body_a].scene != in->bodies[m.body_b].scene) return fail(AVBD_E_ARG, "manifold links two scenes");
+        if (m.n_contacts < 0 || m.n_contacts > 8) return fail(AVBD_E_ARG, "manifold contact count");
+    }
+    for (int i = 0; i < T; ++i)
+        for (int k = 0; k < 4; ++k)
+            if (!body_ok(in->tets[i].v[k])) return fail(AVBD_E_ARG, "tet vertex index");
+    for (int i = 0; i < TR; ++i)
+        for (int k = 0; k < 3; ++k)
+            if (!body_ok(in->tris[i].v[k])) return fail(AVBD_E_ARG, "triangle vertex index");
+
+    c->release();
+    int nonpart = 0;
+    for (int i = 0; i < N; ++i)
+        if (!(in->bodies[i].flags & AVBD_BODY_PARTICLE)) ++nonpart;
+    long long mcap = mcap_req > 0 ? mcap_req : (long long)8 * nonpart + (N - nonpart) / 2 + 1024;
+    mcap = std::max<long long>(mcap, (long long)M + 1024);
+    if (mcap * 8 >= (1ll << 29)) return fail(AVBD_E_CAPACITY, "manifold capacity too large");
+    const int Mcap = int(mcap);
+
+    DevWorld& w = c->w;
+    w.S = S; w.N = N; w.J = J; w.SP = SP; w.T = T; w.TR = TR; w.Mcap = Mcap;
+    int rc = 0;
+    avbd_scene* d_sc;
+    int* d_off;
+#define A(p, n) if ((rc = c->alloc(p, n))) return rc
+    A(&d_sc, S);
+    A(&w.st, (size_t)S * ST_WORDS);
+    A(&d_off, S + 1);
+    w.sc = d_sc;
+    w.sc_off = d_off;
+    A(&w.pos, N); A(&w.rot, N); A(&w.scale, N); A(&w.vel, N); A(&w.angvel, N); A(&w.pvel, N); A(&w.pangvel, N);
+    A(&w.inertia, N); A(&w.flags, N); A(&w.scene, N); A(&w.last_pos, N); A(&w.last_rot, N); A(&w.inert_pos, N);
+    A(&w.inert_rot, N); A(&w.diff0, N); A(&w.diff1, N);
+    if ((rc = alloc_manifolds(c, c->mb[0], Mcap))) return rc;
+    if ((rc = alloc_manifolds(c, c->mb[1], Mcap))) return rc;
+    c->cur = 0;
+    w.m = c->mb[0];
+    A(&w.j_a, J); A(&w.j_b, J); A(&w.j_serial, J); A(&w.j_flags, J); A(&w.j_ra, J); A(&w.j_rb, J); A(&w.j_rest, J);
+    A(&w.j_c0, (size_t)J * 6); A(&w.j_row, (size_t)J * 36);
+    A(&w.s_a, SP); A(&w.s_b, SP); A(&w.s_serial, SP); A(&w.s_flags, SP); A(&w.s_ra, SP); A(&w.s_rb, SP);
+    A(&w.s_row, (size_t)SP * 6);
+    A(&w.t_v, T); A(&w.t_d, (size_t)T * 3);
+    A(&w.r_v, TR); A(&w.r_d, (size_t)TR * 2);
+    w.f_cap = 16 * Mcap + 2 * (J + SP) + 1;
+    A(&w.f_off, N + 1);
+    A(&w.f_ent, w.f_cap);
+    int* e_off;
+    uint32_t* e_ent;
+    A(&e_off, N + 1);
+    A(&e_ent, (size_t)4 * T + 3 * TR + 1);
+    w.e_off = e_off;
+    w.e_ent = e_ent;
+    A(&w.colour, N); A(&w.col_cnt, AVBD_MAX_COLOURS + 1); A(&w.col_off, AVBD_MAX_COLOURS + 1); A(&w.col_list, N);
+    A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1); A(&w.seg_off, N + 1);
+    w.pair_cap = std::max(65536, 4 * N);
+    A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
+    A(&w.alive, Mcap + 1); A(&w.alive_off, Mcap + 1);
+    A(&w.fracture_bodies, (size_t)2 * (J + SP) + 2);
+    A(&w.g, G_WORDS);
+    A(&w.next_serial, 1);
+#undef A
+
+    // ---- host staging (records -> device SoA) ----
+    c->scenes.assign(in->scenes, in->scenes + S);
+    c->sc_off = off;
+    std::vector<int> st((size_t)S * ST_WORDS, 0);
+    for (int s = 0; s < S; ++s) {
+        st[s * ST_WORDS + ST_EMERGENCY] = in->scenes[s].emergency_stop;
+        st[s * ST_WORDS + ST_REMOVED] = in->scenes[s].n_removed;
+        st[s * ST_WORDS + ST_ACTIVE] = 1;
+        st[s * ST_WORDS + ST_DIRTY] = 1;
+    }
+    if ((rc = upload_vec(c, d_sc, c->scenes))) return rc;
+    if ((rc = upload_vec(c, w.st, st))) return rc;
+    if ((rc = upload_vec(c, d_off, off))) return rc;
+    {
+        std::vector<float4> pos(N), rot(N), scale(N), vel(N), av(N), pv(N), pav(N), inert(N);
+        std::vector<uint32_t> flags(N);
+        std::vector<int> scene(N);
+        for (int i = 0; i < N; ++i) {
+            const avbd_body& b = in->bodies[i];
+            pos[i] = f4(b.pos, 0.f);
+            rot[i] = make_float4(b.rot[0], b.rot[1], b.rot[2], b.rot[3]);
+            scale[i] = f4(b.scale, b.radius);
+            vel[i] = f4(b.vel, b.mass);
+            av[i] = f4(b.ang_vel, b.friction);
+            pv[i] = f4(b.prev_vel, 0.f);
+            pav[i] = f4(b.prev_ang_vel, 0.f);
+            inert[i] = f4(b.inertia, 0.f);
+            flags[i] = b.flags;
+            scene[i] = b.scene;
+        }
+        if ((rc = upload_vec(c, w.pos, pos)) || (rc = upload_vec(c, w.rot, rot)) || (rc = upload_vec(c, w.scale, scale)) ||
+            (rc = upload_vec(c, w.vel, vel)) || (rc = upload_vec(c, w.angvel, av)) || (rc = upload_vec(c, w.pvel, pv)) ||
+            (rc = upload_vec(c, w.pangvel, pav)) || (rc = upload_vec(c, w.inertia, inert)) ||
+            (rc = upload_vec(c, w.flags, flags)) || (rc = upload_vec(c, w.scene, scene)))
+            return rc;
+    }
+    long long max_serial = -1;
+    {
+        std::vector<int> a(J), b(J);
+        std::vector<long long> ser(J);
+        std::vector<uint32_t> fl(J);
+        std::vector<float4> ra(J), rb(J), rest(J);
+        std::vector<float> rows((size_t)J * 36);
+        for (int i = 0; i < J; ++i) {
+            const avbd_joint& j = in->joints[i];
+            a[i] = j.body_a; b[i] = j.body_b; ser[i] = j.serial; fl[i] = j.flags & AVBD_FORCE_DISABLED;
+            max_serial = std::max<long long>(max_serial, j.serial);
+            ra[i] = f4(j.r_a, j.torque_arm);
+            rb[i] = f4(j.r_b, 0.f);
+            rest[i] = make_float4(j.rest_rot[0], j.rest_rot[1], j.rest_rot[2], j.rest_rot[3]);
+            for (int r = 0; r < 6; ++r) {
+                float* row = &rows[(size_t)(i * 6 + r) * 6];
+                row[0] = j.stiffness[r]; row[1] = j.fracture[r]; row[2] = j.fmin[r]; row[3] = j.fmax[r];
+                row[4] = j.lambda[r]; row[5] = j.penalty[r];
+            }
+        }
+        if ((rc = upload_vec(c, w.j_a, a)) || (rc = upload_vec(c, w.j_b, b)) || (rc = upload_vec(c, w.j_serial, ser)) ||
+            (rc = upload_vec(c, w.j_flags, fl)) || (rc = upload_vec(c, w.j_ra, ra)) || (rc = upload_vec(c, w.j_rb, rb)) ||
+            (rc = upload_vec(c, w.j_rest, rest)) || (rc = upload_vec(c, w.j_row, rows)))
+            return rc;
+    }
+    {
+        std::vector<int> a(SP), b(SP);
+        std::vector<long long> ser(SP);
+        std::vector<uint32_t> fl(SP);
+        std::vector<float4> ra(SP), rb(SP);
+        std::vector<float> rows((size_t)SP * 6);
+        for (int i = 0; i < SP; ++i) {
+            const avbd_spring& s = in->springs[i];
+            a[i] = s.body_a; b[i] = s.body_b; ser[i] = s.serial; fl[i] = s.flags & AVBD_FORCE_DISABLED;
+            max_serial = std::max<long long>(max_serial, s.serial);
+            ra[i] = f4(s.r_a, s.rest_length);
+            rb[i] = f4(s.r_b, 0.f);
+            float* row = &rows[(size_t)i * 6];
+            row[0] = s.stiffness; row[1] = s.fracture; row[2] = s.fmin; row[3] = s.fmax; row[4] = s.lambda; row[5] = s.penalty;
+        }
+        if ((rc = upload_vec(c, w.s_a, a)) || (rc = upload_vec(c, w.s_b, b)) || (rc = upload_vec(c, w.s_serial, ser)) ||
+            (rc = upload_vec(c, w.s_flags, fl)) || (rc = upload_vec(c, w.s_ra, ra)) || (rc = upload_vec(c, w.s_rb, rb)) ||
+            (rc = upload_vec(c, w.s_row, rows)))
+            return rc;
+    }
+    {
+        // manifolds must be in creation order
+        std::vector<int> order(M);
+        for (int i = 0; i < M; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return in->manifolds[x].serial < in->manifolds[y].serial; });
+        std::vector<int> a(M), b(M), n(M);
+        std::vector<long long> ser(M);
+        std::vector<float> fr(M);
+        std::vector<float4> hot((size_t)M * 8 * CONTACT_HOT_F4, make_float4(0, 0, 0, 0));
+        std::vector<float4> cold((size_t)M * 8 * CONTACT_COLD_F4, make_float4(0, 0, 0, 0));
+        for (int k = 0; k < M; ++k) {
+            const avbd_manifold& m = in->manifolds[order[k]];
+            a[k] = m.body_a; b[k] = m.body_b; n[k] = m.n_contacts; ser[k] = m.serial; fr[k] = m.friction;
+            max_serial = std::max<long long>(max_serial, m.serial);
+            for (int ci = 0; ci < m.n_contacts; ++ci) {
+                const avbd_contact& ct = m.contacts[ci];
+                float4* h = &hot[((size_t)k * 8 + ci) * CONTACT_HOT_F4];
+                h[3].w = ct.lambda[0];
+                h[4].w = ct.lambda[1];
+                h[5] = make_float4(ct.lambda[2], ct.penalty[0], ct.penalty[1], ct.penalty[2]);
+                float4* cd = &cold[((size_t)k * 8 + ci) * CONTACT_COLD_F4];
+                uint32_t id = ct.id;
+                float idf;
+                std::memcpy(&idf, &id, 4);
+                cd[0] = make_float4(idf, (float)(ct.stick != 0), ct.r_a[0], ct.r_a[1]);
+                cd[1] = make_float4(ct.r_a[2], ct.r_b[0], ct.r_b[1], ct.r_b[2]);
+                cd[2] = f4(ct.normal, ct.penetration);
+            }
+        }
+        ManifoldBuf& m0 = c->mb[0];
+        if ((rc = upload_vec(c, m0.a, a)) || (rc = upload_vec(c, m0.b, b)) || (rc = upload_vec(c, m0.n, n)) ||
+            (rc = upload_vec(c, m0.serial, ser)) || (rc = upload_vec(c, m0.fric, fr)) || (rc = upload_vec(c, m0.hot, hot)) ||
+            (rc = upload_vec(c, m0.cold, cold)))
+            return rc;
+    }
+    {
+        // energy incidence: per vertex, energies in creation (serial) order
+        struct ERef { int serial, kind, idx; };
+        std::vector<ERef> refs;
+        refs.reserve(T + TR);
+        for (int i = 0; i < T; ++i) refs.push_back({in->tets[i].serial, 0, i});
+        for (int i = 0; i < TR; ++i) refs.push_back({in->tris[i].serial, 1, i});
+        std::stable_sort(refs.begin(), refs.end(), [](const ERef& x, const ERef& y) { return x.serial < y.serial; });
+        std::vector<int> cnt(N + 1, 0);
+        for (const ERef& r : refs) {
+            if (r.kind == 0) for (int k = 0; k < 4; ++k) cnt[in->tets[r.idx].v[k]]++;
+            else for (int k = 0; k < 3; ++k) cnt[in->tris[r.idx].v[k]]++;
+        }
+        std::vector<int> eoff(N + 1, 0);
+        for (int i = 0; i < N; ++i) eoff[i + 1] = eoff[i] + cnt[i];
+        std::vector<uint32_t> ent(std::max(1, eoff[N]));
+        std::vector<int> cur(eoff.begin(), eoff.end() - 1);
+        for (const ERef& r : refs) {
+            if (r.kind == 0) {
+                for (int k = 0; k < 4; ++k) ent[cur[in->tets[r.idx].v[k]]++] = MAKE_EENT(0, k, r.idx);
+            } else {
+                for (int k = 0; k < 3; ++k) ent[cur[in->tris[r.idx].v[k]]++] = MAKE_EENT(1, k, r.idx);
+            }
+        }
+        std::vector<int4> tv(T), rv(TR);
+        std::vector<float4> td((size_t)T * 3), rd((size_t)TR * 2);
+        for (int i = 0; i < T; ++i) {
+            const avbd_tet& t = in->tets[i];
+            tv[i] = make_int4(t.v[0], t.v[1], t.v[2], t.v[3]);
+            const float* d = t.dm_inv;
+            td[i * 3 + 0] = make_float4(d[0], d[1], d[2], d[3]);
+            td[i * 3 + 1] = make_float4(d[4], d[5], d[6], d[7]);
+            td[i * 3 + 2] = make_float4(d[8], t.rest_volume, t.mu, t.lambda);
+        }
+        for (int i = 0; i < TR; ++i) {
+            const avbd_tri& t = in->tris[i];
+            rv[i] = make_int4(t.v[0], t.v[1], t.v[2], 0);
+            rd[i * 2 + 0] = make_float4(t.dm_inv[0], t.dm_inv[1], t.dm_inv[2], t.dm_inv[3]);
+            rd[i * 2 + 1] = make_float4(t.rest_area, t.mu, t.lambda, 0.f);
+        }
+        if ((rc = upload_vec(c, e_off, eoff)) || (rc = upload_vec(c, e_ent, ent)) || (rc = upload_vec(c, w.t_v, tv)) ||
+            (rc = upload_vec(c, w.t_d, td)) || (rc = upload_vec(c, w.r_v, rv)) || (rc = upload_vec(c, w.r_d, rd)))
+            return rc;
+    }
+    {
+        std::vector<int> g(G_WORDS, 0);
+        g[G_MANIFOLDS] = M;
+        if ((rc = upload_vec(c, w.g, g))) return rc;
+        std::vector<long long> ns(1, max_serial + 1);
+        if ((rc = upload_vec(c, w.next_serial, ns))) return rc;
+    }
+    // broad-phase work items: (scene, i-tile, j-tile >= i-tile)
+    {
+        std::vector<int4> items;
+        for (int s = 0; s < S; ++s) {
+            int n = off[s + 1] - off[s];
+            int tiles = (n + 255) / 256;
+            for (int ti = 0; ti < tiles; ++ti)
+                for (int tj = ti; tj < tiles; ++tj) items.push_back(make_int4(s, ti, tj, 0));
+        }
+        c->n_items = int(items.size());
+        if ((rc = c->alloc(&c->items, std::max<size_t>(items.size(), 1)))) return rc;
+        if ((rc = upload_vec(c, c->items, items))) return rc;
+    }
+    // scan temp storage sized for the largest scan
+    {
+        size_t b1 = 0, b2 = 0;
+        int* dummy = nullptr;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, dummy, dummy, N + 1, c->stream));
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, dummy, dummy, Mcap + 1, c->stream));
+        c->cub_bytes = std::max(b1, b2);
+        char* tmp;
+        if ((rc = c->alloc(&tmp, c->cub_bytes))) return rc;
+        c->cub_tmp = tmp;
+    }
+    c->have_world = true;
+    if ((rc = build_incidence(c))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int check_errors(avbd_ctx* c, const int* g) {
+    (void)c;
+    if (g[G_ERR_MANIFOLD_CAP]) return fail(AVBD_E_CAPACITY, "manifold capacity exceeded (raise manifold_capacity)");
+    if (g[G_ERR_PAIR_CAP]) return fail(AVBD_E_CAPACITY, "new-pair buffer exceeded in the broad phase");
+    if (g[G_ERR_COLOURS]) return fail(AVBD_E_CAPACITY, "more than 256 colours needed");
+    return 0;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+int do_step(avbd_ctx* c) {
+    DevWorld& w = c->w;
+    hipStream_t st = c->stream;
+    const int N = w.N;
+    int max_iters = 0;
+    bool any_post = false;
+    for (const avbd_scene& s : c->scenes) {
+        max_iters = std::max(max_iters, s.iterations);
+        any_post = any_post || s.post_stabilization != 0;
+    }
+    HIP_TRY(hipEventRecord(c->ev[0], st));
+    // ---- 1. broad phase
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_NEW_PAIRS, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(N + 1)), dim3(256), 0, st, w.cnt_i, N + 1, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(N + 1)), dim3(256), 0, st, w.cur_i, N + 1, 0);
+    if (c->n_items > 0) hipLaunchKernelGGL(k_broad, dim3(c->n_items), dim3(256), 0, st, w, (const int4*)c->items);
+    int rc = scan_ints(c, w.cnt_i, w.seg_off, N + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pair_scatter, dim3(grid_for(w.pair_cap)), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_pair_rank_append, dim3(grid_for(w.pair_cap)), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_pair_finish, dim3(1), dim3(1), 0, st, w);
+    HIP_TRY(hipEventRecord(c->ev[1], st));
+    // ---- 2. force warm start, compaction, incidence, colouring
+    hipLaunchKernelGGL(k_manifold_init, dim3(grid_for(w.Mcap, 128, 8192)), dim3(128), 0, st, w);
+    if (w.J + w.SP > 0) hipLaunchKernelGGL(k_force_init, dim3(grid_for(w.J + w.SP)), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.alive + w.Mcap, 0);
+    if ((rc = scan_ints(c, w.alive, w.alive_off, w.Mcap + 1))) return rc;
+    ManifoldBuf& dst = c->mb[c->cur ^ 1];
+    hipLaunchKernelGGL(k_compact, dim3(grid_for(w.Mcap)), dim3(256), 0, st, w, dst);
+    hipLaunchKernelGGL(k_compact_finish, dim3(1), dim3(1), 0, st, w);
+    c->cur ^= 1;
+    w.m = c->mb[c->cur];
+    if ((rc = build_incidence(c))) return rc;
+    hipLaunchKernelGGL(k_colour, dim3(w.S), dim3(1024), 0, st, w);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, w.col_cnt, AVBD_MAX_COLOURS + 1, 0);
+    hipLaunchKernelGGL(k_colour_hist, dim3(grid_for(N)), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_colour_scan, dim3(1), dim3(AVBD_MAX_COLOURS), 0, st, w);
+    hipLaunchKernelGGL(k_colour_fill, dim3(grid_for(N)), dim3(256), 0, st, w);
+    int g[G_WORDS];
+    HIP_TRY(hipMemcpyAsync(g, w.g, sizeof(g), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    if ((rc = check_errors(c, g))) return rc;
+    const int ncol = g[G_NCOLOURS];
+    c->last_ncol = ncol;
+    HIP_TRY(hipEventRecord(c->ev[2], st));
+    // ---- 4. prediction
+    hipLaunchKernelGGL(k_predict, dim3(grid_for(N)), dim3(256), 0, st, w);
+    HIP_TRY(hipEventRecord(c->ev[3], st));
+    // ---- 5. iterations
+    const int pgrid = grid_for((long long)N * c->lanes, 256, 2048);
+    const int dgrid = grid_for((long long)w.Mcap * 8 + w.J + w.SP, 256, 4096);
+    for (int it = 0; it < max_iters; ++it) {
+        for (int col = 0; col < ncol; ++col) launch_primal(c, col, it, 0, pgrid);
+        hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
+        if (w.J + w.SP > 0) {
+            hipLaunchKernelGGL(k_fracture_reset, dim3(grid_for(2 * (w.J + w.SP) + 2)), dim3(256), 0, st, w);
+            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_FRACTURES, 0);
+        }
+    }
+    HIP_TRY(hipEventRecord(c->ev[4], st));
+    // ---- 6. velocities
+    hipLaunchKernelGGL(k_velocity, dim3(grid_for(N)), dim3(256), 0, st, w);
+    HIP_TRY(hipEventRecord(c->ev[5], st));
+    // ---- 7. post-stabilisation
+    if (any_post) {
+        for (int col = 0; col < ncol; ++col) launch_primal(c, col, -1, 1, pgrid);
+        hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, st, w);
+    }
+    HIP_TRY(hipEventRecord(c->ev[6], st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int avbd_abi_version(void) { return AVBD_ABI_VERSION; }
+
+int avbd_record_sizes(int32_t* out, int32_t n) {
+    const int32_t sizes[] = {(int32_t)sizeof(avbd_scene),   (int32_t)sizeof(avbd_body),     (int32_t)sizeof(avbd_joint),
+                             (int32_t)sizeof(avbd_spring),  (int32_t)sizeof(avbd_contact),  (int32_t)sizeof(avbd_manifold),
+                             (int32_t)sizeof(avbd_tet),     (int32_t)sizeof(avbd_tri),      (int32_t)sizeof(avbd_world),
+                             (int32_t)sizeof(avbd_timings)};
+    int k = (int)(sizeof(sizes) / sizeof(sizes[0]));
+    for (int i = 0; i < n && i < k; ++i) out[i] = sizes[i];
+    return k;
+}
+
+const char* avbd_last_error(void) { return g_last_error.c_str(); }
+
+int avbd_create(int32_t device, avbd_ctx** out) {
+    if (!out) return fail(AVBD_E_ARG, "null output pointer");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(AVBD_E_NOGPU, "no HIP device");
+    if (device < 0 || device >= count) return fail(AVBD_E_ARG, "device index out of range");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+        return fail(AVBD_E_NOGPU, std::string("library built for gfx950, device is ") + prop.gcnArchName);
+    HIP_TRY(hipSetDevice(device));
+    avbd_ctx* c = new avbd_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(AVBD_E_HIP, "hipStreamCreate failed");
+    }
+    for (auto& e : c->ev) hipEventCreate(&e);
+    *out = c;
+    return 0;
+}
+
+int avbd_destroy(avbd_ctx* c) {
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    c->release();
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
+    if (!c || !name) return fail(AVBD_E_ARG, "null argument");
+    std::string n(name);
+    if (n == "lanes") {
+        if (value != 1 && value != 4 && value != 8 && value != 16 && value != 32)
+            return fail(AVBD_E_ARG, "lanes must be 1, 4, 8, 16 or 32");
+        c->lanes = int(value);
+        return 0;
+    }
+    return fail(AVBD_E_ARG, "unknown option " + n);
+}
+
+int avbd_upload(avbd_ctx* c, const avbd_world* w, int64_t manifold_capacity) {
+    if (!c || !w) return fail(AVBD_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    return do_upload(c, w, manifold_capacity);
+}
+
+static int step_impl(avbd_ctx* c, int32_t n_steps) {
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    if (n_steps < 0) return fail(AVBD_E_ARG, "negative step count");
+    HIP_TRY(hipSetDevice(c->device));
+    avbd_timings acc{};
+    for (int k = 0; k < n_steps; ++k) {
+        int rc = do_step(c);
+        if (rc) return rc;
+        int g[G_WORDS];
+        HIP_TRY(hipMemcpyAsync(g, c->w.g, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (g[G_ANY_OOB]) {
+            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, c->stream, c->w.g + G_ANY_OOB, 0);
+            hipStreamSynchronize(c->stream);
+            return fail(AVBD_E_UNSUPPORTED, "a body left the world bounds; out-of-bounds removal is not supported on the device yet");
+        }
+        acc.broad_phase_ms = elapsed(c->ev[0], c->ev[1]);
+        acc.warmstart_ms = elapsed(c->ev[1], c->ev[2]);
+        acc.prediction_ms = elapsed(c->ev[2], c->ev[3]);
+        acc.primal_dual_ms = elapsed(c->ev[3], c->ev[4]);
+        acc.velocity_update_ms = elapsed(c->ev[4], c->ev[5]);
+        acc.post_stab_ms = elapsed(c->ev[5], c->ev[6]);
+        acc.total_ms = elapsed(c->ev[0], c->ev[6]);
+        acc.n_colours = c->last_ncol;
+    }
+    if (n_steps > 0) c->timings = acc;
+    return 0;
+}
+
+int avbd_step(avbd_ctx* c, int32_t n_steps) {
+    if (!c) return fail(AVBD_E_ARG, "null context");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    std::vector<int> st((size_t)c->w.S * ST_WORDS);
+    HIP_TRY(hipMemcpy(st.data(), c->w.st, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    bool changed = false;
+    for (int s = 0; s < c->w.S; ++s)
+        if (!st[s * ST_WORDS + ST_ACTIVE]) { st[s * ST_WORDS + ST_ACTIVE] = 1; changed = true; }
+    if (changed) HIP_TRY(hipMemcpy(c->w.st, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    return step_impl(c, n_steps);
+}
+
+int avbd_step_scenes(avbd_ctx* c, const int32_t* ids, int32_t n_ids, int32_t n_steps) {
+    if (!c || (!ids && n_ids > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    std::vector<int> st((size_t)c->w.S * ST_WORDS);
+    HIP_TRY(hipMemcpy(st.data(), c->w.st, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int s = 0; s < c->w.S; ++s) st[s * ST_WORDS + ST_ACTIVE] = 0;
+    for (int i = 0; i < n_ids; ++i) {
+        if (ids[i] < 0 || ids[i] >= c->w.S) return fail(AVBD_E_ARG, "scene id out of range");
+        st[ids[i] * ST_WORDS + ST_ACTIVE] = 1;
+    }
+    HIP_TRY(hipMemcpy(c->w.st, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    return step_impl(c, n_steps);
+}
+
+int avbd_synchronize(avbd_ctx* c) {
+    if (!c) return fail(AVBD_E_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int avbd_get_counts(avbd_ctx* c, avbd_world* out) {
+    if (!c || !out) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    const DevWorld& w = c->w;
+    int g[G_WORDS];
+    HIP_TRY(hipMemcpyAsync(g, w.g, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> jf, sf;
+    int rc;
+    if ((rc = download_vec(c, jf, w.j_flags, w.J)) || (rc = download_vec(c, sf, w.s_flags, w.SP))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out->n_scenes = w.S;
+    out->n_bodies = w.N;
+    out->n_joints = (int)std::count_if(jf.begin(), jf.end(), [](uint32_t f) { return !(f & FORCE_DEAD); });
+    out->n_springs = (int)std::count_if(sf.begin(), sf.end(), [](uint32_t f) { return !(f & FORCE_DEAD); });
+    out->n_manifolds = g[G_MANIFOLDS];
+    out->n_tets = w.T;
+    out->n_tris = w.TR;
+    return 0;
+}
+
+int avbd_download(avbd_ctx* c, avbd_world* out) {
+    if (!c || !out) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    avbd_world cnt{};
+    int rc = avbd_get_counts(c, &cnt);
+    if (rc) return rc;
+    if (out->n_scenes < cnt.n_scenes || out->n_bodies < cnt.n_bodies || out->n_joints < cnt.n_joints ||
+        out->n_springs < cnt.n_springs || out->n_manifolds < cnt.n_manifolds || out->n_tets < cnt.n_tets ||
+        out->n_tris < cnt.n_tris)
+        return fail(AVBD_E_ARG, "output arrays smaller than avbd_get_counts");
+    const DevWorld& w = c->w;
+    const int N = w.N, J = w.J, SP = w.SP, M = cnt.n_manifolds;
+    std::vector<float4> pos, rot, scale, vel, av, pv, pav, inert;
+    std::vector<uint32_t> flags, jf, sf;
+    std::vector<int> scene, st, ja, jb, sa, sb, ma, mbv, mn;
+    std::vector<long long> jser, sser, mser;
+    std::vector<float4> jra, jrb, jrest, sra, srb, hot, cold;
+    std::vector<float> jrow, srow, mfr;
+    if ((rc = download_vec(c, pos, w.pos, N)) || (rc = download_vec(c, rot, w.rot, N)) ||
+        (rc = download_vec(c, scale, w.scale, N)) || (rc = download_vec(c, vel, w.vel, N)) ||
+        (rc = download_vec(c, av, w.angvel, N)) || (rc = download_vec(c, pv, w.pvel, N)) ||
+        (rc = download_vec(c, pav, w.pangvel, N)) || (rc = download_vec(c, inert, w.inertia, N)) ||
+        (rc = download_vec(c, flags, w.flags, N)) || (rc = download_vec(c, scene, w.scene, N)) ||
+        (rc = download_vec(c, st, w.st, (size_t)w.S * ST_WORDS)) || (rc = download_vec(c, ja, w.j_a, J)) ||
+        (rc = download_vec(c, jb, w.j_b, J)) || (rc = download_vec(c, jser, w.j_serial, J)) ||
+        (rc = download_vec(c, jf, w.j_flags, J)) || (rc = download_vec(c, jra, w.j_ra, J)) ||
+        (rc = download_vec(c, jrb, w.j_rb, J)) || (rc = download_vec(c, jrest, w.j_rest, J)) ||
+        (rc = download_vec(c, jrow, w.j_row, (size_t)J * 36)) || (rc = download_vec(c, sa, w.s_a, SP)) ||
+        (rc = download_vec(c, sb, w.s_b, SP)) || (rc = download_vec(c, sser, w.s_serial, SP)) ||
+        (rc = download_vec(c, sf, w.s_flags, SP)) || (rc = download_vec(c, sra, w.s_ra, SP)) ||
+        (rc = download_vec(c, srb, w.s_rb, SP)) || (rc = download_vec(c, srow, w.s_row, (size_t)SP * 6)) ||
+        (rc = download_vec(c, ma, w.m.a, M)) || (rc = download_vec(c, mbv, w.m.b, M)) ||
+        (rc = download_vec(c, mser, w.m.serial, M)) || (rc = download_vec(c, mn, w.m.n, M)) ||
+        (rc = download_vec(c, mfr, w.m.fric, M)) ||
+        (rc = download_vec(c, hot, w.m.hot, (size_t)M * 8 * CONTACT_HOT_F4)) ||
+        (rc = download_vec(c, cold, w.m.cold, (size_t)M * 8 * CONTACT_COLD_F4)))
+        return rc;
+    std::vector<int4> tv, rv;
+    std::vector<float4> td, rd;
+    if ((rc = download_vec(c, tv, w.t_v, w.T)) || (rc = download_vec(c, td, w.t_d, (size_t)w.T * 3)) ||
+        (rc = download_vec(c, rv, w.r_v, w.TR)) || (rc = download_vec(c, rd, w.r_d, (size_t)w.TR * 2)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < w.S; ++s) {
+        out->scenes[s] = c->scenes[s];
+        out->scenes[s].emergency_stop = st[s * ST_WORDS + ST_EMERGENCY];
+        out->scenes[s].n_removed = st[s * ST_WORDS + ST_REMOVED];
+    }
+    auto put3 = [](float* d, float4 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; };
+    for (int i = 0; i < N; ++i) {
+        avbd_body& b = out->bodies[i];
+        std::memset(&b, 0, sizeof(b));
+        put3(b.pos, pos[i]);
+        b.rot[0] = rot[i].x; b.rot[1] = rot[i].y; b.rot[2] = rot[i].z; b.rot[3] = rot[i].w;
+        put3(b.scale, scale[i]);
+        b.radius = scale[i].w;
+        put3(b.vel, vel[i]);
+        b.mass = vel[i].w;
+        put3(b.ang_vel, av[i]);
+        b.friction = av[i].w;
+        put3(b.prev_vel, pv[i]);
+        put3(b.prev_ang_vel, pav[i]);
+        put3(b.inertia, inert[i]);
+        b.flags = flags[i];
+        b.scene = scene[i];
+    }
+    int k = 0;
+    for (int i = 0; i < J; ++i) {
+        if (jf[i] & FORCE_DEAD) continue;
+        avbd_joint& j = out->joints[k++];
+        std::memset(&j, 0, sizeof(j));
+        j.body_a = ja[i]; j.body_b = jb[i]; j.serial = jser[i]; j.flags = jf[i] & AVBD_FORCE_DISABLED;
+        put3(j.r_a, jra[i]); j.torque_arm = jra[i].w;
+        put3(j.r_b, jrb[i]);
+        j.rest_rot[0] = jrest[i].x; j.rest_rot[1] = jrest[i].y; j.rest_rot[2] = jrest[i].z; j.rest_rot[3] = jrest[i].w;
+        for (int r = 0; r < 6; ++r) {
+            const float* row = &jrow[(size_t)(i * 6 + r) * 6];
+            j.stiffness[r] = row[0]; j.fracture[r] = row[1]; j.fmin[r] = row[2]; j.fmax[r] = row[3];
+            j.lambda[r] = row[4]; j.penalty[r] = row[5];
+        }
+    }
+    k = 0;
+    for (int i = 0; i < SP; ++i) {
+        if (sf[i] & FORCE_DEAD) continue;
+        avbd_spring& s = out->springs[k++];
+        std::memset(&s, 0, sizeof(s));
+        s.body_a = sa[i]; s.body_b = sb[i]; s.serial = sser[i]; s.flags = sf[i] & AVBD_FORCE_DISABLED;
+        put3(s.r_a, sra[i]); s.rest_length = sra[i].w;
+        put3(s.r_b, srb[i]);
+        const float* row = &srow[(size_t)i * 6];
+        s.stiffness = row[0]; s.fracture = row[1]; s.fmin = row[2]; s.fmax = row[3]; s.lambda = row[4]; s.penalty = row[5];
+    }
+    for (int i = 0; i < M; ++i) {
+        avbd_manifold& m = out->manifolds[i];
+        std::memset(&m, 0, sizeof(m));
+        m.body_a = ma[i]; m.body_b = mbv[i]; m.serial = mser[i]; m.n_contacts = mn[i]; m.friction = mfr[i];
+        for (int ci = 0; ci < mn[i]; ++ci) {
+            const float4* h = &hot[((size_t)i * 8 + ci) * CONTACT_HOT_F4];
+            const float4* cd = &cold[((size_t)i * 8 + ci) * CONTACT_COLD_F4];
+            avbd_contact& ct = m.contacts[ci];
+            std::memcpy(&ct.id, &cd[0].x, 4);
+            ct.stick = cd[0].y != 0.f;
+            ct.r_a[0] = cd[0].z; ct.r_a[1] = cd[0].w; ct.r_a[2] = cd[1].x;
+            ct.r_b[0] = cd[1].y; ct.r_b[1] = cd[1].z; ct.r_b[2] = cd[1].w;
+            put3(ct.normal, cd[2]);
+            ct.penetration = cd[2].w;
+            ct.lambda[0] = h[3].w; ct.lambda[1] = h[4].w; ct.lambda[2] = h[5].x;
+            ct.penalty[0] = h[5].y; ct.penalty[1] = h[5].z; ct.penalty[2] = h[5].w;
+        }
+    }
+    for (int i = 0; i < w.T; ++i) {
+        avbd_tet& t = out->tets[i];
+        t.v[0] = tv[i].x; t.v[1] = tv[i].y; t.v[2] = tv[i].z; t.v[3] = tv[i].w;
+        const float d[12] = {td[i * 3].x, td[i * 3].y, td[i * 3].z, td[i * 3].w, td[i * 3 + 1].x, td[i * 3 + 1].y,
+                             td[i * 3 + 1].z, td[i * 3 + 1].w, td[i * 3 + 2].x, td[i * 3 + 2].y, td[i * 3 + 2].z,
+                             td[i * 3 + 2].w};
+        for (int q = 0; q < 9; ++q) t.dm_inv[q] = d[q];
+        t.rest_volume = d[9]; t.mu = d[10]; t.lambda = d[11];
+        t.serial = i;
+    }
+    for (int i = 0; i < w.TR; ++i) {
+        avbd_tri& t = out->tris[i];
+        t.v[0] = rv[i].x; t.v[1] = rv[i].y; t.v[2] = rv[i].z;
+        t.dm_inv[0] = rd[i * 2].x; t.dm_inv[1] = rd[i * 2].y; t.dm_inv[2] = rd[i * 2].z; t.dm_inv[3] = rd[i * 2].w;
+        t.rest_area = rd[i * 2 + 1].x; t.mu = rd[i * 2 + 1].y; t.lambda = rd[i * 2 + 1].z;
+        t.serial = w.T + i;
+    }
+    out->n_scenes = cnt.n_scenes;
+    out->n_bodies = cnt.n_bodies;
+    out->n_joints = cnt.n_joints;
+    out->n_springs = cnt.n_springs;
+    out->n_manifolds = cnt.n_manifolds;
+    out->n_tets = cnt.n_tets;
+    out->n_tris = cnt.n_tris;
+    return 0;
+}
+
+int avbd_get_status(avbd_ctx* c, int32_t* emergency, int32_t* removed, int32_t n) {
+    if (!c) return fail(AVBD_E_ARG, "null context");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    std::vector<int> st((size_t)c->w.S * ST_WORDS);
+    HIP_TRY(hipMemcpy(st.data(), c->w.st, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int s = 0; s < n && s < c->w.S; ++s) {
+        if (emergency) emergency[s] = st[s * ST_WORDS + ST_EMERGENCY];
+        if (removed) removed[s] = st[s * ST_WORDS + ST_REMOVED];
+    }
+    return 0;
+}
+
+int avbd_get_timings(avbd_ctx* c, avbd_timings* out) {
+    if (!c || !out) return fail(AVBD_E_ARG, "null argument");
+    *out = c->timings;
+    return 0;
+}
+
+int avbd_get_colours(avbd_ctx* c, int32_t* out, int32_t n) {
+    if (!c || !out) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    std::vector<int> col;
+    int rc = download_vec(c, col, c->w.colour, c->w.N);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n && i < c->w.N; ++i) out[i] = col[i];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,29 @@
+// avbd_kernels.h -- kernel declarations (definitions in avbd_kernels.hip).
+#pragma once
+
+#include "avbd_world.h"
+
+__global__ void k_fill_int(int* p, int n, int v);
+__global__ void k_set_word(int* p, int v);
+__global__ void k_broad(DevWorld w, const int4* items);
+__global__ void k_pair_scatter(DevWorld w);
+__global__ void k_pair_rank_append(DevWorld w);
+__global__ void k_pair_finish(DevWorld w);
+__global__ void k_manifold_init(DevWorld w);
+__global__ void k_force_init(DevWorld w);
+__global__ void k_compact(DevWorld w, ManifoldBuf dst);
+__global__ void k_compact_finish(DevWorld w);
+__global__ void k_inc_count(DevWorld w);
+__global__ void k_inc_fill(DevWorld w);
+__global__ void k_inc_sort(DevWorld w);
+__global__ void k_colour(DevWorld w);
+__global__ void k_colour_hist(DevWorld w);
+__global__ void k_colour_scan(DevWorld w);
+__global__ void k_colour_fill(DevWorld w);
+__global__ void k_predict(DevWorld w);
+template <int G>
+__global__ void k_primal(DevWorld w, int c, int iter, int poststab);
+__global__ void k_dual(DevWorld w, int iter);
+__global__ void k_fracture_reset(DevWorld w);
+__global__ void k_velocity(DevWorld w);
+__global__ void k_stick(DevWorld w);

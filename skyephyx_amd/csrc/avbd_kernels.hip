@@ -1,0 +1,1204 @@
+// avbd_kernels.hip -- HIP kernels of the device-resident AVBD step (gfx950).
+//
+// Step pipeline (Solver::Step, solver.cpp:199-627), all scenes at once:
+//   k_broad -> k_pair_scatter -> k_pair_rank_append -> k_pair_finish   (phase 1)
+//   k_manifold_init, k_force_init -> compaction                       (phase 2)
+//   incidence lists, greedy colouring, colour lists
+//   k_predict                                                          (phase 4)
+//   iterations x { per colour: k_primal ; k_dual ; k_fracture_reset }  (phase 5)
+//   k_velocity                                                         (phase 6)
+//   per colour: k_primal(alpha = 0) ; k_stick                          (phase 7)
+#include <hip/hip_runtime.h>
+
+#include "avbd_collide.h"
+#include "avbd_fem.h"
+#include "avbd_math.h"
+#include "avbd_world.h"
+
+#define GRID_STRIDE(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += blockDim.x * gridDim.x)
+
+AVBD_DEV v3 f3(float4 a) { return v3{a.x, a.y, a.z}; }
+AVBD_DEV qt fq(float4 a) { return qt{a.x, a.y, a.z, a.w}; }
+AVBD_DEV float4 v4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+AVBD_DEV float4 q4(qt q) { return make_float4(q.w, q.x, q.y, q.z); }
+
+AVBD_DEV bool scene_runs(const DevWorld& w, int s) {
+    const int* st = w.st + s * ST_WORDS;
+    return st[ST_ACTIVE] != 0 && st[ST_EMERGENCY] == 0;
+}
+AVBD_DEV bool is_static(const DevWorld& w, int b) { return (w.flags[b] & AVBD_BODY_STATIC) != 0; }
+
+__global__ void k_fill_int(int* p, int n, int v) {
+    GRID_STRIDE(i, n) p[i] = v;
+}
+__global__ void k_set_word(int* p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ------------------------------------------------------------------ phase 1
+// Is there already a force linking bodies i and j?  (isConstrainedTo,
+// force.hpp:81-89): scan the shorter incidence list.
+AVBD_DEV bool constrained(const DevWorld& w, int i, int j) {
+    int li = w.f_off[i + 1] - w.f_off[i];
+    int lj = w.f_off[j + 1] - w.f_off[j];
+    int self = li <= lj ? i : j;
+    int other = li <= lj ? j : i;
+    for (int k = w.f_off[self]; k < w.f_off[self + 1]; ++k) {
+        uint32_t e = w.f_ent[k];
+        uint32_t idx = ENT_INDEX(e);
+        int o;
+        if (ENT_KIND(e) == ENT_CONTACT) {
+            int m = idx >> 3;
+            o = ENT_SIDE_B(e) ? w.m.a[m] : w.m.b[m];
+        } else if (ENT_KIND(e) == ENT_JOINT) {
+            o = ENT_SIDE_B(e) ? w.j_a[idx] : w.j_b[idx];
+        } else {
+            o = ENT_SIDE_B(e) ? w.s_a[idx] : w.s_b[idx];
+        }
+        if (o == other) return true;
+    }
+    return false;
+}
+
+// All-pairs bounding-sphere test (solver.cpp:236-255), tiled: one work item
+// = (scene, i-tile, j-tile >= i-tile); the j-tile is staged in LDS.
+// New (unconstrained) pairs are appended unordered; cnt_i counts per i.
+__global__ __launch_bounds__(256) void k_broad(DevWorld w, const int4* items) {
+    const int4 it = items[blockIdx.x];
+    const int s = it.x;
+    if (!scene_runs(w, s)) return;
+    const int off = w.sc_off[s];
+    const int end = w.sc_off[s + 1];
+    __shared__ float4 tile[256];
+    const int j0 = off + it.z * 256;
+    const int jl = j0 + threadIdx.x;
+    if (jl < end) {
+        float4 p = w.pos[jl];
+        tile[threadIdx.x] = make_float4(p.x, p.y, p.z, w.scale[jl].w);
+    }
+    __syncthreads();
+    const int i = off + it.y * 256 + threadIdx.x;
+    if (i >= end) return;
+    const float4 pi4 = w.pos[i];
+    const v3 pi = f3(pi4);
+    const float ri = w.scale[i].w;
+    const int jn = min(256, end - j0);
+    for (int t = 0; t < jn; ++t) {
+        const int j = j0 + t;
+        if (j <= i) continue;
+        const float4 pj = tile[t];
+        v3 d = sub(pi, f3(pj));
+        float d2 = dot(d, d);
+        float rs = ri + pj.w;
+        if (d2 > (rs * rs) * 1.00001f) continue;  // conservative pre-test
+        float dist = sqrtf(d2);
+        if (dist < rs) {
+            if (!constrained(w, i, j)) {
+                int slot = atomicAdd(&w.g[G_NEW_PAIRS], 1);
+                if (slot < w.pair_cap) {
+                    w.pairs[slot] = make_int2(i, j);
+                    atomicAdd(&w.cnt_i[i], 1);
+                } else {
+                    atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
+                }
+            }
+        }
+    }
+}
+
+// Group the new pairs by first body (seg_off = exclusive scan of cnt_i).
+__global__ void k_pair_scatter(DevWorld w) {
+    int K = min(w.g[G_NEW_PAIRS], w.pair_cap);
+    GRID_STRIDE(p, K) {
+        int2 pr = w.pairs[p];
+        int slot = w.seg_off[pr.x] + atomicAdd(&w.cur_i[pr.x], 1);
+        w.pairs_tmp[slot] = pr;
+    }
+}
+
+// Rank each pair inside its first-body segment by second body: the final
+// position is the pair's lexicographic rank, which is the creation order of
+// the reference's nested loop.  New manifolds are appended in that order.
+__global__ void k_pair_rank_append(DevWorld w) {
+    const int K = min(w.g[G_NEW_PAIRS], w.pair_cap);
+    const int M0 = w.g[G_MANIFOLDS];
+    const long long base = *w.next_serial;
+    GRID_STRIDE(p, K) {
+        int2 pr = w.pairs_tmp[p];
+        int lo = w.seg_off[pr.x], hi = w.seg_off[pr.x + 1];
+        int rank = 0;
+        for (int q = lo; q < hi; ++q) rank += (w.pairs_tmp[q].y < pr.y) ? 1 : 0;
+        int fin = lo + rank;
+        int m = M0 + fin;
+        if (m < w.Mcap) {
+            w.m.a[m] = pr.x;
+            w.m.b[m] = pr.y;
+            w.m.serial[m] = base + fin;
+            w.m.n[m] = 0;
+            w.m.fric[m] = 0.f;
+            w.st[w.scene[pr.x] * ST_WORDS + ST_DIRTY] = 1;
+        }
+    }
+}
+
+__global__ void k_pair_finish(DevWorld w) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int K = min(w.g[G_NEW_PAIRS], w.pair_cap);
+    int M0 = w.g[G_MANIFOLDS];
+    int M1 = M0 + K;
+    if (M1 > w.Mcap) {
+        w.g[G_ERR_MANIFOLD_CAP] = 1;
+        M1 = w.Mcap;
+    }
+    w.g[G_MANIFOLDS_OLD] = M0;
+    w.g[G_MANIFOLDS] = M1;
+    *w.next_serial += K;
+}
+
+// ------------------------------------------------------------------ phase 2
+// Manifold::Initialize (manifold.cpp:32-194) + the force warm start of
+// solver.cpp:269-292, one thread per manifold.
+__global__ void k_manifold_init(DevWorld w) {
+    const int M = w.g[G_MANIFOLDS];
+    GRID_STRIDE(m, w.Mcap) {
+        if (m >= M) {
+            w.alive[m] = 0;
+            continue;
+        }
+        const int a = w.m.a[m], b = w.m.b[m];
+        const int s = w.scene[a];
+        if (!scene_runs(w, s)) {
+            w.alive[m] = 1;
+            continue;
+        }
+        const avbd_scene& sc = w.sc[s];
+        const float fa = w.angvel[a].w, fb = w.angvel[b].w;
+        const float fric = sqrtf(fa * fb);
+        const v3 pA = f3(w.pos[a]), pB = f3(w.pos[b]);
+        const qt qA = fq(w.rot[a]), qB = fq(w.rot[b]);
+        DevCollision col;
+        collide_boxes(pA, qA, f3(w.scale[a]), pB, qB, f3(w.scale[b]), col);
+        float4* hot = w.m.hot + (size_t)m * 8 * CONTACT_HOT_F4;
+        float4* cold = w.m.cold + (size_t)m * 8 * CONTACT_COLD_F4;
+        const int oldN = w.m.n[m];
+        // match new contacts to old ones by feature id (manifold.cpp:74-99)
+        float mk[8][3], ml[8][3];
+        int mstick[8];
+        bool isNew[8];
+        for (int i = 0; i < col.n; ++i) {
+            mk[i][0] = mk[i][1] = mk[i][2] = 0.f;
+            ml[i][0] = ml[i][1] = ml[i][2] = 0.f;
+            mstick[i] = 0;
+            isNew[i] = true;
+            for (int j = 0; j < oldN; ++j) {
+                float4 c0 = cold[j * CONTACT_COLD_F4 + 0];
+                if (__float_as_uint(c0.x) == col.c[i].id) {
+                    float4 h3 = hot[j * CONTACT_HOT_F4 + 3];
+                    float4 h4 = hot[j * CONTACT_HOT_F4 + 4];
+                    float4 h5 = hot[j * CONTACT_HOT_F4 + 5];
+                    ml[i][0] = h3.w; ml[i][1] = h4.w; ml[i][2] = h5.x;
+                    mk[i][0] = h5.y; mk[i][1] = h5.z; mk[i][2] = h5.w;
+                    mstick[i] = (int)c0.y;
+                    if (mstick[i]) {
+                        float4 c1 = cold[j * CONTACT_COLD_F4 + 1];
+                        col.c[i].rA = v3{c0.z, c0.w, c1.x};
+                        col.c[i].rB = v3{c1.y, c1.z, c1.w};
+                    }
+                    isNew[i] = false;
+                    break;
+                }
+            }
+        }
+        const int n = col.n;
+        w.m.n[m] = n;
+        w.m.fric[m] = fric;
+        if (n == 0) {
+            w.alive[m] = 0;
+            w.st[s * ST_WORDS + ST_DIRTY] = 1;
+            continue;
+        }
+        w.alive[m] = 1;
+        const m3 RA = rotmat(qA), RB = rotmat(qB);
+        const bool decay = sc.post_stabilization == 0;
+        for (int i = 0; i < n; ++i) {
+            const DevContact& c = col.c[i];
+            v3 nr = neg(c.normal);
+            v3 t1 = fabsf(nr.x) > 0.9f ? cross(nr, v3{0.f, 1.f, 0.f}) : cross(nr, v3{1.f, 0.f, 0.f});
+            t1 = vdiv(t1, norm(t1));
+            v3 t2 = cross(nr, t1);
+            v3 rAw = mv(RA, c.rA);
+            v3 rBw = mv(RB, c.rB);
+            v3 diff = sub(add(pA, rAw), add(pB, rBw));
+            float c0n = dot(nr, diff) + 0.0005f;
+            float c0t1 = dot(t1, diff);
+            float c0t2 = dot(t2, diff);
+            float k[3] = {mk[i][0], mk[i][1], mk[i][2]};
+            float l[3] = {ml[i][0], ml[i][1], ml[i][2]};
+            if (isNew[i]) {
+                k[0] = clampf(c.penetration * sc.on_penetration_penalty, 1.f, 100000.0f);
+                k[1] = 1.f;
+                k[2] = 1.f;
+            }
+            // warm start (solver.cpp:280-291); manifold rows have infinite stiffness
+            for (int r = 0; r < 3; ++r) {
+                if (decay) l[r] = l[r] * (sc.alpha * sc.gamma);
+                float kk = k[r] * sc.gamma;
+                if (kk < 1.f) kk = 1.f;
+                if (kk > 1000000000.0f) kk = 1000000000.0f;
+                k[r] = kk;
+            }
+            hot[i * CONTACT_HOT_F4 + 0] = v4(nr, c0n);
+            hot[i * CONTACT_HOT_F4 + 1] = v4(t1, c0t1);
+            hot[i * CONTACT_HOT_F4 + 2] = v4(t2, c0t2);
+            hot[i * CONTACT_HOT_F4 + 3] = v4(rAw, l[0]);
+            hot[i * CONTACT_HOT_F4 + 4] = v4(rBw, l[1]);
+            hot[i * CONTACT_HOT_F4 + 5] = make_float4(l[2], k[0], k[1], k[2]);
+            cold[i * CONTACT_COLD_F4 + 0] = make_float4(__uint_as_float(c.id), (float)mstick[i], c.rA.x, c.rA.y);
+            cold[i * CONTACT_COLD_F4 + 1] = make_float4(c.rA.z, c.rB.x, c.rB.y, c.rB.z);
+            cold[i * CONTACT_COLD_F4 + 2] = v4(c.normal, c.penetration);
+        }
+    }
+}
+
+AVBD_DEV v3 transform_point(const DevWorld& w, int b, v3 p) {
+    return add(qrot(fq(w.rot[b]), cmul(f3(w.scale[b]), p)), f3(w.pos[b]));
+}
+// AngularError (joint.cpp:13-21)
+AVBD_DEV v3 joint_angular_error(qt qa, qt qb, qt rest, float arm) {
+    return mul(quaternion_to_rotvec(qmul(qmul(qconj(qa), qb), qconj(rest))), arm);
+}
+
+// Joint::Initialize / Spring::Initialize and the force warm start.
+__global__ void k_force_init(DevWorld w) {
+    GRID_STRIDE(t, w.J + w.SP) {
+        if (t < w.J) {
+            const int j = t;
+            if (w.j_flags[j] & FORCE_DEAD) continue;
+            const int b = w.j_b[j];
+            const int s = w.scene[b];
+            if (!scene_runs(w, s)) continue;
+            if (w.j_flags[j] & AVBD_FORCE_DISABLED) {
+                w.j_flags[j] |= FORCE_DEAD;
+                w.st[s * ST_WORDS + ST_DIRTY] = 1;
+                continue;
+            }
+            const int a = w.j_a[j];
+            const float4 ra = w.j_ra[j];
+            v3 wA = a >= 0 ? transform_point(w, a, f3(ra)) : f3(ra);
+            v3 wB = transform_point(w, b, f3(w.j_rb[j]));
+            v3 cp = sub(wA, wB);
+            qt qa = a >= 0 ? fq(w.rot[a]) : qt{1.f, 0.f, 0.f, 0.f};
+            v3 ca = joint_angular_error(qa, fq(w.rot[b]), fq(w.j_rest[j]), ra.w);
+            float* c0 = w.j_c0 + j * 6;
+            c0[0] = cp.x; c0[1] = cp.y; c0[2] = cp.z;
+            c0[3] = ca.x; c0[4] = ca.y; c0[5] = ca.z;
+            const avbd_scene& sc = w.sc[s];
+            for (int r = 0; r < 6; ++r) {
+                float* row = w.j_row + (j * 6 + r) * 6;
+                if (sc.post_stabilization == 0) row[4] = row[4] * (sc.alpha * sc.gamma);
+                float kk = row[5] * sc.gamma;
+                if (kk < 1.f) kk = 1.f;
+                if (kk > 1000000000.0f) kk = 1000000000.0f;
+                row[5] = fminf(kk, row[0]);
+            }
+        } else {
+            const int sp = t - w.J;
+            if (w.s_flags[sp] & FORCE_DEAD) continue;
+            const int b = w.s_b[sp];
+            const int s = w.scene[b];
+            if (!scene_runs(w, s)) continue;
+            if (w.s_flags[sp] & AVBD_FORCE_DISABLED) {
+                w.s_flags[sp] |= FORCE_DEAD;
+                w.st[s * ST_WORDS + ST_DIRTY] = 1;
+                continue;
+            }
+            const avbd_scene& sc = w.sc[s];
+            float* row = w.s_row + sp * 6;
+            if (sc.post_stabilization == 0) row[4] = row[4] * (sc.alpha * sc.gamma);
+            float kk = row[5] * sc.gamma;
+            if (kk < 1.f) kk = 1.f;
+            if (kk > 1000000000.0f) kk = 1000000000.0f;
+            row[5] = fminf(kk, row[0]);
+        }
+    }
+}
+
+// Stable compaction of the live manifolds into the other buffer.
+__global__ void k_compact(DevWorld w, ManifoldBuf dst) {
+    const int M = w.g[G_MANIFOLDS];
+    GRID_STRIDE(m, M) {
+        if (!w.alive[m]) continue;
+        const int d = w.alive_off[m];
+        dst.a[d] = w.m.a[m];
+        dst.b[d] = w.m.b[m];
+        dst.serial[d] = w.m.serial[m];
+        const int n = w.m.n[m];
+        dst.n[d] = n;
+        dst.fric[d] = w.m.fric[m];
+        const float4* hs = w.m.hot + (size_t)m * 8 * CONTACT_HOT_F4;
+        float4* hd = dst.hot + (size_t)d * 8 * CONTACT_HOT_F4;
+        for (int k = 0; k < n * CONTACT_HOT_F4; ++k) hd[k] = hs[k];
+        const float4* cs = w.m.cold + (size_t)m * 8 * CONTACT_COLD_F4;
+        float4* cd = dst.cold + (size_t)d * 8 * CONTACT_COLD_F4;
+        for (int k = 0; k < n * CONTACT_COLD_F4; ++k) cd[k] = cs[k];
+    }
+}
+
+__global__ void k_compact_finish(DevWorld w) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) w.g[G_MANIFOLDS] = w.alive_off[w.Mcap];
+}
+
+// ------------------------------------------------------- incidence lists
+__global__ void k_inc_count(DevWorld w) {
+    const int M = w.g[G_MANIFOLDS];
+    GRID_STRIDE(t, M + w.J + w.SP) {
+        if (t < M) {
+            const int n = w.m.n[t];
+            atomicAdd(&w.cnt_i[w.m.a[t]], n);
+            atomicAdd(&w.cnt_i[w.m.b[t]], n);
+        } else if (t < M + w.J) {
+            const int j = t - M;
+            if (w.j_flags[j] & FORCE_DEAD) continue;
+            if (w.j_a[j] >= 0) atomicAdd(&w.cnt_i[w.j_a[j]], 1);
+            atomicAdd(&w.cnt_i[w.j_b[j]], 1);
+        } else {
+            const int sp = t - M - w.J;
+            if (w.s_flags[sp] & FORCE_DEAD) continue;
+            if (w.s_a[sp] >= 0) atomicAdd(&w.cnt_i[w.s_a[sp]], 1);
+            atomicAdd(&w.cnt_i[w.s_b[sp]], 1);
+        }
+    }
+}
+
+__global__ void k_inc_fill(DevWorld w) {
+    const int M = w.g[G_MANIFOLDS];
+    GRID_STRIDE(t, M + w.J + w.SP) {
+        if (t < M) {
+            const int n = w.m.n[t];
+            const int a = w.m.a[t], b = w.m.b[t];
+            int pa = w.f_off[a] + atomicAdd(&w.cur_i[a], n);
+            int pb = w.f_off[b] + atomicAdd(&w.cur_i[b], n);
+            for (int c = 0; c < n; ++c) {
+                w.f_ent[pa + c] = MAKE_ENT(ENT_CONTACT, 0, t * 8 + c);
+                w.f_ent[pb + c] = MAKE_ENT(ENT_CONTACT, 1, t * 8 + c);
+            }
+        } else if (t < M + w.J) {
+            const int j = t - M;
+            if (w.j_flags[j] & FORCE_DEAD) continue;
+            const int a = w.j_a[j], b = w.j_b[j];
+            if (a >= 0) w.f_ent[w.f_off[a] + atomicAdd(&w.cur_i[a], 1)] = MAKE_ENT(ENT_JOINT, 0, j);
+            w.f_ent[w.f_off[b] + atomicAdd(&w.cur_i[b], 1)] = MAKE_ENT(ENT_JOINT, 1, j);
+        } else {
+            const int sp = t - M - w.J;
+            if (w.s_flags[sp] & FORCE_DEAD) continue;
+            const int a = w.s_a[sp], b = w.s_b[sp];
+            if (a >= 0) w.f_ent[w.f_off[a] + atomicAdd(&w.cur_i[a], 1)] = MAKE_ENT(ENT_SPRING, 0, sp);
+            w.f_ent[w.f_off[b] + atomicAdd(&w.cur_i[b], 1)] = MAKE_ENT(ENT_SPRING, 1, sp);
+        }
+    }
+}
+
+// Creation order of an incidence entry: (force serial, contact index).
+AVBD_DEV unsigned long long ent_key(const DevWorld& w, uint32_t e) {
+    uint32_t idx = ENT_INDEX(e);
+    unsigned long long s;
+    unsigned c = 0;
+    if (ENT_KIND(e) == ENT_CONTACT) {
+        s = (unsigned long long)w.m.serial[idx >> 3];
+        c = idx & 7u;
+    } else if (ENT_KIND(e) == ENT_JOINT) {
+        s = (unsigned long long)w.j_serial[idx];
+    } else {
+        s = (unsigned long long)w.s_serial[idx];
+    }
+    return (s << 3) | c;
+}
+
+// Order each dynamic body's list by creation (Mesh::forces is append-ordered
+// with stable erase, force.cpp:5-19): insertion sort per body.
+__global__ void k_inc_sort(DevWorld w) {
+    GRID_STRIDE(b, w.N) {
+        if (is_static(w, b)) continue;
+        const int lo = w.f_off[b], hi = w.f_off[b + 1];
+        for (int i = lo + 1; i < hi; ++i) {
+            uint32_t e = w.f_ent[i];
+            unsigned long long k = ent_key(w, e);
+            int j = i - 1;
+            while (j >= lo && ent_key(w, w.f_ent[j]) > k) {
+                w.f_ent[j + 1] = w.f_ent[j];
+                --j;
+            }
+            w.f_ent[j + 1] = e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- colouring
+// Other body of incidence entry e seen from body self.
+AVBD_DEV int ent_other(const DevWorld& w, uint32_t e) {
+    uint32_t idx = ENT_INDEX(e);
+    if (ENT_KIND(e) == ENT_CONTACT) {
+        int m = idx >> 3;
+        return ENT_SIDE_B(e) ? w.m.a[m] : w.m.b[m];
+    } else if (ENT_KIND(e) == ENT_JOINT) {
+        return ENT_SIDE_B(e) ? w.j_a[idx] : w.j_b[idx];
+    }
+    return ENT_SIDE_B(e) ? w.s_a[idx] : w.s_b[idx];
+}
+
+template <class F>
+AVBD_DEV void for_each_neighbour(const DevWorld& w, int v, F f) {
+    for (int k = w.f_off[v]; k < w.f_off[v + 1]; ++k) {
+        int o = ent_other(w, w.f_ent[k]);
+        if (o >= 0) f(o);
+    }
+    for (int k = w.e_off[v]; k < w.e_off[v + 1]; ++k) {
+        uint32_t e = w.e_ent[k];
+        uint32_t idx = ENT_INDEX(e);
+        int local = EENT_LOCAL(e);
+        if (EENT_TRI(e)) {
+            int4 t = w.r_v[idx];
+            if (local != 0) f(t.x);
+            if (local != 1) f(t.y);
+            if (local != 2) f(t.z);
+        } else {
+            int4 t = w.t_v[idx];
+            if (local != 0) f(t.x);
+            if (local != 1) f(t.y);
+            if (local != 2) f(t.z);
+            if (local != 3) f(t.w);
+        }
+    }
+}
+
+// Greedy first-fit colouring of one scene per workgroup, bit-identical to the
+// sequential rule "dynamic bodies in ascending index take the smallest colour
+// not used by lower-index dynamic neighbours".  A body is coloured as soon as
+// all its lower-index neighbours are (a counter per body), so the result does
+// not depend on the order threads run in.
+__global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
+    const int s = blockIdx.x;
+    if (!scene_runs(w, s)) return;
+    int* st = w.st + s * ST_WORDS;
+    if (!st[ST_DIRTY]) return;
+    const int off = w.sc_off[s], end = w.sc_off[s + 1];
+    const int n = end - off;
+    int* qa = w.queue + 2 * off;
+    int* qb = qa + n;
+    __shared__ int qn[2];
+    __shared__ int maxc;
+    if (threadIdx.x == 0) { qn[0] = 0; qn[1] = 0; maxc = -1; }
+    __syncthreads();
+    for (int v = off + threadIdx.x; v < end; v += blockDim.x) {
+        w.colour[v] = -1;
+        if (is_static(w, v)) continue;
+        int c = 0;
+        for_each_neighbour(w, v, [&](int o) {
+            if (o < v && !is_static(w, o)) ++c;
+        });
+        w.ccount[v] = c;
+        if (c == 0) qa[atomicAdd(&qn[0], 1)] = v;
+    }
+    __syncthreads();
+    int cur = 0;
+    while (true) {
+        const int len = qn[cur];
+        if (len == 0) break;
+        int* q = cur ? qb : qa;
+        int* nq = cur ? qa : qb;
+        for (int k = threadIdx.x; k < len; k += blockDim.x) {
+            const int v = q[k];
+            unsigned long long used = 0ull;
+            bool high = false;
+            for_each_neighbour(w, v, [&](int o) {
+                if (o < v && !is_static(w, o)) {
+                    int c = __atomic_load_n(&w.colour[o], __ATOMIC_RELAXED);
+                    if (c >= 0 && c < 64) used |= (1ull << c);
+                    else if (c >= 64) high = true;
+                }
+            });
+            int c = __ffsll((long long)~used) - 1;
+            if (used == ~0ull) {
+                // more than 64 colours: linear search above 64
+                c = 64;
+                while (true) {
+                    bool taken = false;
+                    for_each_neighbour(w, v, [&](int o) {
+                        if (o < v && !is_static(w, o) && __atomic_load_n(&w.colour[o], __ATOMIC_RELAXED) == c) taken = true;
+                    });
+                    if (!taken) break;
+                    ++c;
+                }
+            }
+            (void)high;
+            __atomic_store_n(&w.colour[v], c, __ATOMIC_RELAXED);
+            atomicMax(&maxc, c);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) qn[cur ^ 1] = 0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < len; k += blockDim.x) {
+            const int u = q[k];
+            for_each_neighbour(w, u, [&](int o) {
+                if (o > u && !is_static(w, o)) {
+                    if (atomicSub(&w.ccount[o], 1) == 1) nq[atomicAdd(&qn[cur ^ 1], 1)] = o;
+                }
+            });
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) qn[cur] = 0;
+        cur ^= 1;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        st[ST_NCOL] = maxc + 1;
+        st[ST_DIRTY] = 0;
+        if (maxc + 1 > AVBD_MAX_COLOURS) w.g[G_ERR_COLOURS] = 1;
+    }
+}
+
+__global__ void k_colour_hist(DevWorld w) {
+    GRID_STRIDE(b, w.N) {
+        if (is_static(w, b)) continue;
+        if (!scene_runs(w, w.scene[b])) continue;
+        int c = w.colour[b];
+        if (c >= 0 && c < AVBD_MAX_COLOURS) atomicAdd(&w.col_cnt[c], 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_colour_scan(DevWorld w) {
+    __shared__ int buf[AVBD_MAX_COLOURS];
+    const int t = threadIdx.x;
+    buf[t] = w.col_cnt[t];
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0, last = -1;
+        for (int c = 0; c < AVBD_MAX_COLOURS; ++c) {
+            w.col_off[c] = acc;
+            if (buf[c] > 0) last = c;
+            acc += buf[c];
+        }
+        w.col_off[AVBD_MAX_COLOURS] = acc;
+        w.g[G_NCOLOURS] = last + 1;
+    }
+    __syncthreads();
+    w.col_cnt[t] = 0;
+}
+
+__global__ void k_colour_fill(DevWorld w) {
+    GRID_STRIDE(b, w.N) {
+        if (is_static(w, b)) continue;
+        if (!scene_runs(w, w.scene[b])) continue;
+        int c = w.colour[b];
+        if (c >= 0 && c < AVBD_MAX_COLOURS) w.col_list[w.col_off[c] + atomicAdd(&w.col_cnt[c], 1)] = b;
+    }
+}
+
+// ------------------------------------------------------------------ phase 4
+// Body warm start (solver.cpp:324-384), all bodies of running scenes.
+__global__ void k_predict(DevWorld w) {
+    GRID_STRIDE(b, w.N) {
+        const int s = w.scene[b];
+        if (!scene_runs(w, s)) continue;
+        const avbd_scene& sc = w.sc[s];
+        const float dt = sc.dt;
+        const v3 g{sc.gravity[0], sc.gravity[1], sc.gravity[2]};
+        const uint32_t fl = w.flags[b];
+        const bool st = (fl & AVBD_BODY_STATIC) != 0;
+        const bool part = (fl & AVBD_BODY_PARTICLE) != 0;
+        float4 av4 = w.angvel[b];
+        v3 av = f3(av4);
+        av.x = fmaxf(fminf(av.x, 50.f), -50.f);
+        av.y = fmaxf(fminf(av.y, 50.f), -50.f);
+        av.z = fmaxf(fminf(av.z, 50.f), -50.f);
+        w.angvel[b] = v4(av, av4.w);
+        const float4 vel4 = w.vel[b];
+        const v3 vel = f3(vel4);
+        const v3 p = f3(w.pos[b]);
+        const qt rq = fq(w.rot[b]);
+        v3 ip = add(p, mul(vel, dt));
+        if (!st) ip = add(ip, mul(mul(g, dt), dt));
+        qt ir = rq;
+        qt nr = rq;
+        if (!st && !part) {
+            qt wq;
+            const v3 wv0 = av;
+            const v3 qv{rq.x, rq.y, rq.z};
+            wq.w = 0.f * rq.w - dot(wv0, qv);
+            v3 wv = add(add(smul(0.f, qv), smul(rq.w, wv0)), cross(wv0, qv));
+            qt q = rq;
+            q.w = q.w + (0.5f * wq.w) * dt;
+            q.x = q.x + (0.5f * wv.x) * dt;
+            q.y = q.y + (0.5f * wv.y) * dt;
+            q.z = q.z + (0.5f * wv.z) * dt;
+            ir = qnormalized(q);
+            nr = ir;
+        }
+        v3 acc = vdiv(sub(vel, f3(w.pvel[b])), dt);
+        float gm = norm(g);
+        float wgt = 0.f;
+        if (gm > 1e-6f) {
+            v3 gd = vdiv(g, gm);
+            wgt = dot(acc, gd) / gm;
+            wgt = clampf(wgt, 0.f, 1.f);
+        }
+        v3 np = add(add(p, mul(vel, dt)), mul(mul(mul(g, dt), dt), wgt));
+        w.last_pos[b] = v4(p, 0.f);
+        w.last_rot[b] = q4(rq);
+        w.inert_pos[b] = v4(ip, 0.f);
+        w.inert_rot[b] = q4(ir);
+        w.pos[b] = v4(np, 0.f);
+        w.rot[b] = q4(nr);
+        v3 dl = sub(np, p);
+        v3 da = rotation_difference(nr, rq);
+        w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
+        w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
+    }
+}
+
+// ------------------------------------------------------------------ phase 5
+struct Acc {
+    float L[21];  // packed lower 6x6
+    float r[6];
+};
+
+AVBD_DEV void acc_zero(Acc& A) {
+#pragma unroll
+    for (int i = 0; i < 21; ++i) A.L[i] = 0.f;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) A.r[i] = 0.f;
+}
+
+// rhs += J f ; lhs += (J J^T) k  (solver.cpp:417-419)
+AVBD_DEV void acc_row(Acc& A, const float J[6], float f, float k) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) A.r[i] = A.r[i] + J[i] * f;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) A.L[tri_idx(i, j)] = A.L[tri_idx(i, j)] + (J[i] * J[j]) * k;
+}
+
+AVBD_DEV float dot6(const float a[6], const float b[6]) {
+    float s = a[0] * b[0];
+#pragma unroll
+    for (int i = 1; i < 6; ++i) s = s + a[i] * b[i];
+    return s;
+}
+
+AVBD_DEV void load_diff(const DevWorld& w, int b, float d[6]) {
+    float4 x = w.diff0[b], y = w.diff1[b];
+    d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w; d[4] = y.x; d[5] = y.y;
+}
+
+AVBD_DEV void set_J(float J[6], v3 lin, v3 ang) {
+    J[0] = lin.x; J[1] = lin.y; J[2] = lin.z; J[3] = ang.x; J[4] = ang.y; J[5] = ang.z;
+}
+
+// Constraint values of one contact (Manifold::ComputeConstraints,
+// manifold.cpp:197-236): C for the three rows and the friction bound.
+struct ContactRows {
+    v3 n, t1, t2, rAw, rBw;
+    float C[3];
+    float lam[3];
+    float k[3];
+    float bound;
+};
+
+AVBD_DEV void contact_rows(const DevWorld& w, int m, int c, float alpha, const float dA[6], const float dB[6],
+                           ContactRows& R) {
+    const float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
+    const float4 h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4], h5 = h[5];
+    R.n = f3(h0); R.t1 = f3(h1); R.t2 = f3(h2); R.rAw = f3(h3); R.rBw = f3(h4);
+    R.lam[0] = h3.w; R.lam[1] = h4.w; R.lam[2] = h5.x;
+    R.k[0] = h5.y; R.k[1] = h5.z; R.k[2] = h5.w;
+    const float oma = 1.f - alpha;
+    const float c0[3] = {h0.w, h1.w, h2.w};
+    const v3 dir[3] = {R.n, R.t1, R.t2};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        float JA[6], JB[6];
+        set_J(JA, dir[r], cross(R.rAw, dir[r]));
+        set_J(JB, neg(dir[r]), neg(cross(R.rBw, dir[r])));
+        R.C[r] = (oma * c0[r] + dot6(JA, dA)) + dot6(JB, dB);
+    }
+    R.bound = fabsf(R.lam[0]) * w.m.fric[m];
+}
+
+// Joint rows (joint.cpp:97-173): C, J and the (5,5) Hessian entry per row
+// for the given side.
+struct JointRows {
+    float C[6];
+    float J[6][6];
+    float h55[3];
+};
+
+AVBD_DEV void joint_rows(const DevWorld& w, int j, bool sideB, float alpha, JointRows& R) {
+    const int a = w.j_a[j], b = w.j_b[j];
+    const float4 ra = w.j_ra[j], rb = w.j_rb[j];
+    v3 wA = a >= 0 ? transform_point(w, a, f3(ra)) : f3(ra);
+    v3 wB = transform_point(w, b, f3(rb));
+    v3 cp = sub(wA, wB);
+    qt qa = a >= 0 ? fq(w.rot[a]) : qt{1.f, 0.f, 0.f, 0.f};
+    qt qb = fq(w.rot[b]);
+    v3 ca = joint_angular_error(qa, qb, fq(w.j_rest[j]), ra.w);
+    const float* c0 = w.j_c0 + j * 6;
+    const float* rows = w.j_row + j * 36;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        R.C[i] = isinf(rows[i * 6 + 0]) ? comp(cp, i) - c0[i] * alpha : comp(cp, i);
+        R.C[3 + i] = isinf(rows[(3 + i) * 6 + 0]) ? comp(ca, i) - c0[3 + i] * alpha : comp(ca, i);
+    }
+    const int self = sideB ? b : a;
+    v3 r = qrot(fq(w.rot[self]), cmul(f3(w.scale[self]), sideB ? f3(rb) : f3(ra)));
+    const float arm = ra.w;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        v3 e = unit_axis(i);
+        if (!sideB) {
+            set_J(R.J[i], e, cross(r, e));
+            R.h55[i] = -comp(r, i);
+            set_J(R.J[3 + i], v3{0.f, 0.f, 0.f}, smul(-arm, e));
+        } else {
+            set_J(R.J[i], neg(e), neg(cross(r, e)));
+            R.h55[i] = comp(r, i);
+            set_J(R.J[3 + i], v3{0.f, 0.f, 0.f}, smul(arm, e));
+        }
+    }
+}
+
+AVBD_DEV float spring_C(const DevWorld& w, int sp) {
+    const int a = w.s_a[sp], b = w.s_b[sp];
+    const float4 ra = w.s_ra[sp];
+    v3 pA = a >= 0 ? transform_point(w, a, f3(ra)) : f3(ra);
+    v3 pB = transform_point(w, b, f3(w.s_rb[sp]));
+    return norm(sub(pB, pA)) - ra.w;
+}
+
+// Spring derivatives (spring.cpp:51-109) for one side: J and full 6x6 H.
+AVBD_DEV void spring_derivs(const DevWorld& w, int sp, bool sideB, float J[6], float H[6][6]) {
+    const int a = w.s_a[sp], b = w.s_b[sp];
+    const float4 ra = w.s_ra[sp], rb = w.s_rb[sp];
+    v3 pA = a >= 0 ? transform_point(w, a, f3(ra)) : f3(ra);
+    v3 pB = transform_point(w, b, f3(rb));
+    v3 d = sub(pB, pA);
+    float L = norm(d);
+    float L2 = dot(d, d);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        J[i] = 0.f;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) H[i][j] = 0.f;
+    }
+    if (L2 < 1e-6f) return;
+    v3 n = vdiv(d, L);
+    m3 dxx;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dxx.m[i][j] = ((i == j ? 1.f : 0.f) - comp(n, i) * comp(n, j)) / L;
+    m3 dxr, drr;
+    if (!sideB && a >= 0) {
+        v3 r = qrot(fq(w.rot[a]), f3(ra));
+        m3 Sr = skew(r);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Sr.m[i][j] = -Sr.m[i][j];
+        dxr = mm(dxx, transpose(Sr));
+        m3 t1 = mm(mm(Sr, dxx), transpose(Sr));
+        m3 t2 = mm(skew(n), skew(r));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) drr.m[i][j] = t1.m[i][j] - t2.m[i][j];
+        set_J(J, neg(n), cross(n, r));
+    } else {
+        v3 r = qrot(fq(w.rot[b]), f3(rb));
+        m3 mSr = skew(r);
+        dxr = mm(dxx, transpose(mSr));
+        m3 t1 = mm(mm(mSr, dxx), transpose(mSr));
+        m3 t2 = mm(skew(n), skew(r));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) drr.m[i][j] = t1.m[i][j] + t2.m[i][j];
+        set_J(J, n, cross(r, n));
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            H[i][j] = dxx.m[i][j];
+            H[i][3 + j] = dxr.m[i][j];
+            H[3 + i][j] = dxr.m[j][i];
+            H[3 + i][3 + j] = drr.m[i][j];
+        }
+}
+
+// Contributions of one incidence entry to the block system of body `self`.
+AVBD_DEV void accumulate_force(const DevWorld& w, uint32_t e, float alpha, const float dSelf[6], Acc& A) {
+    const uint32_t idx = ENT_INDEX(e);
+    const bool sideB = ENT_SIDE_B(e) != 0;
+    const uint32_t kind = ENT_KIND(e);
+    if (kind == ENT_CONTACT) {
+        const int m = idx >> 3, c = idx & 7;
+        float dO[6];
+        load_diff(w, sideB ? w.m.a[m] : w.m.b[m], dO);
+        ContactRows R;
+        contact_rows(w, m, c, alpha, sideB ? dO : dSelf, sideB ? dSelf : dO, R);
+        const v3 dir[3] = {R.n, R.t1, R.t2};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            float lo = r == 0 ? -INFINITY : -R.bound;
+            float hi = r == 0 ? 0.f : R.bound;
+            float f = clampf(R.k[r] * R.C[r] + R.lam[r], lo, hi);
+            float J[6];
+            if (!sideB) set_J(J, dir[r], cross(R.rAw, dir[r]));
+            else set_J(J, neg(dir[r]), neg(cross(R.rBw, dir[r])));
+            acc_row(A, J, f, R.k[r]);
+        }
+    } else if (kind == ENT_JOINT) {
+        JointRows R;
+        joint_rows(w, idx, sideB, alpha, R);
+        const float* rows = w.j_row + idx * 36;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const float stiff = rows[r * 6 + 0], fmn = rows[r * 6 + 2], fmx = rows[r * 6 + 3];
+            const float lam = rows[r * 6 + 4], k = rows[r * 6 + 5];
+            float f = clampf(k * R.C[r] + (isinf(stiff) ? lam : 0.f), fmn, fmx);
+            acc_row(A, R.J[r], f, k);
+            if (r < 3) A.L[tri_idx(5, 5)] = A.L[tri_idx(5, 5)] + sqrtf(R.h55[r] * R.h55[r]) * fabsf(f);
+        }
+    } else {
+        const float* row = w.s_row + idx * 6;
+        const float stiff = row[0], fmn = row[2], fmx = row[3], lam = row[4], k = row[5];
+        float C = spring_C(w, idx);
+        float J[6], H[6][6];
+        spring_derivs(w, idx, sideB, J, H);
+        float f = clampf(k * C + (isinf(stiff) ? lam : 0.f), fmn, fmx);
+        acc_row(A, J, f, k);
+        const float af = fabsf(f);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            float cn = H[0][j] * H[0][j];
+#pragma unroll
+            for (int i = 1; i < 6; ++i) cn = cn + H[i][j] * H[i][j];
+            A.L[tri_idx(j, j)] = A.L[tri_idx(j, j)] + sqrtf(cn) * af;
+        }
+    }
+}
+
+AVBD_DEV void accumulate_energy(const DevWorld& w, uint32_t e, int projMode, Acc& A) {
+    const uint32_t idx = ENT_INDEX(e);
+    const int local = EENT_LOCAL(e);
+    v3 g;
+    float H[6];
+    if (!EENT_TRI(e)) {
+        const int4 tv = w.t_v[idx];
+        const float4 d0 = w.t_d[idx * 3 + 0], d1 = w.t_d[idx * 3 + 1], d2 = w.t_d[idx * 3 + 2];
+        m3 Dm;
+        Dm.m[0][0] = d0.x; Dm.m[0][1] = d0.y; Dm.m[0][2] = d0.z;
+        Dm.m[1][0] = d0.w; Dm.m[1][1] = d1.x; Dm.m[1][2] = d1.y;
+        Dm.m[2][0] = d1.z; Dm.m[2][1] = d1.w; Dm.m[2][2] = d2.x;
+        const v3 g1 = row(Dm, 0), g2 = row(Dm, 1), g3 = row(Dm, 2);
+        v3 gN = local == 1 ? g1 : (local == 2 ? g2 : (local == 3 ? g3 : sub(sub(neg(g1), g2), g3)));
+        neohookean_vertex_terms(f3(w.pos[tv.x]), f3(w.pos[tv.y]), f3(w.pos[tv.z]), f3(w.pos[tv.w]), Dm, d2.y, d2.z,
+                                d2.w, gN, projMode, g, H);
+    } else {
+        const int4 tv = w.r_v[idx];
+        const float4 d0 = w.r_d[idx * 2 + 0], d1 = w.r_d[idx * 2 + 1];
+        const float Dm[4] = {d0.x, d0.y, d0.z, d0.w};
+        float gn0, gn1;
+        if (local == 1) { gn0 = Dm[0]; gn1 = Dm[1]; }
+        else if (local == 2) { gn0 = Dm[2]; gn1 = Dm[3]; }
+        else { gn0 = -Dm[0] - Dm[2]; gn1 = -Dm[1] - Dm[3]; }
+        stvk_vertex_terms(f3(w.pos[tv.x]), f3(w.pos[tv.y]), f3(w.pos[tv.z]), Dm, d1.x, d1.y, d1.z, gn0, gn1,
+                          projMode, g, H);
+    }
+    A.r[0] = A.r[0] + g.x;
+    A.r[1] = A.r[1] + g.y;
+    A.r[2] = A.r[2] + g.z;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) A.L[i] = A.L[i] + H[i];
+}
+
+// One primal block update per body of colour `c` (solver.cpp:392-448 and
+// 540-595).  G lanes cooperate on one body: each lane accumulates a strided
+// share of the body's incidence entries, the partial systems are summed with
+// a butterfly over the lane group, and every lane of the group solves the
+// 6x6 system (the result is written by the group's first lane).  G = 1 is the
+// sequential order of the reference loop.
+template <int G>
+__global__ __launch_bounds__(256) void k_primal(DevWorld w, int c, int iter, int poststab) {
+    const int lo = w.col_off[c], hi = w.col_off[c + 1];
+    const int lane = threadIdx.x % G;
+    const int groups_per_block = blockDim.x / G;
+    for (int base = blockIdx.x * groups_per_block; base < hi - lo; base += gridDim.x * groups_per_block) {
+        const int item = base + threadIdx.x / G;
+        const bool valid = item < hi - lo;
+        int b = valid ? w.col_list[lo + item] : 0;
+        bool run = valid;
+        int s = 0;
+        if (run) {
+            s = w.scene[b];
+            const avbd_scene& sc0 = w.sc[s];
+            if (w.st[s * ST_WORDS + ST_EMERGENCY]) run = false;
+            if (poststab) { if (!sc0.post_stabilization) run = false; }
+            else if (iter >= sc0.iterations) run = false;
+        }
+        // all lanes of a group agree on `run`; groups never straddle a wave
+        if (run) {
+            const avbd_scene& sc = w.sc[s];
+            const float alpha = poststab ? 0.f : sc.alpha;
+            const float dt = sc.dt;
+            const float dt2 = dt * dt;
+            float dSelf[6];
+            load_diff(w, b, dSelf);
+            Acc A;
+            acc_zero(A);
+            const qt lr = fq(w.last_rot[b]);
+            const float4 vm = w.vel[b];
+            const v3 I = f3(w.inertia[b]);
+            const v3 p = f3(w.pos[b]);
+            const qt q = fq(w.rot[b]);
+            if (lane == 0) {
+                // lhs = M / dt^2 ; rhs = lhs * (x - y, rotdiff(q, q_y))
+                const m3 R = rotmat(lr);
+                float Iw[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        Iw[i][j] = ((R.m[i][0] * I.x) * R.m[j][0] + (R.m[i][1] * I.y) * R.m[j][1]) +
+                                   (R.m[i][2] * I.z) * R.m[j][2];
+                const float ml = vm.w / dt2;
+                v3 dl = sub(p, f3(w.inert_pos[b]));
+                v3 da = rotation_difference(q, fq(w.inert_rot[b]));
+                A.L[tri_idx(0, 0)] = ml; A.L[tri_idx(1, 1)] = ml; A.L[tri_idx(2, 2)] = ml;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j <= i; ++j) A.L[tri_idx(3 + i, 3 + j)] = Iw[i][j] / dt2;
+                A.r[0] = ml * dl.x; A.r[1] = ml * dl.y; A.r[2] = ml * dl.z;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    A.r[3 + i] = ((Iw[i][0] / dt2) * da.x + (Iw[i][1] / dt2) * da.y) + (Iw[i][2] / dt2) * da.z;
+            }
+            const int f0 = w.f_off[b], f1 = w.f_off[b + 1];
+            for (int k = f0 + lane; k < f1; k += G) accumulate_force(w, w.f_ent[k], alpha, dSelf, A);
+            const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
+            for (int k = e0 + lane; k < e1; k += G) accumulate_energy(w, w.e_ent[k], sc.projection_mode, A);
+            if (G > 1) {
+#pragma unroll
+                for (int off = 1; off < G; off <<= 1) {
+#pragma unroll
+                    for (int i = 0; i < 21; ++i) A.L[i] = A.L[i] + __shfl_xor(A.L[i], off, G);
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) A.r[i] = A.r[i] + __shfl_xor(A.r[i], off, G);
+                }
+            }
+            float dx[6];
+            ldlt6_solve(A.L, A.r, dx);
+            if (lane == 0) {
+                v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
+                qt dq = quaternion_from_difference(v3{dx[3], dx[4], dx[5]}, -1.f);
+                qt nq = qnormalized(qmul(dq, q));
+                w.pos[b] = v4(np, 0.f);
+                w.rot[b] = q4(nq);
+                v3 dl = sub(np, f3(w.last_pos[b]));
+                v3 da = rotation_difference(nq, lr);
+                w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
+                w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
+            }
+        }
+    }
+}
+
+template __global__ void k_primal<1>(DevWorld, int, int, int);
+template __global__ void k_primal<4>(DevWorld, int, int, int);
+template __global__ void k_primal<8>(DevWorld, int, int, int);
+template __global__ void k_primal<16>(DevWorld, int, int, int);
+template __global__ void k_primal<32>(DevWorld, int, int, int);
+
+// Dual / penalty update (solver.cpp:457-489), one thread per contact, joint
+// or spring.  A fracture disables the force and queues its bodies so that
+// k_fracture_reset clears their manifolds (order-independent: a cleared
+// manifold stays at lambda = k = 0 for the rest of the sweep).
+__global__ void k_dual(DevWorld w, int iter) {
+    const int M = w.g[G_MANIFOLDS];
+    const int total = M * 8 + w.J + w.SP;
+    GRID_STRIDE(t, total) {
+        if (t < M * 8) {
+            const int m = t >> 3, c = t & 7;
+            if (c >= w.m.n[m]) continue;
+            const int a = w.m.a[m], b = w.m.b[m];
+            const int s = w.scene[a];
+            if (!scene_runs(w, s)) continue;
+            const avbd_scene& sc = w.sc[s];
+            if (iter >= sc.iterations) continue;
+            float dA[6], dB[6];
+            load_diff(w, a, dA);
+            load_diff(w, b, dB);
+            ContactRows R;
+            contact_rows(w, m, c, sc.alpha, dA, dB, R);
+            float tl = sqrtf(R.lam[1] * R.lam[1] + R.lam[2] * R.lam[2]);
+            const float4 h1 = w.m.hot[((size_t)m * 8 + c) * CONTACT_HOT_F4 + 1];
+            const float4 h2 = w.m.hot[((size_t)m * 8 + c) * CONTACT_HOT_F4 + 2];
+            float tc = sqrtf(h1.w * h1.w + h2.w * h2.w);
+            int stick = (tl < R.bound) && (tc < 0.01f);
+            float lam[3], k[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                float lo = r == 0 ? -INFINITY : -R.bound;
+                float hi = r == 0 ? 0.f : R.bound;
+                float l = R.lam[r] + R.k[r] * R.C[r];
+                if (l < lo) l = lo;
+                else if (l > hi) l = hi;
+                float kk = R.k[r];
+                if (l > lo && l < hi) kk = fminf(kk + sc.beta * fabsf(R.C[r]), 1000000000.0f);
+                lam[r] = l;
+                k[r] = kk;
+            }
+            float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
+            h[3].w = lam[0];
+            h[4].w = lam[1];
+            h[5] = make_float4(lam[2], k[0], k[1], k[2]);
+            w.m.cold[((size_t)m * 8 + c) * CONTACT_COLD_F4].y = (float)stick;
+        } else if (t < M * 8 + w.J) {
+            const int j = t - M * 8;
+            if (w.j_flags[j] & FORCE_DEAD) continue;
+            const int s = w.scene[w.j_b[j]];
+            if (!scene_runs(w, s)) continue;
+            const avbd_scene& sc = w.sc[s];
+            if (iter >= sc.iterations) continue;
+            JointRows R;
+            joint_rows(w, j, true, sc.alpha, R);
+            float* rows = w.j_row + j * 36;
+            for (int r = 0; r < 6; ++r) {
+                float* row = rows + r * 6;
+                const float stiff = row[0], frac = row[1], fmn = row[2], fmx = row[3];
+                float l = (isinf(stiff) ? row[4] : 0.f) + row[5] * R.C[r];
+                if (l < fmn) l = fmn;
+                else if (l > fmx) l = fmx;
+                row[4] = l;
+                if (fabsf(l) >= frac) {
+                    for (int q = 0; q < 6; ++q) { rows[q * 6 + 4] = 0.f; rows[q * 6 + 5] = 0.f; }
+                    w.j_flags[j] |= AVBD_FORCE_DISABLED;
+                    int slot = atomicAdd(&w.g[G_FRACTURES], 2);
+                    if (slot + 1 < 2 * (w.J + w.SP) + 2) {
+                        w.fracture_bodies[slot] = w.j_a[j];
+                        w.fracture_bodies[slot + 1] = w.j_b[j];
+                    }
+                    break;
+                }
+                if (l > fmn && l < fmx) row[5] = fminf(row[5] + sc.beta * fabsf(R.C[r]), fminf(stiff, 1000000000.0f));
+            }
+        } else {
+            const int sp = t - M * 8 - w.J;
+            if (w.s_flags[sp] & FORCE_DEAD) continue;
+            const int s = w.scene[w.s_b[sp]];
+            if (!scene_runs(w, s)) continue;
+            const avbd_scene& sc = w.sc[s];
+            if (iter >= sc.iterations) continue;
+            float* row = w.s_row + sp * 6;
+            const float stiff = row[0], frac = row[1], fmn = row[2], fmx = row[3];
+            float C = spring_C(w, sp);
+            float l = (isinf(stiff) ? row[4] : 0.f) + row[5] * C;
+            if (l < fmn) l = fmn;
+            else if (l > fmx) l = fmx;
+            row[4] = l;
+            if (fabsf(l) >= frac) {
+                row[4] = 0.f;
+                row[5] = 0.f;
+                w.s_flags[sp] |= AVBD_FORCE_DISABLED;
+                int slot = atomicAdd(&w.g[G_FRACTURES], 2);
+                if (slot + 1 < 2 * (w.J + w.SP) + 2) {
+                    w.fracture_bodies[slot] = w.s_a[sp];
+                    w.fracture_bodies[slot + 1] = w.s_b[sp];
+                }
+                continue;
+            }
+            if (l > fmn && l < fmx) row[5] = fminf(row[5] + sc.beta * fabsf(C), fminf(stiff, 1000000000.0f));
+        }
+    }
+}
+
+// Clear lambda and penalty of every manifold touching a fractured force's
+// bodies (Force::ResetValues via solver.cpp:473-478), then clear the queue.
+__global__ void k_fracture_reset(DevWorld w) {
+    const int nf = min(w.g[G_FRACTURES], 2 * (w.J + w.SP) + 2);
+    if (nf == 0) return;
+    GRID_STRIDE(t, nf) {
+        const int b = w.fracture_bodies[t];
+        if (b < 0) continue;
+        for (int k = w.f_off[b]; k < w.f_off[b + 1]; ++k) {
+            uint32_t e = w.f_ent[k];
+            if (ENT_KIND(e) != ENT_CONTACT) continue;
+            const int m = ENT_INDEX(e) >> 3;
+            const int n = w.m.n[m];
+            for (int c = 0; c < n; ++c) {
+                float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
+                h[3].w = 0.f;
+                h[4].w = 0.f;
+                h[5] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ phase 6
+// Velocities from the displacement of the step, bounds and explosion checks
+// (solver.cpp:493-530, CheckExplosion 182-196).
+__global__ void k_velocity(DevWorld w) {
+    GRID_STRIDE(b, w.N) {
+        const int s = w.scene[b];
+        if (!scene_runs(w, s)) continue;
+        const avbd_scene& sc = w.sc[s];
+        const uint32_t fl = w.flags[b];
+        float4 v4v = w.vel[b];
+        if (!(fl & AVBD_BODY_STATIC)) {
+            float4 av4 = w.angvel[b];
+            w.pvel[b] = make_float4(v4v.x, v4v.y, v4v.z, 0.f);
+            w.pangvel[b] = make_float4(av4.x, av4.y, av4.z, 0.f);
+            const float4 d0 = w.diff0[b], d1 = w.diff1[b];
+            const float dt = sc.dt;
+            v4v = make_float4(d0.x / dt, d0.y / dt, d0.z / dt, v4v.w);
+            w.vel[b] = v4v;
+            if (!(fl & AVBD_BODY_PARTICLE)) w.angvel[b] = make_float4(d0.w / dt, d1.x / dt, d1.y / dt, av4.w);
+            const float4 p = w.pos[b];
+            if (p.x < sc.bounds_min[0] || p.x > sc.bounds_max[0] || p.y < sc.bounds_min[1] || p.y > sc.bounds_max[1] ||
+                p.z < sc.bounds_min[2] || p.z > sc.bounds_max[2]) {
+                atomicAdd(&w.st[s * ST_WORDS + ST_OOB], 1);
+                w.g[G_ANY_OOB] = 1;
+            }
+        }
+        const float mx = 500.f;
+        if (v4v.x > mx || v4v.y > mx || v4v.z > mx || v4v.x < -mx || v4v.y < -mx || v4v.z < -mx) {
+            w.st[s * ST_WORDS + ST_EMERGENCY] = 1;
+            w.g[G_ANY_EMERGENCY] = 1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ phase 7
+// Stick flags as left by the last ComputeConstraints of the step: with
+// post-stabilisation that is the post-stab sweep, using the final lambda.
+__global__ void k_stick(DevWorld w) {
+    const int M = w.g[G_MANIFOLDS];
+    GRID_STRIDE(t, M * 8) {
+        const int m = t >> 3, c = t & 7;
+        if (c >= w.m.n[m]) continue;
+        const int a = w.m.a[m], b = w.m.b[m];
+        const int s = w.scene[a];
+        if (!scene_runs(w, s)) continue;
+        if (!w.sc[s].post_stabilization) continue;
+        if (is_static(w, a) && is_static(w, b)) continue;
+        const float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
+        const float4 h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4], h5 = h[5];
+        float bound = fabsf(h3.w) * w.m.fric[m];
+        float tl = sqrtf(h4.w * h4.w + h5.x * h5.x);
+        float tc = sqrtf(h1.w * h1.w + h2.w * h2.w);
+        w.m.cold[((size_t)m * 8 + c) * CONTACT_COLD_F4].y = (float)((tl < bound) && (tc < 0.01f));
+    }
+}

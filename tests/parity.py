@@ -1,0 +1,74 @@
+"""Comparison helpers: GPU world vs oracle world after the same step(s)."""
+from __future__ import annotations
+
+import numpy as np
+
+POS_TOL = 1e-4   # relative, positions and orientations (single step)
+LAMBDA_TOL = 1e-3  # relative, lambda (single step)
+
+
+def pos_rel_err(a, b):
+    """max_i |x_a - x_b| / max(|x_b|, 1 m)."""
+    d = np.linalg.norm(a["pos"].astype(np.float64) - b["pos"].astype(np.float64), axis=1)
+    s = np.maximum(np.linalg.norm(b["pos"].astype(np.float64), axis=1), 1.0)
+    return float((d / s).max()) if len(d) else 0.0
+
+
+def rot_err(a, b):
+    """max_i min(|q_a - q_b|, |q_a + q_b|) (unit quaternions)."""
+    qa = a["rot"].astype(np.float64)
+    qb = b["rot"].astype(np.float64)
+    d = np.minimum(np.linalg.norm(qa - qb, axis=1), np.linalg.norm(qa + qb, axis=1))
+    return float(d.max()) if len(d) else 0.0
+
+
+def manifold_table(w):
+    out = {}
+    for m in w.manifolds:
+        out[(int(m["body_a"]), int(m["body_b"]))] = m
+    return out
+
+
+def compare_manifolds(g, o):
+    """Returns dict with: same pair set, contact counts/ids equal, lambda rel err."""
+    tg, to = manifold_table(g), manifold_table(o)
+    res = {"pairs_equal": set(tg) == set(to), "ids_equal": True, "n_equal": True, "lambda_rel": 0.0,
+           "n_manifolds": len(to), "missing": sorted(set(to) - set(tg))[:5], "extra": sorted(set(tg) - set(to))[:5]}
+    lam_all = [np.abs(m["contacts"]["lambda"][: m["n_contacts"]]).max() for m in o.manifolds if m["n_contacts"]]
+    lam_scale = max(lam_all) if lam_all else 1.0
+    worst = 0.0
+    for key, mo in to.items():
+        mg = tg.get(key)
+        if mg is None:
+            continue
+        n = int(mo["n_contacts"])
+        if int(mg["n_contacts"]) != n:
+            res["n_equal"] = False
+            continue
+        if not np.array_equal(mg["contacts"]["id"][:n], mo["contacts"]["id"][:n]):
+            res["ids_equal"] = False
+        lg = mg["contacts"]["lambda"][:n].astype(np.float64)
+        lo = mo["contacts"]["lambda"][:n].astype(np.float64)
+        rel = np.abs(lg - lo) / (np.abs(lo) + 1e-3 * lam_scale)
+        worst = max(worst, float(rel.max()) if rel.size else 0.0)
+    res["lambda_rel"] = worst
+    return res
+
+
+def compare_rows(g_rows, o_rows):
+    g = np.asarray(g_rows, np.float64)
+    o = np.asarray(o_rows, np.float64)
+    if g.size == 0:
+        return 0.0
+    scale = np.abs(o).max() if o.size else 1.0
+    return float((np.abs(g - o) / (np.abs(o) + 1e-3 * max(scale, 1e-12))).max())
+
+
+def summary(g, o):
+    s = {"pos_rel": pos_rel_err(g.bodies, o.bodies), "rot": rot_err(g.bodies, o.bodies)}
+    s.update(compare_manifolds(g, o))
+    if len(o.joints):
+        s["joint_lambda_rel"] = compare_rows(g.joints["lambda"], o.joints["lambda"])
+    if len(o.springs):
+        s["spring_lambda_rel"] = compare_rows(g.springs["lambda"], o.springs["lambda"])
+    return s
