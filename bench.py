@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X AVBD step.
+
+Default workload: the batched mix (4096 independent scenes of uneven size:
+rigid box piles with 8-2048 boxes, hanging Neo-Hookean blocks with 1k-50k
+tets, per-scene dt in {1/60, 1/120} and 8-16 iterations), resident in HBM and
+advanced with one step call over all scenes.  Prints ONE JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mix|fem|boxes20k|pyramid]
+
+value = whole-job scene-steps/s.  The line also carries the roofline of the
+dominant kernel (the primal block-solve sweep, timed per launch with HIP
+events on the library's stream) and a CPU baseline: the FP32 oracle (a
+restatement of the reference's single-threaded Solver::Step) timed on one
+host core over a bounded, size-stratified sample of the same scenes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "batched scene-steps/s + p99 step latency; body/vertex updates/s vs plain/CPU"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    from skyephyx_amd import scenes, scenes_fast
+
+    if name == "mix":
+        return scenes_fast.batched_mix(4096), "batched mix, 4096 scenes (boxes 8-2048 log-uniform, FEM 1k-50k tets)"
+    if name == "fem":
+        return scenes_fast.fem_block(32, cell=0.25), "FEM block 32^3 vertices, 148955 tets, Neo-Hookean E=1e5 nu=0.3"
+    if name == "boxes20k":
+        return (scenes.box_lattice(shape=(40, 25, 20), spacing=3.5, n_chain=1000, n_springs=5000),
+                "20k boxes in a 40x25x20 lattice + 1k-link ball-joint chain + 5k springs")
+    if name == "pyramid":
+        return scenes.pyramid_2d(), "2-D pyramid of 55 unit boxes on a static ground"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def measured_traffic(workload, lanes):
+    """HBM bytes per k_primal launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size), or
+    None when no summary exists for this workload and lane count."""
+    path = os.path.join(ROOT, "profiles", f"k_primal_batched_{workload}_hbm.md")
+    if workload != "mix" or lanes != 8 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def stratified_sample(world, k, seed=3):
+    """k scene ids, one from each size quantile bin."""
+    sizes = np.bincount(world.bodies["scene"], minlength=len(world.scenes)).astype(np.float64)
+    if len(world.tets):
+        sizes += np.bincount(world.bodies["scene"][world.tets["v"][:, 0]], minlength=len(world.scenes))
+    order = np.argsort(sizes, kind="stable")
+    bins = np.array_split(order, min(k, len(order)))
+    rng = np.random.default_rng(seed)
+    return [int(rng.choice(b)) for b in bins if len(b)], [len(b) for b in bins if len(b)]
+
+
+def cpu_baseline(world, budget_s=20.0):
+    """Oracle FP32 step on one core over a size-stratified scene sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+
+    S = len(world.scenes)
+    ids, weights = stratified_sample(world, 16)
+    total_w, t_w, spent, used = 0, 0.0, 0.0, 0
+    per = []
+    for sid, wgt in zip(ids, weights):
+        if spent > budget_s:
+            break
+        sub = world.extract_scene(sid)
+        o = O.Oracle(sub, fp64=False, order=O.NATURAL)
+        t0 = time.perf_counter()
+        o.step(1)
+        dt = time.perf_counter() - t0
+        spent += dt
+        used += 1
+        total_w += wgt
+        t_w += wgt * dt
+        per.append((sid, len(sub.bodies), len(sub.tets), dt))
+    mean_t = t_w / total_w  # estimated mean CPU seconds per scene-step
+    scene_steps = 1.0 / mean_t
+    desc = (f"{used} scenes (one per size bin of {S}), 1 step each from the GPU's post-benchmark state, "
+            f"{spent:.1f} s of CPU work; FP32 oracle, natural order, 1 thread")
+    return scene_steps, desc, per
+
+
+def plain_gpu_rate(world, steps=3, lanes=8):
+    """Plain per-scene GPU step: each sampled scene alone in its own context."""
+    from skyephyx_amd.capi import Context
+
+    ids, weights = stratified_sample(world, 16, seed=5)
+    t_w, total_w = 0.0, 0
+    for sid, wgt in zip(ids, weights):
+        sub = world.extract_scene(sid)
+        with Context(lanes=lanes) as ctx:
+            ctx.upload(sub)
+            ctx.step(1)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.step(steps)
+            ctx.synchronize()
+            dt = (time.perf_counter() - t0) / steps
+        t_w += wgt * dt
+        total_w += wgt
+    return 1.0 / (t_w / total_w)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="mix")
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: only one GPU is supported in this build (asked for --gpus {args.gpus}); "
+              "scenes are independent, so a multi-GPU run would shard whole scenes across GPUs", file=sys.stderr)
+        sys.exit(2)
+
+    from skyephyx_amd import roofline
+    from skyephyx_amd.capi import Context
+
+    t0 = time.perf_counter()
+    world, desc = make_workload(args.workload)
+    log(f"[bench] workload {args.workload}: {len(world.scenes)} scenes, {len(world.bodies)} bodies, "
+        f"{len(world.tets)} tets, built in {time.perf_counter() - t0:.1f}s")
+    ctx = Context(lanes=args.lanes)
+    ctx.upload(world)
+    log("[bench] uploaded")
+    for i in range(args.warmup):
+        ctx.step(1)
+    ctx.synchronize()
+    log(f"[bench] warmup {args.warmup} done")
+    ctx.set_option("kernel_timing", 1)
+    ctx.kernel_stats("primal", reset=True)
+    ctx.kernel_stats("dual", reset=True)
+    step_ms = []
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        a = time.perf_counter()
+        ctx.step(1)
+        step_ms.append((time.perf_counter() - a) * 1e3)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t_start
+    n_primal, ms_primal = ctx.kernel_stats("primal")
+    n_dual, ms_dual = ctx.kernel_stats("dual")
+    ctx.set_option("kernel_timing", 0)
+    timings = ctx.timings()
+    state = ctx.download()
+    em, _ = ctx.status(len(state.scenes))
+    log(f"[bench] timed {args.steps} steps in {elapsed:.3f}s; emergency scenes: {int(em.sum())}")
+
+    S = len(state.scenes)
+    scene_steps = S * args.steps / elapsed
+    primal_bytes, dual_bytes = roofline.step_bytes(state)
+    updates = roofline.body_updates_per_step(state)
+    achieved = primal_bytes * args.steps / (ms_primal * 1e-3) / 1e9 if ms_primal > 0 else 0.0
+    res = {
+        "metric": METRIC,
+        "value": round(scene_steps, 3),
+        "unit": "scene-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded scene generators)",
+        "config": {
+            "workload": desc,
+            "n_scenes": S,
+            "n_bodies": int(len(state.bodies)),
+            "n_dynamic": int(((state.bodies["flags"] & 1) == 0).sum()),
+            "n_tets": int(len(state.tets)),
+            "n_manifolds": int(len(state.manifolds)),
+            "lanes_per_body": args.lanes,
+            "colours_last_step": int(timings["n_colours"]),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_primal (per-colour block-solve sweep)",
+            "achieved": round(achieved, 2),
+            "peak": roofline.HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / roofline.HBM_PEAK_GBPS, 5),
+            "traffic": measured_traffic(args.workload, args.lanes),
+            "launches": n_primal,
+            "avg_launch_us": round(ms_primal * 1e3 / max(n_primal, 1), 3),
+            "algorithmic_bytes_per_launch": int(primal_bytes * args.steps / max(n_primal, 1)),
+        },
+        "p50_step_ms": round(float(np.percentile(step_ms, 50)), 4),
+        "p99_step_ms": round(float(np.percentile(step_ms, 99)), 4),
+        "body_vertex_updates_per_s": round(updates * args.steps / elapsed, 1),
+        "phase_ms_last_step": {k: round(v, 4) for k, v in timings.items() if k.endswith("_ms")},
+        "dual_avg_launch_us": round(ms_dual * 1e3 / max(n_dual, 1), 3),
+    }
+    if not args.no_plain and S > 1:
+        log("[bench] plain per-scene comparison")
+        plain = plain_gpu_rate(state, lanes=args.lanes)
+        res["plain_scene_steps_per_s"] = round(plain, 3)
+        res["batched_over_plain"] = round(scene_steps / plain, 3)
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        cpu_rate, cpu_desc, per = cpu_baseline(state)
+        for sid, nb, nt, dt in per:
+            log(f"[bench]   cpu scene {sid}: {nb} bodies {nt} tets {dt * 1e3:.1f} ms")
+        res["cpu_baseline"] = {
+            "value": round(cpu_rate, 4),
+            "unit": "scene-steps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": cpu_desc,
+            "host": platform.processor() or platform.machine(),
+        }
+        res["gpu_over_cpu"] = round(scene_steps / cpu_rate, 2)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -260,9 +260,17 @@ int avbd_get_timings(avbd_ctx* ctx, avbd_timings* out);
 int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
 
 /* Tuning knobs (no reference counterpart):
- *   "lanes"  lanes cooperating on one body's block solve: 1, 4, 8, 16 or 32
- *            (1 = the reference's sequential accumulation order). */
+ *   "lanes"          lanes cooperating on one body's block solve: 1, 4, 8,
+ *                    16 or 32 (1 = the reference's sequential accumulation
+ *                    order)
+ *   "kernel_timing"  1 = bracket every primal-sweep and dual launch with HIP
+ *                    events on the context's stream (see avbd_kernel_stats) */
 int avbd_set_option(avbd_ctx* ctx, const char* name, int64_t value);
+
+/* Accumulated launch count and device time (ms) of one kernel class since
+ * the last reset: kernel = "primal" or "dual".  reset != 0 clears after
+ * reading.  Requires the "kernel_timing" option. */
+int avbd_kernel_stats(avbd_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms, int32_t reset);
 
 #ifdef __cplusplus
 }

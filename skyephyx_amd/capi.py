@@ -49,6 +49,7 @@ def lib():
         L.avbd_get_timings.argtypes = [vp, C.POINTER(AvbdTimings)]
         L.avbd_get_colours.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.avbd_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
+        L.avbd_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int32]
         _lib = L
     return _lib
 
@@ -128,6 +129,13 @@ class Context:
         t = AvbdTimings()
         _check(self.L.avbd_get_timings(self.h, C.byref(t)))
         return {name: getattr(t, name) for name, _ in AvbdTimings._fields_}
+
+    def kernel_stats(self, kernel: str, reset: bool = False):
+        """(launches, total device ms) of 'primal' or 'dual' launches since the last reset."""
+        n = C.c_int64()
+        ms = C.c_double()
+        _check(self.L.avbd_kernel_stats(self.h, kernel.encode(), C.byref(n), C.byref(ms), 1 if reset else 0))
+        return int(n.value), float(ms.value)
 
     def colours(self, n: int) -> np.ndarray:
         out = np.zeros(n, np.int32)

@@ -58,6 +58,43 @@ struct avbd_ctx {
     hipEvent_t ev[8]{};
     avbd_timings timings{};
     int last_ncol = 0;
+    // per-launch timing of the primal and dual kernels (option kernel_timing)
+    bool ktiming = false;
+    std::vector<hipEvent_t> kev;
+    std::vector<int> kev_kind;  // 0 primal, 1 dual
+    int kev_used = 0;
+    long long kstat_n[2] = {0, 0};
+    double kstat_ms[2] = {0.0, 0.0};
+
+    hipEvent_t next_event() {
+        if (kev_used >= (int)kev.size()) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            kev.push_back(e);
+        }
+        return kev[kev_used++];
+    }
+    void mark_begin(int kind) {
+        if (!ktiming) return;
+        hipEventRecord(next_event(), stream);
+        kev_kind.push_back(kind);
+    }
+    void mark_end() {
+        if (!ktiming) return;
+        hipEventRecord(next_event(), stream);
+    }
+    // called after the stream has been synchronised
+    void collect_kernel_times() {
+        for (int i = 0; i + 1 < kev_used; i += 2) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, kev[i], kev[i + 1]);
+            int kind = kev_kind[i / 2];
+            kstat_n[kind] += 1;
+            kstat_ms[kind] += ms;
+        }
+        kev_used = 0;
+        kev_kind.clear();
+    }
 
     template <class T>
     int alloc(T** p, size_t count) {
@@ -75,6 +112,8 @@ struct avbd_ctx {
         if (stream) hipStreamSynchronize(stream);
         for (void* p : allocs) hipFree(p);
         allocs.clear();
+        kev_used = 0;
+        kev_kind.clear();
         w = DevWorld{};
         mb[0] = mb[1] = ManifoldBuf{};
         items = nullptr;
@@ -138,6 +177,7 @@ int build_incidence(avbd_ctx* c) {
 
 int launch_primal(avbd_ctx* c, int colour, int iter, int poststab, int grid) {
     const DevWorld& w = c->w;
+    c->mark_begin(0);
     switch (c->lanes) {
         case 1: hipLaunchKernelGGL(k_primal<1>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
         case 4: hipLaunchKernelGGL(k_primal<4>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
@@ -145,6 +185,7 @@ int launch_primal(avbd_ctx* c, int colour, int iter, int poststab, int grid) {
         case 16: hipLaunchKernelGGL(k_primal<16>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
         default: hipLaunchKernelGGL(k_primal<32>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
     }
+    c->mark_end();
     return 0;
 }
 
@@ -507,7 +548,9 @@ int do_step(avbd_ctx* c) {
     const int dgrid = grid_for((long long)w.Mcap * 8 + w.J + w.SP, 256, 4096);
     for (int it = 0; it < max_iters; ++it) {
         for (int col = 0; col < ncol; ++col) launch_primal(c, col, it, 0, pgrid);
+        c->mark_begin(1);
         hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
+        c->mark_end();
         if (w.J + w.SP > 0) {
             hipLaunchKernelGGL(k_fracture_reset, dim3(grid_for(2 * (w.J + w.SP) + 2)), dim3(256), 0, st, w);
             hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_FRACTURES, 0);
@@ -572,6 +615,7 @@ int avbd_destroy(avbd_ctx* c) {
     c->release();
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
+    for (auto& e : c->kev) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -586,7 +630,25 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
         c->lanes = int(value);
         return 0;
     }
+    if (n == "kernel_timing") {
+        c->ktiming = value != 0;
+        return 0;
+    }
     return fail(AVBD_E_ARG, "unknown option " + n);
+}
+
+int avbd_kernel_stats(avbd_ctx* c, const char* kernel, int64_t* launches, double* total_ms, int32_t reset) {
+    if (!c || !kernel) return fail(AVBD_E_ARG, "null argument");
+    std::string k(kernel);
+    int kind = k == "primal" ? 0 : (k == "dual" ? 1 : -1);
+    if (kind < 0) return fail(AVBD_E_ARG, "unknown kernel class " + k);
+    if (launches) *launches = c->kstat_n[kind];
+    if (total_ms) *total_ms = c->kstat_ms[kind];
+    if (reset) {
+        c->kstat_n[kind] = 0;
+        c->kstat_ms[kind] = 0.0;
+    }
+    return 0;
 }
 
 int avbd_upload(avbd_ctx* c, const avbd_world* w, int64_t manifold_capacity) {
@@ -606,6 +668,7 @@ static int step_impl(avbd_ctx* c, int32_t n_steps) {
         int g[G_WORDS];
         HIP_TRY(hipMemcpyAsync(g, c->w.g, sizeof(g), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
+        c->collect_kernel_times();
         if (g[G_ANY_OOB]) {
             hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, c->stream, c->w.g + G_ANY_OOB, 0);
             hipStreamSynchronize(c->stream);

@@ -557,13 +557,26 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
     }
 }
 
-__global__ void k_colour_hist(DevWorld w) {
-    GRID_STRIDE(b, w.N) {
-        if (is_static(w, b)) continue;
-        if (!scene_runs(w, w.scene[b])) continue;
-        int c = w.colour[b];
-        if (c >= 0 && c < AVBD_MAX_COLOURS) atomicAdd(&w.col_cnt[c], 1);
+// Colour of body b if it takes part in this step's sweeps, else -1.
+AVBD_DEV int sweep_colour(const DevWorld& w, int b) {
+    if (b >= w.N || is_static(w, b) || !scene_runs(w, w.scene[b])) return -1;
+    int c = w.colour[b];
+    return (c >= 0 && c < AVBD_MAX_COLOURS) ? c : -1;
+}
+
+// Colour histogram: LDS counters per block, one global atomic per block and
+// colour (a single global counter per colour would serialise millions of
+// atomics).
+__global__ __launch_bounds__(256) void k_colour_hist(DevWorld w) {
+    __shared__ int h[AVBD_MAX_COLOURS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < w.N; b += blockDim.x * gridDim.x) {
+        int c = sweep_colour(w, b);
+        if (c >= 0) atomicAdd(&h[c], 1);
     }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&w.col_cnt[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(256) void k_colour_scan(DevWorld w) {
@@ -585,12 +598,23 @@ __global__ __launch_bounds__(256) void k_colour_scan(DevWorld w) {
     w.col_cnt[t] = 0;
 }
 
-__global__ void k_colour_fill(DevWorld w) {
-    GRID_STRIDE(b, w.N) {
-        if (is_static(w, b)) continue;
-        if (!scene_runs(w, w.scene[b])) continue;
-        int c = w.colour[b];
-        if (c >= 0 && c < AVBD_MAX_COLOURS) w.col_list[w.col_off[c] + atomicAdd(&w.col_cnt[c], 1)] = b;
+// Scatter bodies into per-colour lists: each block reserves one range per
+// colour with a single global atomic, then places its bodies with LDS atomics.
+// The order inside a colour is irrelevant (its bodies are independent).
+__global__ __launch_bounds__(256) void k_colour_fill(DevWorld w) {
+    __shared__ int h[AVBD_MAX_COLOURS];
+    __shared__ int base[AVBD_MAX_COLOURS];
+    for (int start = blockIdx.x * blockDim.x; start < w.N; start += blockDim.x * gridDim.x) {
+        h[threadIdx.x] = 0;
+        __syncthreads();
+        const int b = start + threadIdx.x;
+        const int c = sweep_colour(w, b);
+        int r = c >= 0 ? atomicAdd(&h[c], 1) : 0;
+        __syncthreads();
+        if (h[threadIdx.x]) base[threadIdx.x] = w.col_off[threadIdx.x] + atomicAdd(&w.col_cnt[threadIdx.x], h[threadIdx.x]);
+        __syncthreads();
+        if (c >= 0) w.col_list[base[c] + r] = b;
+        __syncthreads();
     }
 }
 
