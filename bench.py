@@ -56,7 +56,7 @@ def measured_traffic(workload, lanes):
     (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size), or
     None when no summary exists for this workload and lane count."""
     path = os.path.join(ROOT, "profiles", f"k_primal_batched_{workload}_hbm.md")
-    if workload != "mix" or lanes != 8 or not os.path.exists(path):
+    if workload != "mix" or lanes != 0 or not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
@@ -104,7 +104,7 @@ def cpu_baseline(world, budget_s=20.0):
     return scene_steps, desc, per
 
 
-def plain_gpu_rate(world, steps=3, lanes=8):
+def plain_gpu_rate(world, steps=3, lanes=8, fem_lanes=16):
     """Plain per-scene GPU step: each sampled scene alone in its own context."""
     from skyephyx_amd.capi import Context
 
@@ -112,7 +112,7 @@ def plain_gpu_rate(world, steps=3, lanes=8):
     t_w, total_w = 0.0, 0
     for sid, wgt in zip(ids, weights):
         sub = world.extract_scene(sid)
-        with Context(lanes=lanes) as ctx:
+        with Context(lanes=lanes, fem_lanes=fem_lanes) as ctx:
             ctx.upload(sub)
             ctx.step(1)
             ctx.synchronize()
@@ -131,7 +131,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="mix")
-    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per body with forces (0 = autotuned)")
+    ap.add_argument("--fem-lanes", type=int, default=0, help="lanes per FEM-only body (0 = autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene comparison")
     args = ap.parse_args()
@@ -147,7 +148,7 @@ def main():
     world, desc = make_workload(args.workload)
     log(f"[bench] workload {args.workload}: {len(world.scenes)} scenes, {len(world.bodies)} bodies, "
         f"{len(world.tets)} tets, built in {time.perf_counter() - t0:.1f}s")
-    ctx = Context(lanes=args.lanes)
+    ctx = Context(lanes=args.lanes, fem_lanes=args.fem_lanes)
     ctx.upload(world)
     log("[bench] uploaded")
     for i in range(args.warmup):
@@ -199,7 +200,9 @@ def main():
             "n_dynamic": int(((state.bodies["flags"] & 1) == 0).sum()),
             "n_tets": int(len(state.tets)),
             "n_manifolds": int(len(state.manifolds)),
-            "lanes_per_body": args.lanes,
+            "lanes_per_body": args.lanes or "autotuned",
+            "lanes_per_fem_vertex": args.fem_lanes or "autotuned",
+            "tuned_lanes": {f"{'fem' if k[0] else 'forces'}/2^{k[1]}": v for k, v in sorted(ctx.tune_table().items())},
             "colours_last_step": int(timings["n_colours"]),
         },
         "roofline": {
@@ -222,7 +225,7 @@ def main():
     }
     if not args.no_plain and S > 1:
         log("[bench] plain per-scene comparison")
-        plain = plain_gpu_rate(state, lanes=args.lanes)
+        plain = plain_gpu_rate(state, lanes=args.lanes, fem_lanes=args.fem_lanes)
         res["plain_scene_steps_per_s"] = round(plain, 3)
         res["batched_over_plain"] = round(scene_steps / plain, 3)
     if not args.no_cpu:

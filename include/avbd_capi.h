@@ -260,12 +260,21 @@ int avbd_get_timings(avbd_ctx* ctx, avbd_timings* out);
 int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
 
 /* Tuning knobs (no reference counterpart):
- *   "lanes"          lanes cooperating on one body's block solve: 1, 4, 8,
- *                    16 or 32 (1 = the reference's sequential accumulation
- *                    order)
+ *   "lanes"          lanes cooperating on one block solve of a body with
+ *                    forces: 1, 2, 4, 8, 16 or 32 (1 = the reference's
+ *                    sequential accumulation order), 0 = autotuned (default)
+ *   "fem_lanes"      the same for particles whose only terms are FEM
+ *                    energies
  *   "kernel_timing"  1 = bracket every primal-sweep and dual launch with HIP
  *                    events on the context's stream (see avbd_kernel_stats) */
 int avbd_set_option(avbd_ctx* ctx, const char* name, int64_t value);
+
+/* Autotuner table (no reference counterpart): the lane width chosen per
+ * (kernel class, log2 of bodies per launch) as JSON {"key": width, ...}.
+ * Writes at most len bytes (NUL-terminated) and returns the full length. */
+int avbd_tune_table(avbd_ctx* ctx, char* out, int32_t len);
+/* Seed the autotuner with a table written by avbd_tune_table. */
+int avbd_tune_load(avbd_ctx* ctx, const char* json);
 
 /* Accumulated launch count and device time (ms) of one kernel class since
  * the last reset: kernel = "primal" or "dual".  reset != 0 clears after

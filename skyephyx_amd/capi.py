@@ -50,6 +50,8 @@ def lib():
         L.avbd_get_colours.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.avbd_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
         L.avbd_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int32]
+        L.avbd_tune_table.argtypes = [vp, C.c_char_p, C.c_int32]
+        L.avbd_tune_load.argtypes = [vp, C.c_char_p]
         _lib = L
     return _lib
 
@@ -70,12 +72,14 @@ def _check(rc):
 class Context:
     """A device-resident world on one GPU."""
 
-    def __init__(self, device: int = 0, lanes: int | None = None):
+    def __init__(self, device: int = 0, lanes: int | None = None, fem_lanes: int | None = None):
         self.L = lib()
         self.h = C.c_void_p()
         _check(self.L.avbd_create(device, C.byref(self.h)))
         if lanes is not None:
             self.set_option("lanes", lanes)
+        if fem_lanes is not None:
+            self.set_option("fem_lanes", fem_lanes)
 
     def close(self):
         if self.h:
@@ -136,6 +140,22 @@ class Context:
         ms = C.c_double()
         _check(self.L.avbd_kernel_stats(self.h, kernel.encode(), C.byref(n), C.byref(ms), 1 if reset else 0))
         return int(n.value), float(ms.value)
+
+    def tune_table(self) -> dict:
+        """Autotuned lane widths: {(kernel class, log2 bodies per launch): lanes}."""
+        import json
+
+        n = self.L.avbd_tune_table(self.h, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.L.avbd_tune_table(self.h, buf, n + 1)
+        raw = json.loads(buf.value.decode() or "{}")
+        return {(int(k) // 64, int(k) % 64): v for k, v in raw.items()}
+
+    def tune_load(self, table: dict):
+        import json
+
+        raw = {str(cls * 64 + band): v for (cls, band), v in table.items()}
+        _check(self.L.avbd_tune_load(self.h, json.dumps(raw).encode()))
 
     def colours(self, n: int) -> np.ndarray:
         out = np.zeros(n, np.int32)

@@ -8,11 +8,23 @@
 // singular vectors), so these are pinned to tolerance, not to the bit.
 // Everything stays in registers: loops over matrix indices are unrolled and
 // run-time permutations are applied as predicated compile-time swaps.
+//
+// This is the compute-heavy part of the step (one SVD and one 3x3
+// eigensystem per tet visit), and nothing here feeds a discrete decision
+// that must match the CPU bit for bit, so it is built for throughput: FMA
+// contraction is allowed and the Jacobi rotation parameters and
+// normalisations use the hardware reciprocal / reciprocal-sqrt / sqrt
+// (v_rcp_f32, v_rsq_f32, v_sqrt_f32, ~1 ulp) instead of the correctly
+// rounded sequences.
 #pragma once
 
 #include "avbd_math.h"
 
 #define AVBD_FLT_EPS 1.1920929e-07f
+
+AVBD_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+AVBD_DEV float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+AVBD_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 AVBD_DEV void swap_cols(m3& a, int p, int q) {
 #pragma unroll
@@ -25,14 +37,16 @@ AVBD_DEV void swap_cols(m3& a, int p, int q) {
 
 // Jacobi rotation parameters that zero g_pq of a 2x2 symmetric block.
 AVBD_DEV void jacobi_cs(float app, float aqq, float apq, float& c, float& s) {
-    float theta = (aqq - app) / (2.f * apq);
-    float t = (theta >= 0.f ? 1.f : -1.f) / (fabsf(theta) + sqrtf(theta * theta + 1.f));
-    c = 1.f / sqrtf(t * t + 1.f);
+#pragma clang fp contract(fast)
+    float theta = (aqq - app) * frcp(2.f * apq);
+    float t = (theta >= 0.f ? 1.f : -1.f) * frcp(fabsf(theta) + fsqrt(theta * theta + 1.f));
+    c = frsq(t * t + 1.f);
     s = t * c;
 }
 
 // Symmetric 3x3 eigen-decomposition, ascending eigenvalues, vectors in columns.
 AVBD_DEV void sym_eig3(const m3& A, v3& w, m3& V) {
+#pragma clang fp contract(fast)
     m3 a = A;
     m3 v;
 #pragma unroll
@@ -99,6 +113,7 @@ AVBD_DEV void sym_eig2(float a00, float a01, float a11, float& w0, float& w1, fl
 // F V = U S by one-sided Jacobi; S descending; U, V proper rotations with the
 // sign moved into S(2) (math.cpp:22-32).
 AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
+#pragma clang fp contract(fast)
     m3 A = F;
     m3 W;
 #pragma unroll
@@ -113,11 +128,11 @@ AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
             const int q = (k == 0) ? 1 : 2;
             v3 ap = col(A, p), aq = col(A, q);
             float alpha = dot(ap, ap), beta = dot(aq, aq), gamma = dot(ap, aq);
-            if (fabsf(gamma) <= 2.f * AVBD_FLT_EPS * sqrtf(alpha * beta) || gamma == 0.f) continue;
+            if (gamma * gamma <= (4.f * AVBD_FLT_EPS * AVBD_FLT_EPS) * (alpha * beta) || gamma == 0.f) continue;
             rotated = true;
-            float zeta = (beta - alpha) / (2.f * gamma);
-            float t = (zeta >= 0.f ? 1.f : -1.f) / (fabsf(zeta) + sqrtf(1.f + zeta * zeta));
-            float c = 1.f / sqrtf(1.f + t * t);
+            float zeta = (beta - alpha) * frcp(2.f * gamma);
+            float t = (zeta >= 0.f ? 1.f : -1.f) * frcp(fabsf(zeta) + fsqrt(1.f + zeta * zeta));
+            float c = frsq(1.f + t * t);
             float s = c * t;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
@@ -140,14 +155,14 @@ AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
     }
     const float tiny = 1.17549435e-38f * 1e6f;
     v3 u0 = col(A, 0), u1 = col(A, 1), u2 = col(A, 2);
-    if (s0 > tiny) u0 = vdiv(u0, s0); else u0 = v3{1.f, 0.f, 0.f};
-    if (s1 > tiny) u1 = vdiv(u1, s1);
+    if (s0 > tiny) u0 = mul(u0, frcp(s0)); else u0 = v3{1.f, 0.f, 0.f};
+    if (s1 > tiny) u1 = mul(u1, frcp(s1));
     else {
         v3 e = fabsf(u0.x) < 0.9f ? v3{1.f, 0.f, 0.f} : v3{0.f, 1.f, 0.f};
         u1 = cross(u0, e);
-        u1 = vdiv(u1, norm(u1));
+        u1 = mul(u1, frsq(dot(u1, u1)));
     }
-    if (s2 > tiny) u2 = vdiv(u2, s2); else u2 = cross(u0, u1);
+    if (s2 > tiny) u2 = mul(u2, frcp(s2)); else u2 = cross(u0, u1);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         U.m[r][0] = comp(u0, r);
@@ -192,6 +207,7 @@ AVBD_DEV void add_outer(float Hv[6], float l, v3 w) {
 // respect to the vertex whose shape gradient is gradN.
 AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmInv, float V0, float mu, float lam,
                                       v3 gradN, int projMode, v3& g, float H[6]) {
+#pragma clang fp contract(fast)
     m3 Ds;
     v3 e1 = sub(p1, p0), e2 = sub(p2, p0), e3 = sub(p3, p0);
     Ds.m[0][0] = e1.x; Ds.m[1][0] = e1.y; Ds.m[2][0] = e1.z;
@@ -209,7 +225,8 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
         return;
     }
     float logJ = logf(J);
-    float scal = (lam * logJ - mu) / J;
+    const float invJ = frcp(J);
+    float scal = (lam * logJ - mu) * invJ;
     m3 cof = cofactor(F);
     m3 P;
 #pragma unroll
@@ -221,17 +238,18 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
     v3 S;
     svd3(F, U, S, V);
     const float s0 = S.x, s1 = S.y, s2 = S.z;
-    float dPhi = (lam * logJ - mu) / J;
+    float dPhi = scal;
     float ev3 = mu + dPhi * s2, ev4 = mu + dPhi * s1, ev5 = mu + dPhi * s0;
     float ev6 = mu - dPhi * s2, ev7 = mu - dPhi * s1, ev8 = mu - dPhi * s0;
     float diagExtra = mu + lam * (1.f - logJ);
     m3 a;
-    a.m[0][0] = mu + diagExtra / (s0 * s0);
-    a.m[1][1] = mu + diagExtra / (s1 * s1);
-    a.m[2][2] = mu + diagExtra / (s2 * s2);
-    a.m[0][1] = a.m[1][0] = lam * s2 / J;
-    a.m[0][2] = a.m[2][0] = lam * s1 / J;
-    a.m[1][2] = a.m[2][1] = lam * s0 / J;
+    a.m[0][0] = mu + diagExtra * frcp(s0 * s0);
+    a.m[1][1] = mu + diagExtra * frcp(s1 * s1);
+    a.m[2][2] = mu + diagExtra * frcp(s2 * s2);
+    const float lJ = lam * invJ;
+    a.m[0][1] = a.m[1][0] = lJ * s2;
+    a.m[0][2] = a.m[2][0] = lJ * s1;
+    a.m[1][2] = a.m[2][1] = lJ * s0;
     v3 aw;
     m3 av;
     sym_eig3(a, aw, av);
@@ -262,6 +280,7 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
 // 2-D shape gradient (gn0, gn1).
 AVBD_DEV void stvk_vertex_terms(v3 p0, v3 p1, v3 p2, const float Dm[4], float area, float mu, float lam, float gn0,
                                 float gn1, int projMode, v3& g, float H[6]) {
+#pragma clang fp contract(fast)
     v3 e0 = sub(p1, p0), e1 = sub(p2, p0);
     v3 f0 = add(mul(e0, Dm[0]), mul(e1, Dm[2]));
     v3 f1 = add(mul(e0, Dm[1]), mul(e1, Dm[3]));

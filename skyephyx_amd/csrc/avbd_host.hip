@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "avbd_kernels.h"
+#include "avbd_tune.h"
 #include "avbd_world.h"
 
 namespace {
@@ -41,6 +42,8 @@ int grid_for(long long n, int block = 256, int max_blocks = 4096) {
 struct avbd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // second stream for the FEM bucket of a colour
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<void*> allocs;
     DevWorld w{};
     ManifoldBuf mb[2]{};
@@ -49,8 +52,11 @@ struct avbd_ctx {
     int n_items = 0;
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
-    int lanes = 8;
+    int lanes = 0;      // 0 = chosen by the tuner
+    int fem_lanes = 0;  // 0 = chosen by the tuner
+    LaneTuner tuner;
     bool have_world = false;
+    std::vector<int> bucket_off = std::vector<int>(AVBD_BUCKETS + 1, 0);
     // host mirrors of the static parts
     std::vector<avbd_scene> scenes;
     std::vector<int> sc_off;
@@ -175,15 +181,55 @@ int build_incidence(avbd_ctx* c) {
     return 0;
 }
 
-int launch_primal(avbd_ctx* c, int colour, int iter, int poststab, int grid) {
+template <bool FEM>
+void launch_primal_g(avbd_ctx* c, int G, int lo, int hi, int iter, int poststab) {
     const DevWorld& w = c->w;
+    const int grid = grid_for((long long)(hi - lo) * G, 256, 16384);
+    switch (G) {
+        case 1: hipLaunchKernelGGL((k_primal<1, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
+        case 2: hipLaunchKernelGGL((k_primal<2, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
+        case 4: hipLaunchKernelGGL((k_primal<4, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
+        case 8: hipLaunchKernelGGL((k_primal<8, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
+        case 16: hipLaunchKernelGGL((k_primal<16, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
+        default: hipLaunchKernelGGL((k_primal<32, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
+    }
+}
+
+// One colour of a sweep: the bodies with forces and the FEM-only bodies, two
+// instantiations that run concurrently (the FEM bucket on the side stream,
+// forked from and joined back into the main stream with events, so the next
+// colour still sees both).  Empty buckets are not launched.  The kernel
+// timing brackets the whole colour.
+// One bucket launch on c->stream with the lane width fixed by the user or
+// picked by the tuner (class 0 = bodies with forces, 1 = FEM-only).
+template <bool FEM>
+void launch_bucket(avbd_ctx* c, int lo, int hi, int iter, int poststab) {
+    int fixed = FEM ? c->fem_lanes : c->lanes;
+    int measure = -1;
+    int G = fixed > 0 ? fixed : c->tuner.choose(FEM ? 1 : 0, hi - lo, &measure);
+    if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].a, c->stream);
+    launch_primal_g<FEM>(c, G, lo, hi, iter, poststab);
+    if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].b, c->stream);
+}
+
+int launch_colour(avbd_ctx* c, const int* off, int colour, int iter, int poststab) {
+    const int g0 = off[2 * colour], g1 = off[2 * colour + 1], g2 = off[2 * colour + 2];
+    const bool gen = g1 > g0, fem = g2 > g1;
+    if (!gen && !fem) return 0;
     c->mark_begin(0);
-    switch (c->lanes) {
-        case 1: hipLaunchKernelGGL(k_primal<1>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
-        case 4: hipLaunchKernelGGL(k_primal<4>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
-        case 8: hipLaunchKernelGGL(k_primal<8>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
-        case 16: hipLaunchKernelGGL(k_primal<16>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
-        default: hipLaunchKernelGGL(k_primal<32>, dim3(grid), dim3(256), 0, c->stream, w, colour, iter, poststab); break;
+    if (gen && fem) {
+        HIP_TRY(hipEventRecord(c->fork, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->side, c->fork, 0));
+        std::swap(c->stream, c->side);
+        launch_bucket<true>(c, g1, g2, iter, poststab);
+        std::swap(c->stream, c->side);
+        launch_bucket<false>(c, g0, g1, iter, poststab);
+        HIP_TRY(hipEventRecord(c->join, c->side));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->join, 0));
+    } else if (gen) {
+        launch_bucket<false>(c, g0, g1, iter, poststab);
+    } else {
+        launch_bucket<true>(c, g1, g2, iter, poststab);
     }
     c->mark_end();
     return 0;
@@ -271,7 +317,7 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     A(&e_ent, (size_t)4 * T + 3 * TR + 1);
     w.e_off = e_off;
     w.e_ent = e_ent;
-    A(&w.colour, N); A(&w.col_cnt, AVBD_MAX_COLOURS + 1); A(&w.col_off, AVBD_MAX_COLOURS + 1); A(&w.col_list, N);
+    A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
     A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1); A(&w.seg_off, N + 1);
     w.pair_cap = std::max(65536, 4 * N);
     A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
@@ -528,12 +574,14 @@ int do_step(avbd_ctx* c) {
     w.m = c->mb[c->cur];
     if ((rc = build_incidence(c))) return rc;
     hipLaunchKernelGGL(k_colour, dim3(w.S), dim3(1024), 0, st, w);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, w.col_cnt, AVBD_MAX_COLOURS + 1, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, w.col_cnt, AVBD_BUCKETS + 1, 0);
     hipLaunchKernelGGL(k_colour_hist, dim3(grid_for(N)), dim3(256), 0, st, w);
-    hipLaunchKernelGGL(k_colour_scan, dim3(1), dim3(AVBD_MAX_COLOURS), 0, st, w);
+    hipLaunchKernelGGL(k_colour_scan, dim3(1), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_colour_fill, dim3(grid_for(N)), dim3(256), 0, st, w);
     int g[G_WORDS];
+    int* off = c->bucket_off.data();
     HIP_TRY(hipMemcpyAsync(g, w.g, sizeof(g), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(off, w.col_off, (AVBD_BUCKETS + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipGetLastError());
     if ((rc = check_errors(c, g))) return rc;
@@ -544,10 +592,9 @@ int do_step(avbd_ctx* c) {
     hipLaunchKernelGGL(k_predict, dim3(grid_for(N)), dim3(256), 0, st, w);
     HIP_TRY(hipEventRecord(c->ev[3], st));
     // ---- 5. iterations
-    const int pgrid = grid_for((long long)N * c->lanes, 256, 2048);
-    const int dgrid = grid_for((long long)w.Mcap * 8 + w.J + w.SP, 256, 4096);
+    const int dgrid = grid_for((long long)g[G_MANIFOLDS] * 8 + w.J + w.SP, 256, 8192);
     for (int it = 0; it < max_iters; ++it) {
-        for (int col = 0; col < ncol; ++col) launch_primal(c, col, it, 0, pgrid);
+        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, it, 0);
         c->mark_begin(1);
         hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
         c->mark_end();
@@ -562,7 +609,7 @@ int do_step(avbd_ctx* c) {
     HIP_TRY(hipEventRecord(c->ev[5], st));
     // ---- 7. post-stabilisation
     if (any_post) {
-        for (int col = 0; col < ncol; ++col) launch_primal(c, col, -1, 1, pgrid);
+        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, -1, 1);
         hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, st, w);
     }
     HIP_TRY(hipEventRecord(c->ev[6], st));
@@ -605,6 +652,12 @@ int avbd_create(int32_t device, avbd_ctx** out) {
         return fail(AVBD_E_HIP, "hipStreamCreate failed");
     }
     for (auto& e : c->ev) hipEventCreate(&e);
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        avbd_destroy(c);
+        return fail(AVBD_E_HIP, "stream/event creation failed");
+    }
     *out = c;
     return 0;
 }
@@ -616,6 +669,10 @@ int avbd_destroy(avbd_ctx* c) {
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     for (auto& e : c->kev) hipEventDestroy(e);
+    if (c->side) hipStreamSynchronize(c->side);
+    if (c->fork) hipEventDestroy(c->fork);
+    if (c->join) hipEventDestroy(c->join);
+    if (c->side) hipStreamDestroy(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -624,10 +681,10 @@ int avbd_destroy(avbd_ctx* c) {
 int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
     if (!c || !name) return fail(AVBD_E_ARG, "null argument");
     std::string n(name);
-    if (n == "lanes") {
-        if (value != 1 && value != 4 && value != 8 && value != 16 && value != 32)
-            return fail(AVBD_E_ARG, "lanes must be 1, 4, 8, 16 or 32");
-        c->lanes = int(value);
+    if (n == "lanes" || n == "fem_lanes") {
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+            return fail(AVBD_E_ARG, n + " must be 0 (tuned), 1, 2, 4, 8, 16 or 32");
+        (n == "lanes" ? c->lanes : c->fem_lanes) = int(value);
         return 0;
     }
     if (n == "kernel_timing") {
@@ -635,6 +692,22 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
         return 0;
     }
     return fail(AVBD_E_ARG, "unknown option " + n);
+}
+
+int avbd_tune_table(avbd_ctx* c, char* out, int32_t len) {
+    if (!c) return fail(AVBD_E_ARG, "null context");
+    std::string s = c->tuner.to_json();
+    if (out && len > 0) {
+        std::strncpy(out, s.c_str(), (size_t)len - 1);
+        out[len - 1] = 0;
+    }
+    return (int)s.size();
+}
+
+int avbd_tune_load(avbd_ctx* c, const char* json) {
+    if (!c || !json) return fail(AVBD_E_ARG, "null argument");
+    c->tuner.from_json(json);
+    return 0;
 }
 
 int avbd_kernel_stats(avbd_ctx* c, const char* kernel, int64_t* launches, double* total_ms, int32_t reset) {
@@ -669,6 +742,7 @@ static int step_impl(avbd_ctx* c, int32_t n_steps) {
         HIP_TRY(hipMemcpyAsync(g, c->w.g, sizeof(g), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         c->collect_kernel_times();
+        c->tuner.collect();
         if (g[G_ANY_OOB]) {
             hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, c->stream, c->w.g + G_ANY_OOB, 0);
             hipStreamSynchronize(c->stream);

@@ -558,62 +558,70 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
 }
 
 // Colour of body b if it takes part in this step's sweeps, else -1.
-AVBD_DEV int sweep_colour(const DevWorld& w, int b) {
+// Sweep bucket of body b (2 * colour + 1 for bodies whose only terms are FEM
+// energies, 2 * colour otherwise), or -1 if it takes no part in the sweeps.
+AVBD_DEV int sweep_bucket(const DevWorld& w, int b) {
     if (b >= w.N || is_static(w, b) || !scene_runs(w, w.scene[b])) return -1;
     int c = w.colour[b];
-    return (c >= 0 && c < AVBD_MAX_COLOURS) ? c : -1;
+    if (c < 0 || c >= AVBD_MAX_COLOURS) return -1;
+    const float4 I = w.inertia[b];
+    const bool fem_only = w.f_off[b + 1] == w.f_off[b] && w.e_off[b + 1] > w.e_off[b] && I.x == 0.f && I.y == 0.f &&
+                          I.z == 0.f;
+    return 2 * c + (fem_only ? 1 : 0);
 }
 
-// Colour histogram: LDS counters per block, one global atomic per block and
-// colour (a single global counter per colour would serialise millions of
+// Bucket histogram: LDS counters per block, one global atomic per block and
+// bucket (a single global counter per colour would serialise millions of
 // atomics).
 __global__ __launch_bounds__(256) void k_colour_hist(DevWorld w) {
-    __shared__ int h[AVBD_MAX_COLOURS];
-    h[threadIdx.x] = 0;
+    __shared__ int h[AVBD_BUCKETS];
+    for (int k = threadIdx.x; k < AVBD_BUCKETS; k += blockDim.x) h[k] = 0;
     __syncthreads();
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < w.N; b += blockDim.x * gridDim.x) {
-        int c = sweep_colour(w, b);
-        if (c >= 0) atomicAdd(&h[c], 1);
+        int k = sweep_bucket(w, b);
+        if (k >= 0) atomicAdd(&h[k], 1);
     }
     __syncthreads();
-    if (h[threadIdx.x]) atomicAdd(&w.col_cnt[threadIdx.x], h[threadIdx.x]);
+    for (int k = threadIdx.x; k < AVBD_BUCKETS; k += blockDim.x)
+        if (h[k]) atomicAdd(&w.col_cnt[k], h[k]);
 }
 
 __global__ __launch_bounds__(256) void k_colour_scan(DevWorld w) {
-    __shared__ int buf[AVBD_MAX_COLOURS];
-    const int t = threadIdx.x;
-    buf[t] = w.col_cnt[t];
+    __shared__ int buf[AVBD_BUCKETS];
+    for (int k = threadIdx.x; k < AVBD_BUCKETS; k += blockDim.x) buf[k] = w.col_cnt[k];
     __syncthreads();
-    if (t == 0) {
+    if (threadIdx.x == 0) {
         int acc = 0, last = -1;
-        for (int c = 0; c < AVBD_MAX_COLOURS; ++c) {
-            w.col_off[c] = acc;
-            if (buf[c] > 0) last = c;
-            acc += buf[c];
+        for (int k = 0; k < AVBD_BUCKETS; ++k) {
+            w.col_off[k] = acc;
+            if (buf[k] > 0) last = k / 2;
+            acc += buf[k];
         }
-        w.col_off[AVBD_MAX_COLOURS] = acc;
+        w.col_off[AVBD_BUCKETS] = acc;
         w.g[G_NCOLOURS] = last + 1;
     }
     __syncthreads();
-    w.col_cnt[t] = 0;
+    for (int k = threadIdx.x; k < AVBD_BUCKETS; k += blockDim.x) w.col_cnt[k] = 0;
 }
 
-// Scatter bodies into per-colour lists: each block reserves one range per
-// colour with a single global atomic, then places its bodies with LDS atomics.
-// The order inside a colour is irrelevant (its bodies are independent).
+// Scatter bodies into the bucket lists: each block reserves one range per
+// bucket with a single global atomic, then places its bodies with LDS
+// atomics.  The order inside a colour is irrelevant (its bodies are
+// independent).
 __global__ __launch_bounds__(256) void k_colour_fill(DevWorld w) {
-    __shared__ int h[AVBD_MAX_COLOURS];
-    __shared__ int base[AVBD_MAX_COLOURS];
+    __shared__ int h[AVBD_BUCKETS];
+    __shared__ int base[AVBD_BUCKETS];
     for (int start = blockIdx.x * blockDim.x; start < w.N; start += blockDim.x * gridDim.x) {
-        h[threadIdx.x] = 0;
+        for (int k = threadIdx.x; k < AVBD_BUCKETS; k += blockDim.x) h[k] = 0;
         __syncthreads();
         const int b = start + threadIdx.x;
-        const int c = sweep_colour(w, b);
-        int r = c >= 0 ? atomicAdd(&h[c], 1) : 0;
+        const int kb = sweep_bucket(w, b);
+        int r = kb >= 0 ? atomicAdd(&h[kb], 1) : 0;
         __syncthreads();
-        if (h[threadIdx.x]) base[threadIdx.x] = w.col_off[threadIdx.x] + atomicAdd(&w.col_cnt[threadIdx.x], h[threadIdx.x]);
+        for (int k = threadIdx.x; k < AVBD_BUCKETS; k += blockDim.x)
+            if (h[k]) base[k] = w.col_off[k] + atomicAdd(&w.col_cnt[k], h[k]);
         __syncthreads();
-        if (c >= 0) w.col_list[base[c] + r] = b;
+        if (kb >= 0) w.col_list[base[kb] + r] = b;
         __syncthreads();
     }
 }
@@ -913,11 +921,23 @@ AVBD_DEV void accumulate_force(const DevWorld& w, uint32_t e, float alpha, const
     }
 }
 
+AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int projMode, v3& g, float H[6]);
+
 AVBD_DEV void accumulate_energy(const DevWorld& w, uint32_t e, int projMode, Acc& A) {
-    const uint32_t idx = ENT_INDEX(e);
-    const int local = EENT_LOCAL(e);
     v3 g;
     float H[6];
+    energy_terms(w, e, projMode, g, H);
+    A.r[0] = A.r[0] + g.x;
+    A.r[1] = A.r[1] + g.y;
+    A.r[2] = A.r[2] + g.z;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) A.L[i] = A.L[i] + H[i];
+}
+
+// Gradient and packed lower 3x3 Hessian of one energy entry for its vertex.
+AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int projMode, v3& g, float H[6]) {
+    const uint32_t idx = ENT_INDEX(e);
+    const int local = EENT_LOCAL(e);
     if (!EENT_TRI(e)) {
         const int4 tv = w.t_v[idx];
         const float4 d0 = w.t_d[idx * 3 + 0], d1 = w.t_d[idx * 3 + 1], d2 = w.t_d[idx * 3 + 2];
@@ -940,11 +960,6 @@ AVBD_DEV void accumulate_energy(const DevWorld& w, uint32_t e, int projMode, Acc
         stvk_vertex_terms(f3(w.pos[tv.x]), f3(w.pos[tv.y]), f3(w.pos[tv.z]), Dm, d1.x, d1.y, d1.z, gn0, gn1,
                           projMode, g, H);
     }
-    A.r[0] = A.r[0] + g.x;
-    A.r[1] = A.r[1] + g.y;
-    A.r[2] = A.r[2] + g.z;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) A.L[i] = A.L[i] + H[i];
 }
 
 // One primal block update per body of colour `c` (solver.cpp:392-448 and
@@ -953,9 +968,81 @@ AVBD_DEV void accumulate_energy(const DevWorld& w, uint32_t e, int projMode, Acc
 // a butterfly over the lane group, and every lane of the group solves the
 // 6x6 system (the result is written by the group's first lane).  G = 1 is the
 // sequential order of the reference loop.
+//
+// The list [lo, hi) is one sweep bucket of the colour.  FEM = true is the
+// instantiation for bodies whose only terms are FEM energies: it drops the
+// contact/joint/spring code and so needs far fewer registers.
+// Block update of FEM-only particles (no forces this step, zero inertia):
+// the system is blockdiag(A, 0), so only the translational 3x3 is
+// accumulated (9 values through the butterfly instead of 27) and solved with
+// the 3x3 pivoted LDL^T, which performs exactly the operations the 6x6 one
+// performs on that block; the rotation increment is exactly zero.
 template <int G>
-__global__ __launch_bounds__(256) void k_primal(DevWorld w, int c, int iter, int poststab) {
-    const int lo = w.col_off[c], hi = w.col_off[c + 1];
+AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int poststab) {
+    const int lane = threadIdx.x % G;
+    const int gpb = blockDim.x / G;
+    for (int base = blockIdx.x * gpb; base < hi - lo; base += gridDim.x * gpb) {
+        const int item = base + threadIdx.x / G;
+        if (item >= hi - lo) continue;  // lane groups never straddle a wave: the group leaves together
+        const int b = w.col_list[lo + item];
+        const int s = w.scene[b];
+        const avbd_scene& sc = w.sc[s];
+        if (w.st[s * ST_WORDS + ST_EMERGENCY]) continue;
+        if (poststab ? !sc.post_stabilization : iter >= sc.iterations) continue;
+        const float dt2 = sc.dt * sc.dt;
+        float L[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float r[3] = {0.f, 0.f, 0.f};
+        const v3 p = f3(w.pos[b]);
+        if (lane == 0) {
+            const float ml = w.vel[b].w / dt2;
+            const v3 dl = sub(p, f3(w.inert_pos[b]));
+            L[0] = ml; L[2] = ml; L[5] = ml;
+            r[0] = ml * dl.x; r[1] = ml * dl.y; r[2] = ml * dl.z;
+        }
+        const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
+        for (int k = e0 + lane; k < e1; k += G) {
+            v3 g;
+            float H[6];
+            energy_terms(w, w.e_ent[k], sc.projection_mode, g, H);
+            r[0] = r[0] + g.x;
+            r[1] = r[1] + g.y;
+            r[2] = r[2] + g.z;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) L[i] = L[i] + H[i];
+        }
+        if (G > 1) {
+#pragma unroll
+            for (int off = 1; off < G; off <<= 1) {
+#pragma unroll
+                for (int i = 0; i < 6; ++i) L[i] = L[i] + __shfl_xor(L[i], off, G);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) r[i] = r[i] + __shfl_xor(r[i], off, G);
+            }
+        }
+        if (lane == 0) {
+            float dx[3];
+            ldlt3_solve(L, r, dx);
+            const qt q = fq(w.rot[b]);
+            const qt lr = fq(w.last_rot[b]);
+            v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
+            qt dq = quaternion_from_difference(v3{0.f, 0.f, 0.f}, -1.f);
+            qt nq = qnormalized(qmul(dq, q));
+            w.pos[b] = v4(np, 0.f);
+            w.rot[b] = q4(nq);
+            v3 dl = sub(np, f3(w.last_pos[b]));
+            v3 da = rotation_difference(nq, lr);
+            w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
+            w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
+        }
+    }
+}
+
+template <int G, bool FEM>
+__global__ __launch_bounds__(256) void k_primal(DevWorld w, int lo, int hi, int iter, int poststab) {
+    if constexpr (FEM) {
+        primal_fem_body<G>(w, lo, hi, iter, poststab);
+        return;
+    }
     const int lane = threadIdx.x % G;
     const int groups_per_block = blockDim.x / G;
     for (int base = blockIdx.x * groups_per_block; base < hi - lo; base += gridDim.x * groups_per_block) {
@@ -1009,8 +1096,10 @@ __global__ __launch_bounds__(256) void k_primal(DevWorld w, int c, int iter, int
                 for (int i = 0; i < 3; ++i)
                     A.r[3 + i] = ((Iw[i][0] / dt2) * da.x + (Iw[i][1] / dt2) * da.y) + (Iw[i][2] / dt2) * da.z;
             }
-            const int f0 = w.f_off[b], f1 = w.f_off[b + 1];
-            for (int k = f0 + lane; k < f1; k += G) accumulate_force(w, w.f_ent[k], alpha, dSelf, A);
+            if (!FEM) {
+                const int f0 = w.f_off[b], f1 = w.f_off[b + 1];
+                for (int k = f0 + lane; k < f1; k += G) accumulate_force(w, w.f_ent[k], alpha, dSelf, A);
+            }
             const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
             for (int k = e0 + lane; k < e1; k += G) accumulate_energy(w, w.e_ent[k], sc.projection_mode, A);
             if (G > 1) {
@@ -1039,11 +1128,15 @@ __global__ __launch_bounds__(256) void k_primal(DevWorld w, int c, int iter, int
     }
 }
 
-template __global__ void k_primal<1>(DevWorld, int, int, int);
-template __global__ void k_primal<4>(DevWorld, int, int, int);
-template __global__ void k_primal<8>(DevWorld, int, int, int);
-template __global__ void k_primal<16>(DevWorld, int, int, int);
-template __global__ void k_primal<32>(DevWorld, int, int, int);
+#define AVBD_PRIMAL_INST(G)                                                        \
+    template __global__ void k_primal<G, false>(DevWorld, int, int, int, int); \
+    template __global__ void k_primal<G, true>(DevWorld, int, int, int, int);
+AVBD_PRIMAL_INST(1)
+AVBD_PRIMAL_INST(2)
+AVBD_PRIMAL_INST(4)
+AVBD_PRIMAL_INST(8)
+AVBD_PRIMAL_INST(16)
+AVBD_PRIMAL_INST(32)
 
 // Dual / penalty update (solver.cpp:457-489), one thread per contact, joint
 // or spring.  A fracture disables the force and queues its bodies so that

@@ -157,6 +157,86 @@ AVBD_HD float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v
 // ---- packed lower-triangular 6x6 (21 entries, row-major lower) ----
 AVBD_HD int tri_idx(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
 
+// 3x3 version of ldlt6_solve below.  For a body whose system is
+// blockdiag(A, 0) (a particle with only FEM terms) the 6x6 factorisation
+// never pivots onto the zero block (pivots must be strictly larger), so it
+// performs exactly these operations on A and returns 0 for the rotation.
+AVBD_DEV void ldlt3_solve(const float* lower6, const float* b, float* x) {
+    float a[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            float v = lower6[tri_idx(i, j)];
+            a[i][j] = v;
+            a[j][i] = v;
+        }
+    int perm[3] = {0, 1, 2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int p = k;
+        float best = fabsf(a[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < 3; ++i) {
+            float v = fabsf(a[i][i]);
+            if (v > best) { best = v; p = i; }
+        }
+#pragma unroll
+        for (int q = k + 1; q < 3; ++q) {
+            if (p == q) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    float t = a[k][j];
+                    a[k][j] = a[q][j];
+                    a[q][j] = t;
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    float t = a[i][k];
+                    a[i][k] = a[i][q];
+                    a[i][q] = t;
+                }
+                int t = perm[k];
+                perm[k] = perm[q];
+                perm[q] = t;
+            }
+        }
+        float d = a[k][k];
+        if (fabsf(d) > 0.f) {
+            float l[3];
+#pragma unroll
+            for (int i = k + 1; i < 3; ++i) l[i] = a[i][k] / d;
+#pragma unroll
+            for (int i = k + 1; i < 3; ++i)
+#pragma unroll
+                for (int j = k + 1; j <= i; ++j) {
+                    a[i][j] = a[i][j] - l[i] * a[j][k];
+                    a[j][i] = a[i][j];
+                }
+#pragma unroll
+            for (int i = k + 1; i < 3; ++i) { a[i][k] = l[i]; a[k][i] = l[i]; }
+        } else {
+#pragma unroll
+            for (int i = k + 1; i < 3; ++i) { a[i][k] = 0.f; a[k][i] = 0.f; }
+        }
+    }
+    float y[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) y[k] = perm[k] == 0 ? b[0] : (perm[k] == 1 ? b[1] : b[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < i; ++j) y[i] = y[i] - a[i][j] * y[j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) y[i] = (fabsf(a[i][i]) > 1.17549435e-38f) ? y[i] / a[i][i] : 0.f;
+#pragma unroll
+    for (int i = 2; i >= 0; --i)
+#pragma unroll
+        for (int j = i + 1; j < 3; ++j) y[i] = y[i] - a[j][i] * y[j];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) x[q] = perm[0] == q ? y[0] : (perm[1] == q ? y[1] : y[2]);
+}
+
 // LDL^T with symmetric diagonal pivoting (largest remaining |diagonal|, first
 // on ties) and a pseudo-inverse solve (|D_i| <= FLT_MIN -> component 0): the
 // semantics of the LDLT in the reference primal (solver.cpp:437-438).

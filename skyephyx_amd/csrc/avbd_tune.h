@@ -1,0 +1,132 @@
+// avbd_tune.h -- autotuner for the launch shape of the primal sweep.
+//
+// The lanes per body (G) trade per-body latency (more lanes share a body's
+// incidence list) against redundant per-body work and reduction cost (every
+// lane of the group solves the block system).  Which wins depends on how
+// much parallel work a launch has: a 32^3 FEM block alone wants 16 lanes, the
+// 4096-scene mix wants 4.  The tuner keys on (kernel class, log2 of the
+// bodies in the launch), tries every candidate width a few times on live
+// launches (timed with HIP events on the stream that runs the kernel), and
+// then keeps the fastest.  The table can be written to and read from a small
+// JSON file so later runs start tuned.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <string>
+#include <vector>
+
+struct LaneTuner {
+    static constexpr int kNumCands = 6;
+    static constexpr int kCands[kNumCands] = {1, 2, 4, 8, 16, 32};
+    static constexpr int kSamples = 2;
+
+    struct Entry {
+        int best = 0;  // chosen width, 0 while tuning
+        int next = 0;  // next candidate index to sample
+        int count[kNumCands] = {0};
+        double us_per_body[kNumCands] = {0.0};
+    };
+    struct Pending {
+        int key;
+        int cand;
+        int n;
+        hipEvent_t a, b;
+    };
+
+    bool enabled = true;
+    std::map<int, Entry> table;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+
+    static int band(int n) {
+        int b = 0;
+        while ((1 << (b + 1)) <= n && b < 30) ++b;
+        return b;
+    }
+    static int key(int cls, int n) { return cls * 64 + band(n); }
+
+    ~LaneTuner() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+
+    hipEvent_t event() {
+        if (pool_used == pool.size()) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            pool.push_back(e);
+        }
+        return pool[pool_used++];
+    }
+
+    // Width to use for a launch of class cls over n bodies.  While the key is
+    // being tuned, *measure receives the index of a pending record whose
+    // events the caller records around the launch; else -1.
+    int choose(int cls, int n, int* measure) {
+        *measure = -1;
+        Entry& e = table[key(cls, n)];
+        if (!enabled || e.best) return e.best ? e.best : 8;
+        const int ci = e.next % kNumCands;
+        e.next++;
+        Pending p{key(cls, n), ci, n, event(), event()};
+        pending.push_back(p);
+        *measure = int(pending.size()) - 1;
+        return kCands[ci];
+    }
+
+    // After the stream(s) have been synchronised.
+    void collect() {
+        for (const Pending& p : pending) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) continue;
+            Entry& e = table[p.key];
+            if (e.best) continue;
+            e.count[p.cand]++;
+            e.us_per_body[p.cand] += 1e3 * ms / (p.n > 0 ? p.n : 1);
+            bool done = true;
+            for (int c = 0; c < kNumCands; ++c) done = done && e.count[c] >= kSamples;
+            if (done) {
+                int bi = 0;
+                for (int c = 1; c < kNumCands; ++c)
+                    if (e.us_per_body[c] / e.count[c] < e.us_per_body[bi] / e.count[bi]) bi = c;
+                e.best = kCands[bi];
+            }
+        }
+        pending.clear();
+        pool_used = 0;
+    }
+
+    std::string to_json() const {
+        std::string s = "{";
+        bool first = true;
+        for (const auto& kv : table) {
+            if (!kv.second.best) continue;
+            char buf[96];
+            std::snprintf(buf, sizeof(buf), "%s\"%d\": %d", first ? "" : ", ", kv.first, kv.second.best);
+            s += buf;
+            first = false;
+        }
+        return s + "}";
+    }
+
+    // Parses what to_json writes: {"key": width, ...}.
+    void from_json(const std::string& s) {
+        size_t i = 0;
+        while ((i = s.find('"', i)) != std::string::npos) {
+            size_t j = s.find('"', i + 1);
+            if (j == std::string::npos) break;
+            int k = std::atoi(s.substr(i + 1, j - i - 1).c_str());
+            size_t c = s.find(':', j);
+            if (c == std::string::npos) break;
+            int v = std::atoi(s.c_str() + c + 1);
+            bool ok = false;
+            for (int q = 0; q < kNumCands; ++q) ok = ok || kCands[q] == v;
+            if (ok) table[k].best = v;
+            i = c + 1;
+        }
+    }
+};

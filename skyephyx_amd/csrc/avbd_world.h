@@ -47,6 +47,9 @@
 #define G_WORDS 16
 
 #define AVBD_MAX_COLOURS 256
+// sweep lists: bucket 2c holds the colour-c bodies with forces (rigid bodies,
+// particles in contact), bucket 2c+1 those with only FEM energies
+#define AVBD_BUCKETS (2 * AVBD_MAX_COLOURS)
 #define CONTACT_HOT_F4 6   // float4 per contact (hot)
 #define CONTACT_COLD_F4 3  // float4 per contact (cold)
 
@@ -133,8 +136,8 @@ struct DevWorld {
     const uint32_t* e_ent;
     // colouring
     int* colour;        // [N]
-    int* col_cnt;       // [AVBD_MAX_COLOURS + 1]
-    int* col_off;       // [AVBD_MAX_COLOURS + 1]
+    int* col_cnt;       // [AVBD_BUCKETS + 1]
+    int* col_off;       // [AVBD_BUCKETS + 1]
     int* col_list;      // [N]
     // scratch for the colouring and the broad phase
     int* ccount;        // [N]

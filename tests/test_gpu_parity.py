@@ -52,12 +52,14 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("lanes", [1, 8])
+@pytest.mark.parametrize("lanes", [(1, 1), (8, 16), (0, 0)], ids=["seq", "coop", "tuned"])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_single_step_parity(case, lanes):
+    """lanes = (lanes per body with forces, lanes per FEM-only body); (1, 1)
+    accumulates in the reference's sequential order."""
     make, warm = CASES[case]
     state = warm_state(make(), warm)
-    ctx = Context(lanes=lanes)
+    ctx = Context(lanes=lanes[0], fem_lanes=lanes[1])
     ctx.upload(state)
     ctx.step(1)
     g = ctx.download()
@@ -74,7 +76,7 @@ def test_single_step_parity(case, lanes):
     res["colours_equal"] = bool(np.array_equal(gcol[dyn], ocol[dyn]))
     res["fp64_pos_rel_gpu"] = summary(g, ref64)["pos_rel"]
     res["fp64_pos_rel_oracle32"] = summary(ref, ref64)["pos_rel"]
-    _record(f"{case}/lanes{lanes}", res)
+    _record(f"{case}/lanes{lanes[0]}-{lanes[1]}", res)
     assert res["pairs_equal"], res
     assert res["n_equal"] and res["ids_equal"], res
     assert res["colours_equal"], res
