@@ -10,7 +10,10 @@ advanced with one step call over all scenes.  Prints ONE JSON line.
 
 value = whole-job scene-steps/s.  The line also carries the roofline of the
 dominant kernel (the primal block-solve sweep, timed per launch with HIP
-events on the library's stream) and a CPU baseline: the FP32 oracle (a
+events on the library's stream), the served form over the same scenes
+(request server with every scene resident under a 200 GB HBM budget,
+Poisson arrivals at --serve-load fractions of the batched rate: request
+latency p50/p99 and tick p99) and a CPU baseline: the FP32 oracle (a
 restatement of the reference's single-threaded Solver::Step) timed on one
 host core over a bounded, size-stratified sample of the same scenes.
 """
@@ -125,6 +128,36 @@ def plain_gpu_rate(world, steps=3, lanes=8, fem_lanes=16):
     return 1.0 / (t_w / total_w)
 
 
+def served(world, saturation, fractions, duration, budget=200e9):
+    """Served form over the same scenes: every scene registered and resident
+    under an HBM budget, Poisson request arrivals at fractions of the batched
+    saturation rate, each request U{1..8} steps of a random scene.  The
+    server keeps its default pinned lane widths (a scene's result does not
+    depend on the other scenes in its ticks)."""
+    from skyephyx_amd.serve import Server, poisson_run
+
+    mean_k = 4.5
+    out = []
+    with Server(budget_bytes=budget) as srv:
+        t0 = time.perf_counter()
+        ids = srv.add_scenes(world)
+        srv.preload()
+        st = srv.stats()
+        log(f"[serve] {len(ids)} scenes registered and resident ({st['resident_bytes'] / 1e9:.2f} GB of "
+            f"{budget / 1e9:.0f} GB budget) in {time.perf_counter() - t0:.1f}s")
+        # untimed warm-up
+        warm = poisson_run(srv, ids, 0.5 * saturation / mean_k, 2.0, steps=(1, 8), seed=99)
+        log(f"[serve] warm-up: {warm['requests']} requests")
+        for f in fractions:
+            rate = f * saturation / mean_k
+            r = poisson_run(srv, ids, rate, duration, steps=(1, 8), seed=int(f * 1000), log=log)
+            r["load_fraction"] = f
+            log(f"[serve] load {f:.2f}: {json.dumps(r)}")
+            out.append(r)
+        st = srv.stats()
+    return out, st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +168,10 @@ def main():
     ap.add_argument("--fem-lanes", type=int, default=0, help="lanes per FEM-only body (0 = autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson request) measurement")
+    ap.add_argument("--serve-load", type=float, nargs="+", default=[0.8],
+                    help="served load points as fractions of the batched saturation rate")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per served load point")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only one GPU is supported in this build (asked for --gpus {args.gpus}); "
@@ -173,6 +210,9 @@ def main():
     timings = ctx.timings()
     state = ctx.download()
     em, _ = ctx.status(len(state.scenes))
+    tune = ctx.tune_table()
+    ctx_bytes = ctx.memory()
+    ctx.close()
     log(f"[bench] timed {args.steps} steps in {elapsed:.3f}s; emergency scenes: {int(em.sum())}")
 
     S = len(state.scenes)
@@ -202,7 +242,7 @@ def main():
             "n_manifolds": int(len(state.manifolds)),
             "lanes_per_body": args.lanes or "autotuned",
             "lanes_per_fem_vertex": args.fem_lanes or "autotuned",
-            "tuned_lanes": {f"{'fem' if k[0] else 'forces'}/2^{k[1]}": v for k, v in sorted(ctx.tune_table().items())},
+            "tuned_lanes": {f"{'fem' if k[0] else 'forces'}/2^{k[1]}": v for k, v in sorted(tune.items())},
             "colours_last_step": int(timings["n_colours"]),
         },
         "roofline": {
@@ -228,6 +268,23 @@ def main():
         plain = plain_gpu_rate(state, lanes=args.lanes, fem_lanes=args.fem_lanes)
         res["plain_scene_steps_per_s"] = round(plain, 3)
         res["batched_over_plain"] = round(scene_steps / plain, 3)
+    res["device_bytes"] = ctx_bytes
+    if not args.no_serve and S > 1:
+        log("[bench] served form")
+        points, sst = served(state, scene_steps, args.serve_load, args.serve_seconds)
+        res["served"] = {
+            "hbm_budget_bytes": int(200e9),
+            "resident_bytes": int(sst["resident_bytes"]),
+            "resident_scenes": int(sst["resident_scenes"]),
+            "lanes": "pinned 4/4 (result independent of the mix)",
+            "request": "U{1..8} steps of a uniformly drawn scene, Poisson arrivals",
+            "saturation_scene_steps_per_s": round(scene_steps, 3),
+            "points": points,
+        }
+        main_pt = points[0]
+        res["served_p50_latency_ms"] = main_pt["latency_ms_p50"]
+        res["served_p99_latency_ms"] = main_pt["latency_ms_p99"]
+        res["served_p99_tick_ms"] = main_pt["tick_ms_p99"]
     if not args.no_cpu:
         log("[bench] CPU baseline")
         cpu_rate, cpu_desc, per = cpu_baseline(state)

@@ -29,6 +29,10 @@
  *                                (solver.hpp:10-19, solver.cpp:599-626)
  *   avbd_step_scenes          -> (no counterpart: batched/served stepping of a
  *                                subset of resident scenes)
+ *   avbd_server_*             -> (no counterpart: the served request loop)
+ *   avbd_footprint, avbd_get_memory, avbd_set_option, avbd_tune_*,
+ *   avbd_kernel_stats         -> (no counterpart: device memory, tuning and
+ *                                measurement)
  */
 #ifndef AVBD_CAPI_H
 #define AVBD_CAPI_H
@@ -221,7 +225,9 @@ typedef struct avbd_ctx avbd_ctx;
 
 /* Library identification; returns AVBD_ABI_VERSION. */
 int avbd_abi_version(void);
-/* sizeof of each record, for bindings to check their layouts. */
+/* sizeof of each record, for bindings to check their layouts: scene, body,
+ * joint, spring, contact, manifold, tet, tri, world, timings, request_info,
+ * server_stats.  Returns the number of records. */
 int avbd_record_sizes(int32_t* out, int32_t n);
 
 /* Message describing the last failure on the calling thread. */
@@ -280,6 +286,84 @@ int avbd_tune_load(avbd_ctx* ctx, const char* json);
  * the last reset: kernel = "primal" or "dual".  reset != 0 clears after
  * reading.  Requires the "kernel_timing" option. */
 int avbd_kernel_stats(avbd_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms, int32_t reset);
+
+/* Device memory (no reference counterpart).  avbd_footprint: the bytes an
+ * avbd_upload of `w` with this manifold_capacity allocates (host-only, needs
+ * no device).  avbd_get_memory: the bytes the context holds now. */
+int avbd_footprint(const avbd_world* w, int64_t manifold_capacity, int64_t* bytes);
+int avbd_get_memory(avbd_ctx* ctx, int64_t* bytes);
+
+/* ------------------------------------------------------------------------
+ * Served form (no reference counterpart): requests for k steps of a
+ * registered scene, continuous batching.
+ *
+ * Scenes are registered once (the server keeps a host copy).  A scene is
+ * resident when its state lives in the server's device world; residency is
+ * bounded by an HBM budget in bytes (the exact allocation of the device
+ * world, avbd_footprint) and idle scenes are evicted least recently used
+ * when another must come in.  Each avbd_server_tick admits queued requests
+ * in arrival order into free slots (at most max_slots in flight, at most one
+ * per scene), makes their scenes resident, advances every in-flight scene by
+ * one step in a single batched device step and retires the requests whose
+ * steps are done.  Requests submitted between ticks join at the next tick.
+ * A scene's result does not depend on which other scenes share the device
+ * world or its ticks: the server's context starts with fixed lane widths
+ * (options "lanes" = "fem_lanes" = 4), which makes it bit-identical.  With
+ * autotuned widths (0, set through avbd_server_context) the accumulation
+ * order follows the launch size and results agree only to rounding.  One
+ * thread drives a server.
+ */
+typedef struct avbd_server avbd_server;
+
+typedef struct avbd_request_info {
+    int64_t ticket;
+    int32_t scene;
+    int32_t steps;
+    int32_t state;            /* 0 queued, 1 running, 2 done */
+    int32_t pad_;
+    double t_arrival;         /* seconds on the server clock (avbd_server_now) */
+    double t_start;           /* start of the tick of the first step, -1 before */
+    double t_done;            /* end of the tick of the last step, -1 before */
+} avbd_request_info;
+
+typedef struct avbd_server_stats {
+    int64_t ticks;            /* ticks that stepped at least one scene */
+    int64_t scene_steps;      /* scenes advanced by one step, summed over ticks */
+    int64_t completed;
+    int64_t layout_rebuilds;  /* residency changes (download + upload) */
+    int64_t resident_bytes;   /* device bytes of the resident world */
+    int32_t resident_scenes;
+    int32_t queued;
+    int32_t running;
+    int32_t pad_;
+} avbd_server_stats;
+
+int avbd_server_create(int32_t device, int64_t hbm_budget_bytes, int32_t max_slots, avbd_server** out);
+int avbd_server_destroy(avbd_server* s);
+/* Register every scene of `w` (the server copies them); they get the
+ * consecutive ids first_id, first_id + 1, ...  Fails with AVBD_E_CAPACITY if
+ * a scene alone exceeds the budget (then none is registered). */
+int avbd_server_add_scenes(avbd_server* s, const avbd_world* w, int32_t* first_id);
+/* Make registered scenes resident now, in id order, while they fit. */
+int avbd_server_preload(avbd_server* s);
+/* Seconds since the server was created. */
+double avbd_server_now(avbd_server* s);
+/* Queue a request for n_steps steps of a scene; t_arrival < 0 means now. */
+int avbd_server_submit(avbd_server* s, int32_t scene_id, int32_t n_steps, double t_arrival, int64_t* ticket);
+/* One scheduling round: admit, one batched step, retire.  n_running gets the
+ * number of scenes stepped (0 when nothing was runnable). */
+int avbd_server_tick(avbd_server* s, int32_t* n_running);
+/* State of a request that has not been polled yet. */
+int avbd_server_request(avbd_server* s, int64_t ticket, avbd_request_info* out);
+/* Move up to cap finished requests (completion order) into out. */
+int avbd_server_poll(avbd_server* s, avbd_request_info* out, int32_t cap, int32_t* n_out);
+int avbd_server_get_stats(avbd_server* s, avbd_server_stats* out);
+/* Current state of one scene (local body indices, force serials renumbered
+ * by rank): call with every out->n_* = 0 to get the counts, then with arrays
+ * of at least those sizes. */
+int avbd_server_download_scene(avbd_server* s, int32_t scene_id, avbd_world* out);
+/* The server's solver context (options, autotuner, timings). */
+avbd_ctx* avbd_server_context(avbd_server* s);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,33 @@ class AvbdTimings(C.Structure):
     ]
 
 
+class AvbdRequestInfo(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_int64),
+        ("scene", C.c_int32),
+        ("steps", C.c_int32),
+        ("state", C.c_int32),
+        ("pad_", C.c_int32),
+        ("t_arrival", C.c_double),
+        ("t_start", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+
+class AvbdServerStats(C.Structure):
+    _fields_ = [
+        ("ticks", C.c_int64),
+        ("scene_steps", C.c_int64),
+        ("completed", C.c_int64),
+        ("layout_rebuilds", C.c_int64),
+        ("resident_bytes", C.c_int64),
+        ("resident_scenes", C.c_int32),
+        ("queued", C.c_int32),
+        ("running", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
 _NAMES = ("scenes", "bodies", "joints", "springs", "manifolds", "tets", "tris")
 _DTYPES = (SCENE_DTYPE, BODY_DTYPE, JOINT_DTYPE, SPRING_DTYPE, MANIFOLD_DTYPE, TET_DTYPE, TRI_DTYPE)
 

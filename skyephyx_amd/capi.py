@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .abi import AvbdTimings, AvbdWorld, alloc_world, world_to_c
+from .abi import AvbdRequestInfo, AvbdServerStats, AvbdTimings, AvbdWorld, alloc_world, world_to_c
 from .world import World
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -52,6 +52,22 @@ def lib():
         L.avbd_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int32]
         L.avbd_tune_table.argtypes = [vp, C.c_char_p, C.c_int32]
         L.avbd_tune_load.argtypes = [vp, C.c_char_p]
+        L.avbd_footprint.argtypes = [C.POINTER(AvbdWorld), C.c_int64, C.POINTER(C.c_int64)]
+        L.avbd_get_memory.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.avbd_server_create.argtypes = [C.c_int32, C.c_int64, C.c_int32, C.POINTER(vp)]
+        L.avbd_server_destroy.argtypes = [vp]
+        L.avbd_server_add_scenes.argtypes = [vp, C.POINTER(AvbdWorld), C.POINTER(C.c_int32)]
+        L.avbd_server_preload.argtypes = [vp]
+        L.avbd_server_now.argtypes = [vp]
+        L.avbd_server_now.restype = C.c_double
+        L.avbd_server_submit.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.POINTER(C.c_int64)]
+        L.avbd_server_tick.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.avbd_server_request.argtypes = [vp, C.c_int64, C.POINTER(AvbdRequestInfo)]
+        L.avbd_server_poll.argtypes = [vp, C.POINTER(AvbdRequestInfo), C.c_int32, C.POINTER(C.c_int32)]
+        L.avbd_server_get_stats.argtypes = [vp, C.POINTER(AvbdServerStats)]
+        L.avbd_server_download_scene.argtypes = [vp, C.c_int32, C.POINTER(AvbdWorld)]
+        L.avbd_server_context.argtypes = [vp]
+        L.avbd_server_context.restype = vp
         _lib = L
     return _lib
 
@@ -61,7 +77,16 @@ def exported_symbols():
     import re
 
     text = open(HEADER_PATH).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(avbd_\w+)\s*\(", text, re.M)))
+    pat = r"^\s*(?:int|double|const char\*|avbd_ctx\*)\s+(avbd_\w+)\s*\("
+    return sorted(set(re.findall(pat, text, re.M)))
+
+
+def footprint(world: World, manifold_capacity: int = 0) -> int:
+    """Device bytes an upload of `world` allocates (host-only computation)."""
+    keep, cw = world_to_c(world)
+    out = C.c_int64()
+    _check(lib().avbd_footprint(C.byref(cw), int(manifold_capacity), C.byref(out)))
+    return int(out.value)
 
 
 def _check(rc):
@@ -72,19 +97,29 @@ def _check(rc):
 class Context:
     """A device-resident world on one GPU."""
 
-    def __init__(self, device: int = 0, lanes: int | None = None, fem_lanes: int | None = None):
+    def __init__(self, device: int = 0, lanes: int | None = None, fem_lanes: int | None = None, _handle=None):
         self.L = lib()
         self.h = C.c_void_p()
-        _check(self.L.avbd_create(device, C.byref(self.h)))
+        self._owned = _handle is None
+        if _handle is None:
+            _check(self.L.avbd_create(device, C.byref(self.h)))
+        else:
+            self.h = C.c_void_p(_handle)
         if lanes is not None:
             self.set_option("lanes", lanes)
         if fem_lanes is not None:
             self.set_option("fem_lanes", fem_lanes)
 
     def close(self):
-        if self.h:
+        if self.h and self._owned:
             self.L.avbd_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
+
+    def memory(self) -> int:
+        """Device bytes the context holds."""
+        out = C.c_int64()
+        _check(self.L.avbd_get_memory(self.h, C.byref(out)))
+        return int(out.value)
 
     def __del__(self):
         try:

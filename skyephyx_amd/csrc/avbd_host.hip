@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "avbd_internal.h"
 #include "avbd_kernels.h"
 #include "avbd_tune.h"
 #include "avbd_world.h"
@@ -22,6 +23,13 @@ int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
+
+// Temporary storage given to the device-wide scans, sized for n items.  An
+// upper bound of the decoupled-lookback scan's needs (one 16-byte state per
+// 256 items plus padding); the upload takes the larger of this and the
+// library's own query, so the byte count of a world is known without a
+// device.
+size_t scan_tmp_bound(long long n) { return (size_t)(16 * (n / 256 + 64) + 1024); }
 
 #define HIP_TRY(expr)                                                                      \
     do {                                                                                   \
@@ -102,10 +110,19 @@ struct avbd_ctx {
         kev_kind.clear();
     }
 
+    // dry: allocations are only counted (world_device_bytes)
+    bool dry = false;
+    size_t alloc_bytes = 0;
+
     template <class T>
     int alloc(T** p, size_t count) {
         void* q = nullptr;
         size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        alloc_bytes += bytes;
+        if (dry) {
+            *p = nullptr;
+            return 0;
+        }
         hipError_t e = hipMalloc(&q, bytes);
         if (e != hipSuccess) return fail(AVBD_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
         e = hipMemsetAsync(q, 0, bytes, stream);
@@ -118,6 +135,7 @@ struct avbd_ctx {
         if (stream) hipStreamSynchronize(stream);
         for (void* p : allocs) hipFree(p);
         allocs.clear();
+        alloc_bytes = 0;
         kev_used = 0;
         kev_kind.clear();
         w = DevWorld{};
@@ -235,6 +253,79 @@ int launch_colour(avbd_ctx* c, const int* off, int colour, int iter, int poststa
     return 0;
 }
 
+// Sizes of every device array of a world.
+struct WorldDims {
+    int S = 0, N = 0, J = 0, SP = 0, T = 0, TR = 0, Mcap = 0;
+    long long n_items = 0;
+    size_t scan_bytes = 0;
+};
+
+int make_dims(const WorldCounts& k, long long mcap_req, WorldDims* d) {
+    if (k.S < 1) return fail(AVBD_E_ARG, "world needs at least one scene");
+    const long long lim = (1ll << 29) - 1;
+    if (k.N > lim || k.J > lim || k.SP > lim || k.T > lim || k.TR > lim || k.items > (1ll << 31) - 1)
+        return fail(AVBD_E_CAPACITY, "world too large for 29-bit element indices");
+    long long mcap = mcap_req > 0 ? mcap_req : 8 * k.nonpart + (k.N - k.nonpart) / 2 + 1024;
+    mcap = std::max<long long>(mcap, k.M + 1024);
+    if (mcap * 8 >= (1ll << 29)) return fail(AVBD_E_CAPACITY, "manifold capacity too large");
+    d->S = int(k.S); d->N = int(k.N); d->J = int(k.J); d->SP = int(k.SP); d->T = int(k.T); d->TR = int(k.TR);
+    d->Mcap = int(mcap);
+    d->n_items = k.items;
+    d->scan_bytes = scan_tmp_bound(std::max<long long>(k.N + 1, mcap + 1));
+    return 0;
+}
+
+// Allocate every device array of a world of the given dimensions (or, on a
+// dry context, only count their bytes in c->alloc_bytes).
+int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_off, int** e_off, uint32_t** e_ent) {
+    const int S = d.S, N = d.N, J = d.J, SP = d.SP, T = d.T, TR = d.TR, Mcap = d.Mcap;
+    DevWorld& w = c->w;
+    w.S = S; w.N = N; w.J = J; w.SP = SP; w.T = T; w.TR = TR; w.Mcap = Mcap;
+    int rc = 0;
+#define A(p, n) if ((rc = c->alloc(p, n))) return rc
+    A(d_sc, S);
+    A(&w.st, (size_t)S * ST_WORDS);
+    A(d_off, S + 1);
+    w.sc = *d_sc;
+    w.sc_off = *d_off;
+    A(&w.pos, N); A(&w.rot, N); A(&w.scale, N); A(&w.vel, N); A(&w.angvel, N); A(&w.pvel, N); A(&w.pangvel, N);
+    A(&w.inertia, N); A(&w.flags, N); A(&w.scene, N); A(&w.last_pos, N); A(&w.last_rot, N); A(&w.inert_pos, N);
+    A(&w.inert_rot, N); A(&w.diff0, N); A(&w.diff1, N);
+    if ((rc = alloc_manifolds(c, c->mb[0], Mcap))) return rc;
+    if ((rc = alloc_manifolds(c, c->mb[1], Mcap))) return rc;
+    c->cur = 0;
+    w.m = c->mb[0];
+    A(&w.j_a, J); A(&w.j_b, J); A(&w.j_serial, J); A(&w.j_flags, J); A(&w.j_ra, J); A(&w.j_rb, J); A(&w.j_rest, J);
+    A(&w.j_c0, (size_t)J * 6); A(&w.j_row, (size_t)J * 36);
+    A(&w.s_a, SP); A(&w.s_b, SP); A(&w.s_serial, SP); A(&w.s_flags, SP); A(&w.s_ra, SP); A(&w.s_rb, SP);
+    A(&w.s_row, (size_t)SP * 6);
+    A(&w.t_v, T); A(&w.t_d, (size_t)T * 3);
+    A(&w.r_v, TR); A(&w.r_d, (size_t)TR * 2);
+    w.f_cap = 16 * Mcap + 2 * (J + SP) + 1;
+    A(&w.f_off, N + 1);
+    A(&w.f_ent, w.f_cap);
+    A(e_off, N + 1);
+    A(e_ent, (size_t)4 * T + 3 * TR + 1);
+    w.e_off = *e_off;
+    w.e_ent = *e_ent;
+    A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
+    A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1); A(&w.seg_off, N + 1);
+    w.pair_cap = std::max(65536, 4 * N);
+    A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
+    A(&w.alive, Mcap + 1); A(&w.alive_off, Mcap + 1);
+    A(&w.fracture_bodies, (size_t)2 * (J + SP) + 2);
+    A(&w.g, G_WORDS);
+    A(&w.next_serial, 1);
+    c->n_items = int(d.n_items);
+    A(&c->items, std::max<long long>(d.n_items, 1));
+    char* tmp;
+    A(&tmp, d.scan_bytes);
+    c->cub_tmp = tmp;
+    c->cub_bytes = d.scan_bytes;
+#undef A
+    return 0;
+}
+
 int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     const int S = in->n_scenes, N = in->n_bodies, J = in->n_joints, SP = in->n_springs;
     const int M = in->n_manifolds, T = in->n_tets, TR = in->n_tris;
@@ -276,56 +367,22 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
             if (!body_ok(in->tris[i].v[k])) return fail(AVBD_E_ARG, "triangle vertex index");
 
     c->release();
-    int nonpart = 0;
-    for (int i = 0; i < N; ++i)
-        if (!(in->bodies[i].flags & AVBD_BODY_PARTICLE)) ++nonpart;
-    long long mcap = mcap_req > 0 ? mcap_req : (long long)8 * nonpart + (N - nonpart) / 2 + 1024;
-    mcap = std::max<long long>(mcap, (long long)M + 1024);
-    if (mcap * 8 >= (1ll << 29)) return fail(AVBD_E_CAPACITY, "manifold capacity too large");
-    const int Mcap = int(mcap);
-
+    WorldDims dims;
+    int rc = make_dims(world_counts(in), mcap_req, &dims);
+    if (rc) return rc;
+    {
+        size_t b1 = 0, b2 = 0;
+        int* dummy = nullptr;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, dummy, dummy, N + 1, c->stream));
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, dummy, dummy, dims.Mcap + 1, c->stream));
+        dims.scan_bytes = std::max(dims.scan_bytes, std::max(b1, b2));
+    }
     DevWorld& w = c->w;
-    w.S = S; w.N = N; w.J = J; w.SP = SP; w.T = T; w.TR = TR; w.Mcap = Mcap;
-    int rc = 0;
     avbd_scene* d_sc;
     int* d_off;
-#define A(p, n) if ((rc = c->alloc(p, n))) return rc
-    A(&d_sc, S);
-    A(&w.st, (size_t)S * ST_WORDS);
-    A(&d_off, S + 1);
-    w.sc = d_sc;
-    w.sc_off = d_off;
-    A(&w.pos, N); A(&w.rot, N); A(&w.scale, N); A(&w.vel, N); A(&w.angvel, N); A(&w.pvel, N); A(&w.pangvel, N);
-    A(&w.inertia, N); A(&w.flags, N); A(&w.scene, N); A(&w.last_pos, N); A(&w.last_rot, N); A(&w.inert_pos, N);
-    A(&w.inert_rot, N); A(&w.diff0, N); A(&w.diff1, N);
-    if ((rc = alloc_manifolds(c, c->mb[0], Mcap))) return rc;
-    if ((rc = alloc_manifolds(c, c->mb[1], Mcap))) return rc;
-    c->cur = 0;
-    w.m = c->mb[0];
-    A(&w.j_a, J); A(&w.j_b, J); A(&w.j_serial, J); A(&w.j_flags, J); A(&w.j_ra, J); A(&w.j_rb, J); A(&w.j_rest, J);
-    A(&w.j_c0, (size_t)J * 6); A(&w.j_row, (size_t)J * 36);
-    A(&w.s_a, SP); A(&w.s_b, SP); A(&w.s_serial, SP); A(&w.s_flags, SP); A(&w.s_ra, SP); A(&w.s_rb, SP);
-    A(&w.s_row, (size_t)SP * 6);
-    A(&w.t_v, T); A(&w.t_d, (size_t)T * 3);
-    A(&w.r_v, TR); A(&w.r_d, (size_t)TR * 2);
-    w.f_cap = 16 * Mcap + 2 * (J + SP) + 1;
-    A(&w.f_off, N + 1);
-    A(&w.f_ent, w.f_cap);
     int* e_off;
     uint32_t* e_ent;
-    A(&e_off, N + 1);
-    A(&e_ent, (size_t)4 * T + 3 * TR + 1);
-    w.e_off = e_off;
-    w.e_ent = e_ent;
-    A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
-    A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1); A(&w.seg_off, N + 1);
-    w.pair_cap = std::max(65536, 4 * N);
-    A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
-    A(&w.alive, Mcap + 1); A(&w.alive_off, Mcap + 1);
-    A(&w.fracture_bodies, (size_t)2 * (J + SP) + 2);
-    A(&w.g, G_WORDS);
-    A(&w.next_serial, 1);
-#undef A
+    if ((rc = allocate_world(c, dims, &d_sc, &d_off, &e_off, &e_ent))) return rc;
 
     // ---- host staging (records -> device SoA) ----
     c->scenes.assign(in->scenes, in->scenes + S);
@@ -504,20 +561,8 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
             for (int ti = 0; ti < tiles; ++ti)
                 for (int tj = ti; tj < tiles; ++tj) items.push_back(make_int4(s, ti, tj, 0));
         }
-        c->n_items = int(items.size());
-        if ((rc = c->alloc(&c->items, std::max<size_t>(items.size(), 1)))) return rc;
+        if ((long long)items.size() != dims.n_items) return fail(AVBD_E_ARG, "broad-phase item count mismatch");
         if ((rc = upload_vec(c, c->items, items))) return rc;
-    }
-    // scan temp storage sized for the largest scan
-    {
-        size_t b1 = 0, b2 = 0;
-        int* dummy = nullptr;
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, dummy, dummy, N + 1, c->stream));
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, dummy, dummy, Mcap + 1, c->stream));
-        c->cub_bytes = std::max(b1, b2);
-        char* tmp;
-        if ((rc = c->alloc(&tmp, c->cub_bytes))) return rc;
-        c->cub_tmp = tmp;
     }
     c->have_world = true;
     if ((rc = build_incidence(c))) return rc;
@@ -619,15 +664,66 @@ int do_step(avbd_ctx* c) {
 
 }  // namespace
 
+int avbd_set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+WorldCounts world_counts(const avbd_world* w) {
+    WorldCounts k;
+    k.S = w->n_scenes; k.N = w->n_bodies; k.J = w->n_joints; k.SP = w->n_springs;
+    k.M = w->n_manifolds; k.T = w->n_tets; k.TR = w->n_tris;
+    long long run = 0;
+    for (long long i = 0; i < k.N; ++i) {
+        if (!(w->bodies[i].flags & AVBD_BODY_PARTICLE)) ++k.nonpart;
+        ++run;
+        if (i + 1 == k.N || w->bodies[i + 1].scene != w->bodies[i].scene) {
+            k.items += broad_items(run);
+            run = 0;
+        }
+    }
+    return k;
+}
+
+long long world_device_bytes(const WorldCounts& k, long long manifold_capacity) {
+    WorldDims d;
+    int rc = make_dims(k, manifold_capacity, &d);
+    if (rc) return rc;
+    avbd_ctx dry;
+    dry.dry = true;
+    avbd_scene* sc;
+    int *off, *eoff;
+    uint32_t* eent;
+    rc = allocate_world(&dry, d, &sc, &off, &eoff, &eent);
+    if (rc) return rc;
+    return (long long)dry.alloc_bytes;
+}
+
 extern "C" {
 
 int avbd_abi_version(void) { return AVBD_ABI_VERSION; }
+
+int avbd_footprint(const avbd_world* w, int64_t manifold_capacity, int64_t* bytes) {
+    if (!w || !bytes) return fail(AVBD_E_ARG, "null argument");
+    if (w->n_scenes < 1 || w->n_bodies < 0 || w->n_joints < 0 || w->n_springs < 0 || w->n_manifolds < 0 ||
+        w->n_tets < 0 || w->n_tris < 0)
+        return fail(AVBD_E_ARG, "bad counts");
+    if (w->n_bodies > 0 && !w->bodies) return fail(AVBD_E_ARG, "null body array");
+    long long b = world_device_bytes(world_counts(w), manifold_capacity);
+    if (b < 0) return (int)b;
+    *bytes = b;
+    return 0;
+}
+
+int avbd_get_memory(avbd_ctx* c, int64_t* bytes) {
+    if (!c || !bytes) return fail(AVBD_E_ARG, "null argument");
+    *bytes = (int64_t)c->alloc_bytes;
+    return 0;
+}
 
 int avbd_record_sizes(int32_t* out, int32_t n) {
     const int32_t sizes[] = {(int32_t)sizeof(avbd_scene),   (int32_t)sizeof(avbd_body),     (int32_t)sizeof(avbd_joint),
                              (int32_t)sizeof(avbd_spring),  (int32_t)sizeof(avbd_contact),  (int32_t)sizeof(avbd_manifold),
                              (int32_t)sizeof(avbd_tet),     (int32_t)sizeof(avbd_tri),      (int32_t)sizeof(avbd_world),
-                             (int32_t)sizeof(avbd_timings)};
+                             (int32_t)sizeof(avbd_timings), (int32_t)sizeof(avbd_request_info),
+                             (int32_t)sizeof(avbd_server_stats)};
     int k = (int)(sizeof(sizes) / sizeof(sizes[0]));
     for (int i = 0; i < n && i < k; ++i) out[i] = sizes[i];
     return k;

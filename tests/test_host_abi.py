@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from skyephyx_amd import capi, roofline, scenes, scenes_fast, world
-from skyephyx_amd.abi import AvbdTimings, AvbdWorld
+from skyephyx_amd.abi import AvbdRequestInfo, AvbdServerStats, AvbdTimings, AvbdWorld
 import ctypes as C
 
 
@@ -26,10 +26,41 @@ def test_library_exports_every_declared_symbol():
 def test_record_layouts_match_numpy_dtypes():
     sizes = (C.c_int32 * 16)()
     n = capi.lib().avbd_record_sizes(sizes, 16)
-    assert n == 10
+    assert n == 12
     assert list(sizes)[:8] == [d.itemsize for d in world.RECORD_DTYPES]
     assert sizes[8] == C.sizeof(AvbdWorld)
     assert sizes[9] == C.sizeof(AvbdTimings)
+    assert sizes[10] == C.sizeof(AvbdRequestInfo)
+    assert sizes[11] == C.sizeof(AvbdServerStats)
+
+
+def test_footprint_is_host_only_and_consistent():
+    """avbd_footprint needs no device.  The device world of several scenes
+    shares its fixed-size arrays, so it is smaller than the sum of the
+    scenes' worlds, and it grows with every element kind."""
+    a = scenes.box_stack_3d(levels=3)
+    b = scenes.fem_block(n=4, cell=0.25)
+    c = scenes.joint_chain(5)
+    fa, fb, fc = capi.footprint(a), capi.footprint(b), capi.footprint(c)
+    both = capi.footprint(world.World.concat([a, b, c]))
+    assert 0 < max(fa, fb, fc) < both < fa + fb + fc
+    bigger = capi.footprint(scenes.box_stack_3d(levels=4))
+    assert bigger > fa
+    assert capi.footprint(a, manifold_capacity=1 << 16) > capi.footprint(a, manifold_capacity=1 << 12)
+
+
+def test_server_without_gpu_fails_loudly():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            pytest.skip("GPU present")
+    except ImportError:
+        pass
+    from skyephyx_amd.serve import Server
+
+    with pytest.raises(capi.AvbdError):
+        Server()
 
 
 def test_create_without_gpu_fails_loudly():
