@@ -283,8 +283,10 @@ int avbd_tune_table(avbd_ctx* ctx, char* out, int32_t len);
 int avbd_tune_load(avbd_ctx* ctx, const char* json);
 
 /* Accumulated launch count and device time (ms) of one kernel class since
- * the last reset: kernel = "primal" or "dual".  reset != 0 clears after
- * reading.  Requires the "kernel_timing" option. */
+ * the last reset: kernel = "primal_forces" / "primal_fem" (each primal
+ * bucket launch: bodies with forces / FEM-only particles, timed on the
+ * stream it runs on), "primal" (both together) or "dual".  reset != 0 clears
+ * after reading.  Requires the "kernel_timing" option. */
 int avbd_kernel_stats(avbd_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms, int32_t reset);
 
 /* Device memory (no reference counterpart).  avbd_footprint: the bytes an

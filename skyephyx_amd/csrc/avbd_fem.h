@@ -250,20 +250,47 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
     a.m[0][1] = a.m[1][0] = lJ * s2;
     a.m[0][2] = a.m[2][0] = lJ * s1;
     a.m[1][2] = a.m[2][1] = lJ * s0;
-    v3 aw;
-    m3 av;
-    sym_eig3(a, aw, av);
-    float ev[9] = {aw.x, aw.y, aw.z, ev3, ev4, ev5, ev6, ev7, ev8};
+    float ev[9] = {0.f, 0.f, 0.f, ev3, ev4, ev5, ev6, ev7, ev8};
 #pragma unroll
-    for (int i = 0; i < 9; ++i) ev[i] = project_ev(ev[i], projMode);
+    for (int i = 3; i < 9; ++i) ev[i] = project_ev(ev[i], projMode);
     v3 u0 = col(U, 0), u1 = col(U, 1), u2 = col(U, 2);
     float vg0 = dot(col(V, 0), gradN), vg1 = dot(col(V, 1), gradN), vg2 = dot(col(V, 2), gradN);
     float Hv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // Scaling modes: with a = Va diag(aw) Va^T and the columns vg_j u_j of
+    // Ut, their part of the vertex block is Ut Va proj(aw) Va^T Ut^T.  Both
+    // projections (clamp and absolute, floor 1e-6) leave every eigenvalue
+    // >= 1e-6 unchanged, so when a - 1e-6 I is positive definite (Sylvester's
+    // criterion; the usual case) that part is Ut a Ut^T and the eigensystem
+    // is not needed.  Otherwise it is projected through Jacobi as in the
+    // reference.
+    const float e6 = 1e-6f;
+    const float b00 = a.m[0][0] - e6, b11 = a.m[1][1] - e6, b22 = a.m[2][2] - e6;
+    const float m01 = b00 * b11 - a.m[0][1] * a.m[0][1];
+    const float dB = b00 * (b11 * b22 - a.m[1][2] * a.m[1][2]) - a.m[0][1] * (a.m[0][1] * b22 - a.m[1][2] * a.m[0][2]) +
+                     a.m[0][2] * (a.m[0][1] * a.m[1][2] - b11 * a.m[0][2]);
+    if (b00 > 0.f && m01 > 0.f && dB > 0.f) {
+        const v3 t0 = smul(vg0, u0), t1 = smul(vg1, u1), t2 = smul(vg2, u2);
+        // rows of Ut a: r_j = sum_k a_jk t_k; block = sum_j t_j r_j^T
+        const v3 r0 = add(add(smul(a.m[0][0], t0), smul(a.m[0][1], t1)), smul(a.m[0][2], t2));
+        const v3 r1 = add(add(smul(a.m[1][0], t0), smul(a.m[1][1], t1)), smul(a.m[1][2], t2));
+        const v3 r2 = add(add(smul(a.m[2][0], t0), smul(a.m[2][1], t1)), smul(a.m[2][2], t2));
+        Hv[0] = (t0.x * r0.x + t1.x * r1.x) + t2.x * r2.x;
+        Hv[1] = (t0.y * r0.x + t1.y * r1.x) + t2.y * r2.x;
+        Hv[2] = (t0.y * r0.y + t1.y * r1.y) + t2.y * r2.y;
+        Hv[3] = (t0.z * r0.x + t1.z * r1.x) + t2.z * r2.x;
+        Hv[4] = (t0.z * r0.y + t1.z * r1.y) + t2.z * r2.y;
+        Hv[5] = (t0.z * r0.z + t1.z * r1.z) + t2.z * r2.z;
+    } else {
+        v3 aw;
+        m3 av;
+        sym_eig3(a, aw, av);
+        const float pw[3] = {project_ev(aw.x, projMode), project_ev(aw.y, projMode), project_ev(aw.z, projMode)};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        v3 c = col(av, k);
-        v3 w = add(add(smul(c.x * vg0, u0), smul(c.y * vg1, u1)), smul(c.z * vg2, u2));
-        add_outer(Hv, ev[k], w);
+        for (int k = 0; k < 3; ++k) {
+            v3 c = col(av, k);
+            v3 w = add(add(smul(c.x * vg0, u0), smul(c.y * vg1, u1)), smul(c.z * vg2, u2));
+            add_outer(Hv, pw[k], w);
+        }
     }
     const float is2 = 0.70710678118f;
     add_outer(Hv, ev[3], smul(is2, sub(smul(vg1, u0), smul(vg0, u1))));

@@ -24,12 +24,12 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-// Temporary storage given to the device-wide scans, sized for n items.  An
+// Temporary storage given to the device-wide scans, sized for n items: an
 // upper bound of the decoupled-lookback scan's needs (one 16-byte state per
-// 256 items plus padding); the upload takes the larger of this and the
+// 256 items plus padding).  The upload takes the larger of this and the
 // library's own query, so the byte count of a world is known without a
 // device.
-size_t scan_tmp_bound(long long n) { return (size_t)(16 * (n / 256 + 64) + 1024); }
+size_t cub_tmp_bound(long long n) { return (size_t)(16 * (n / 256 + 64) + 1024); }
 
 #define HIP_TRY(expr)                                                                      \
     do {                                                                                   \
@@ -56,8 +56,9 @@ struct avbd_ctx {
     DevWorld w{};
     ManifoldBuf mb[2]{};
     int cur = 0;
-    int4* items = nullptr;
+    int4* items = nullptr;  // broad-phase work items (scene, i-tile, j-tile)
     int n_items = 0;
+    int n_tiles = 0;
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
     int lanes = 0;      // 0 = chosen by the tuner
@@ -72,42 +73,54 @@ struct avbd_ctx {
     hipEvent_t ev[8]{};
     avbd_timings timings{};
     int last_ncol = 0;
-    // per-launch timing of the primal and dual kernels (option kernel_timing)
+    // Per-launch timing (option kernel_timing): HIP events recorded on the
+    // stream the work runs on, around each dual launch (K_DUAL) and each
+    // primal bucket launch (K_FORCES, K_FEM).  Two events per launch and no
+    // more: on small launches every extra event shows in the step time.
+    enum { K_DUAL = 0, K_FORCES = 1, K_FEM = 2, K_KINDS = 3 };
+    struct Mark {
+        int a, b, kind;
+    };
     bool ktiming = false;
     std::vector<hipEvent_t> kev;
-    std::vector<int> kev_kind;  // 0 primal, 1 dual
+    std::vector<Mark> kmarks;
     int kev_used = 0;
-    long long kstat_n[2] = {0, 0};
-    double kstat_ms[2] = {0.0, 0.0};
+    long long kstat_n[K_KINDS] = {};
+    double kstat_ms[K_KINDS] = {};
 
-    hipEvent_t next_event() {
+    int next_event() {
         if (kev_used >= (int)kev.size()) {
             hipEvent_t e;
             hipEventCreate(&e);
             kev.push_back(e);
         }
-        return kev[kev_used++];
+        return kev_used++;
     }
-    void mark_begin(int kind) {
-        if (!ktiming) return;
-        hipEventRecord(next_event(), stream);
-        kev_kind.push_back(kind);
+    // returns a mark id for mark_end (-1 when timing is off)
+    int mark_begin(int kind) {
+        if (!ktiming) return -1;
+        int e = next_event();
+        hipEventRecord(kev[e], stream);
+        kmarks.push_back({e, -1, kind});
+        return (int)kmarks.size() - 1;
     }
-    void mark_end() {
-        if (!ktiming) return;
-        hipEventRecord(next_event(), stream);
+    void mark_end(int id) {
+        if (id < 0) return;
+        int e = next_event();
+        hipEventRecord(kev[e], stream);
+        kmarks[id].b = e;
     }
-    // called after the stream has been synchronised
+    // called after both streams have been synchronised
     void collect_kernel_times() {
-        for (int i = 0; i + 1 < kev_used; i += 2) {
+        for (const Mark& m : kmarks) {
+            if (m.b < 0) continue;
             float ms = 0.f;
-            hipEventElapsedTime(&ms, kev[i], kev[i + 1]);
-            int kind = kev_kind[i / 2];
-            kstat_n[kind] += 1;
-            kstat_ms[kind] += ms;
+            hipEventElapsedTime(&ms, kev[m.a], kev[m.b]);
+            kstat_n[m.kind] += 1;
+            kstat_ms[m.kind] += ms;
         }
         kev_used = 0;
-        kev_kind.clear();
+        kmarks.clear();
     }
 
     // dry: allocations are only counted (world_device_bytes)
@@ -137,10 +150,11 @@ struct avbd_ctx {
         allocs.clear();
         alloc_bytes = 0;
         kev_used = 0;
-        kev_kind.clear();
+        kmarks.clear();
         w = DevWorld{};
         mb[0] = mb[1] = ManifoldBuf{};
         items = nullptr;
+        n_items = n_tiles = 0;
         cub_tmp = nullptr;
         have_world = false;
     }
@@ -213,11 +227,6 @@ void launch_primal_g(avbd_ctx* c, int G, int lo, int hi, int iter, int poststab)
     }
 }
 
-// One colour of a sweep: the bodies with forces and the FEM-only bodies, two
-// instantiations that run concurrently (the FEM bucket on the side stream,
-// forked from and joined back into the main stream with events, so the next
-// colour still sees both).  Empty buckets are not launched.  The kernel
-// timing brackets the whole colour.
 // One bucket launch on c->stream with the lane width fixed by the user or
 // picked by the tuner (class 0 = bodies with forces, 1 = FEM-only).
 template <bool FEM>
@@ -225,16 +234,21 @@ void launch_bucket(avbd_ctx* c, int lo, int hi, int iter, int poststab) {
     int fixed = FEM ? c->fem_lanes : c->lanes;
     int measure = -1;
     int G = fixed > 0 ? fixed : c->tuner.choose(FEM ? 1 : 0, hi - lo, &measure);
+    const int mark = c->mark_begin(FEM ? avbd_ctx::K_FEM : avbd_ctx::K_FORCES);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].a, c->stream);
     launch_primal_g<FEM>(c, G, lo, hi, iter, poststab);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].b, c->stream);
+    c->mark_end(mark);
 }
 
+// One colour of a sweep: the bodies with forces and the FEM-only bodies, two
+// instantiations that run concurrently (the FEM bucket on the side stream,
+// forked from and joined back into the main stream with events, so the next
+// colour still sees both).  Empty buckets are not launched.
 int launch_colour(avbd_ctx* c, const int* off, int colour, int iter, int poststab) {
     const int g0 = off[2 * colour], g1 = off[2 * colour + 1], g2 = off[2 * colour + 2];
     const bool gen = g1 > g0, fem = g2 > g1;
     if (!gen && !fem) return 0;
-    c->mark_begin(0);
     if (gen && fem) {
         HIP_TRY(hipEventRecord(c->fork, c->stream));
         HIP_TRY(hipStreamWaitEvent(c->side, c->fork, 0));
@@ -249,14 +263,13 @@ int launch_colour(avbd_ctx* c, const int* off, int colour, int iter, int poststa
     } else {
         launch_bucket<true>(c, g1, g2, iter, poststab);
     }
-    c->mark_end();
     return 0;
 }
 
 // Sizes of every device array of a world.
 struct WorldDims {
     int S = 0, N = 0, J = 0, SP = 0, T = 0, TR = 0, Mcap = 0;
-    long long n_items = 0;
+    int n_tiles = 0, n_items = 0;
     size_t scan_bytes = 0;
 };
 
@@ -270,8 +283,9 @@ int make_dims(const WorldCounts& k, long long mcap_req, WorldDims* d) {
     if (mcap * 8 >= (1ll << 29)) return fail(AVBD_E_CAPACITY, "manifold capacity too large");
     d->S = int(k.S); d->N = int(k.N); d->J = int(k.J); d->SP = int(k.SP); d->T = int(k.T); d->TR = int(k.TR);
     d->Mcap = int(mcap);
-    d->n_items = k.items;
-    d->scan_bytes = scan_tmp_bound(std::max<long long>(k.N + 1, mcap + 1));
+    d->n_tiles = int(k.tiles);
+    d->n_items = int(k.items);
+    d->scan_bytes = cub_tmp_bound(std::max<long long>(k.N + 1, mcap + 1));
     return 0;
 }
 
@@ -308,16 +322,25 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     A(e_ent, (size_t)4 * T + 3 * TR + 1);
     w.e_off = *e_off;
     w.e_ent = *e_ent;
+    A(&w.ev_a, (size_t)4 * T + 3 * TR + 1); A(&w.ev_b, (size_t)4 * T + 3 * TR + 1);
+    A(&w.ev_c, (size_t)4 * T + 3 * TR + 1); A(&w.ev_v, (size_t)4 * T + 3 * TR + 1);
     A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
     A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1); A(&w.seg_off, N + 1);
     w.pair_cap = std::max(65536, 4 * N);
     A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
+    int* toff;
+    int* tscene;
+    A(&toff, S + 1); A(&tscene, std::max(d.n_tiles, 1));
+    w.tile_off = toff;
+    w.tile_scene = tscene;
+    A(&w.tile_lo, std::max(d.n_tiles, 1)); A(&w.tile_hi, std::max(d.n_tiles, 1));
+    c->n_tiles = d.n_tiles;
+    c->n_items = d.n_items;
+    A(&c->items, std::max(d.n_items, 1));
     A(&w.alive, Mcap + 1); A(&w.alive_off, Mcap + 1);
     A(&w.fracture_bodies, (size_t)2 * (J + SP) + 2);
     A(&w.g, G_WORDS);
     A(&w.next_serial, 1);
-    c->n_items = int(d.n_items);
-    A(&c->items, std::max<long long>(d.n_items, 1));
     char* tmp;
     A(&tmp, d.scan_bytes);
     c->cub_tmp = tmp;
@@ -541,8 +564,28 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
             rd[i * 2 + 0] = make_float4(t.dm_inv[0], t.dm_inv[1], t.dm_inv[2], t.dm_inv[3]);
             rd[i * 2 + 1] = make_float4(t.rest_area, t.mu, t.lambda, 0.f);
         }
+        // the element stream in incidence order
+        const size_t nE = ent.size();
+        std::vector<float4> ea(nE), eb(nE), ec(nE, make_float4(0.f, 0.f, 0.f, 0.f));
+        std::vector<int4> evv(nE, make_int4(0, 0, 0, 0));
+        for (int k = 0; k < eoff[N]; ++k) {
+            const uint32_t e = ent[k];
+            const uint32_t idx = e & 0x1FFFFFFFu;
+            if (e >> 31) {
+                ea[k] = rd[idx * 2 + 0];
+                eb[k] = rd[idx * 2 + 1];
+                evv[k] = rv[idx];
+            } else {
+                ea[k] = td[idx * 3 + 0];
+                eb[k] = td[idx * 3 + 1];
+                ec[k] = td[idx * 3 + 2];
+                evv[k] = tv[idx];
+            }
+        }
         if ((rc = upload_vec(c, e_off, eoff)) || (rc = upload_vec(c, e_ent, ent)) || (rc = upload_vec(c, w.t_v, tv)) ||
-            (rc = upload_vec(c, w.t_d, td)) || (rc = upload_vec(c, w.r_v, rv)) || (rc = upload_vec(c, w.r_d, rd)))
+            (rc = upload_vec(c, w.t_d, td)) || (rc = upload_vec(c, w.r_v, rv)) || (rc = upload_vec(c, w.r_d, rd)) ||
+            (rc = upload_vec(c, w.ev_a, ea)) || (rc = upload_vec(c, w.ev_b, eb)) || (rc = upload_vec(c, w.ev_c, ec)) ||
+            (rc = upload_vec(c, w.ev_v, evv)))
             return rc;
     }
     {
@@ -552,17 +595,25 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
         std::vector<long long> ns(1, max_serial + 1);
         if ((rc = upload_vec(c, w.next_serial, ns))) return rc;
     }
-    // broad-phase work items: (scene, i-tile, j-tile >= i-tile)
+    // broad-phase tiles and work items (scene, i-tile, j-tile >= i-tile)
     {
+        std::vector<int> toff(S + 1, 0), tscene;
         std::vector<int4> items;
+        tscene.reserve(dims.n_tiles);
+        items.reserve(dims.n_items);
         for (int s = 0; s < S; ++s) {
-            int n = off[s + 1] - off[s];
-            int tiles = (n + 255) / 256;
-            for (int ti = 0; ti < tiles; ++ti)
+            const int tiles = (int)broad_tiles(off[s + 1] - off[s]);
+            toff[s + 1] = toff[s] + tiles;
+            for (int ti = 0; ti < tiles; ++ti) {
+                tscene.push_back(s);
                 for (int tj = ti; tj < tiles; ++tj) items.push_back(make_int4(s, ti, tj, 0));
+            }
         }
-        if ((long long)items.size() != dims.n_items) return fail(AVBD_E_ARG, "broad-phase item count mismatch");
-        if ((rc = upload_vec(c, c->items, items))) return rc;
+        if ((int)tscene.size() != dims.n_tiles || (int)items.size() != dims.n_items)
+            return fail(AVBD_E_ARG, "broad-phase tile count mismatch");
+        if ((rc = upload_vec(c, const_cast<int*>(w.tile_off), toff)) ||
+            (rc = upload_vec(c, const_cast<int*>(w.tile_scene), tscene)) || (rc = upload_vec(c, c->items, items)))
+            return rc;
     }
     c->have_world = true;
     if ((rc = build_incidence(c))) return rc;
@@ -598,9 +649,14 @@ int do_step(avbd_ctx* c) {
     HIP_TRY(hipEventRecord(c->ev[0], st));
     // ---- 1. broad phase
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_NEW_PAIRS, 0);
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_CANDIDATES, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(grid_for(N + 1)), dim3(256), 0, st, w.cnt_i, N + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(grid_for(N + 1)), dim3(256), 0, st, w.cur_i, N + 1, 0);
-    if (c->n_items > 0) hipLaunchKernelGGL(k_broad, dim3(c->n_items), dim3(256), 0, st, w, (const int4*)c->items);
+    if (c->n_items > 0) {
+        hipLaunchKernelGGL(k_tile_bounds, dim3(c->n_tiles), dim3(256), 0, st, w);
+        hipLaunchKernelGGL(k_broad, dim3(c->n_items), dim3(256), 0, st, w, (const int4*)c->items);
+        hipLaunchKernelGGL(k_pair_filter, dim3(grid_for(N)), dim3(256), 0, st, w);
+    }
     int rc = scan_ints(c, w.cnt_i, w.seg_off, N + 1);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pair_scatter, dim3(grid_for(w.pair_cap)), dim3(256), 0, st, w);
@@ -640,9 +696,9 @@ int do_step(avbd_ctx* c) {
     const int dgrid = grid_for((long long)g[G_MANIFOLDS] * 8 + w.J + w.SP, 256, 8192);
     for (int it = 0; it < max_iters; ++it) {
         for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, it, 0);
-        c->mark_begin(1);
+        const int mark = c->mark_begin(avbd_ctx::K_DUAL);
         hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
-        c->mark_end();
+        c->mark_end(mark);
         if (w.J + w.SP > 0) {
             hipLaunchKernelGGL(k_fracture_reset, dim3(grid_for(2 * (w.J + w.SP) + 2)), dim3(256), 0, st, w);
             hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_FRACTURES, 0);
@@ -675,6 +731,7 @@ WorldCounts world_counts(const avbd_world* w) {
         if (!(w->bodies[i].flags & AVBD_BODY_PARTICLE)) ++k.nonpart;
         ++run;
         if (i + 1 == k.N || w->bodies[i + 1].scene != w->bodies[i].scene) {
+            k.tiles += broad_tiles(run);
             k.items += broad_items(run);
             run = 0;
         }
@@ -809,14 +866,25 @@ int avbd_tune_load(avbd_ctx* c, const char* json) {
 int avbd_kernel_stats(avbd_ctx* c, const char* kernel, int64_t* launches, double* total_ms, int32_t reset) {
     if (!c || !kernel) return fail(AVBD_E_ARG, "null argument");
     std::string k(kernel);
-    int kind = k == "primal" ? 0 : (k == "dual" ? 1 : -1);
-    if (kind < 0) return fail(AVBD_E_ARG, "unknown kernel class " + k);
-    if (launches) *launches = c->kstat_n[kind];
-    if (total_ms) *total_ms = c->kstat_ms[kind];
-    if (reset) {
-        c->kstat_n[kind] = 0;
-        c->kstat_ms[kind] = 0.0;
+    // "primal" = both bucket classes together
+    int lo = k == "primal" ? avbd_ctx::K_FORCES
+           : k == "dual" ? avbd_ctx::K_DUAL
+           : k == "primal_forces" ? avbd_ctx::K_FORCES
+           : k == "primal_fem" ? avbd_ctx::K_FEM : -1;
+    if (lo < 0) return fail(AVBD_E_ARG, "unknown kernel class " + k);
+    const int hi = k == "primal" ? avbd_ctx::K_FEM : lo;
+    long long n = 0;
+    double ms = 0.0;
+    for (int kind = lo; kind <= hi; ++kind) {
+        n += c->kstat_n[kind];
+        ms += c->kstat_ms[kind];
+        if (reset) {
+            c->kstat_n[kind] = 0;
+            c->kstat_ms[kind] = 0.0;
+        }
     }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
     return 0;
 }
 

@@ -10,13 +10,15 @@
 // The element counts a device world's allocations depend on.
 struct WorldCounts {
     long long S = 0, N = 0, nonpart = 0, J = 0, SP = 0, M = 0, T = 0, TR = 0;
+    long long tiles = 0;  // broad-phase tiles (256 bodies), summed over scenes
     long long items = 0;  // broad-phase tile pairs, summed over scenes
 };
 
-// Broad-phase work items of one scene of n bodies: 256-body tiles, pairs
-// (i-tile, j-tile >= i-tile).
+// Broad-phase tiles and tile pairs (i-tile, j-tile >= i-tile) of one scene
+// of n bodies.
+inline long long broad_tiles(long long n) { return (n + 255) / 256; }
 inline long long broad_items(long long n) {
-    long long t = (n + 255) / 256;
+    const long long t = broad_tiles(n);
     return t * (t + 1) / 2;
 }
 

@@ -5,7 +5,9 @@
 
 __global__ void k_fill_int(int* p, int n, int v);
 __global__ void k_set_word(int* p, int v);
+__global__ void k_tile_bounds(DevWorld w);
 __global__ void k_broad(DevWorld w, const int4* items);
+__global__ void k_pair_filter(DevWorld w);
 __global__ void k_pair_scatter(DevWorld w);
 __global__ void k_pair_rank_append(DevWorld w);
 __global__ void k_pair_finish(DevWorld w);

@@ -1,7 +1,8 @@
 // avbd_kernels.hip -- HIP kernels of the device-resident AVBD step (gfx950).
 //
 // Step pipeline (Solver::Step, solver.cpp:199-627), all scenes at once:
-//   k_broad -> k_pair_scatter -> k_pair_rank_append -> k_pair_finish   (phase 1)
+//   k_tile_bounds, k_broad -> k_pair_scatter -> k_pair_rank_append
+//     -> k_pair_finish                                                 (phase 1)
 //   k_manifold_init, k_force_init -> compaction                       (phase 2)
 //   incidence lists, greedy colouring, colour lists
 //   k_predict                                                          (phase 4)
@@ -60,13 +61,70 @@ AVBD_DEV bool constrained(const DevWorld& w, int i, int j) {
     return false;
 }
 
-// All-pairs bounding-sphere test (solver.cpp:236-255), tiled: one work item
-// = (scene, i-tile, j-tile >= i-tile); the j-tile is staged in LDS.
-// New (unconstrained) pairs are appended unordered; cnt_i counts per i.
+// Broad phase: the bounding-sphere test of every body pair of a scene
+// (solver.cpp:236-255), tiled.  A scene's bodies form tiles of 256 in index
+// order; one work item = (scene, i-tile, j-tile >= i-tile), the j-tile staged
+// in LDS.  k_tile_bounds gives each tile the box around its bodies' spheres,
+// widened by 1e-5 (|x| + r) per side (far more than the rounding of the
+// bounds), and an item whose two boxes are disjoint is skipped: no pair of
+// it can pass the exact test, so the pair set equals that of the full loop.
+// Overlapping pairs go to a candidate list (pairs_tmp); k_pair_filter then
+// drops those already linked by a force, one thread per candidate, and
+// appends the new pairs unordered (creation order is restored by rank
+// later); cnt_i counts per lower index.
+__global__ __launch_bounds__(256) void k_tile_bounds(DevWorld w) {
+    const int t = blockIdx.x;
+    const int s = w.tile_scene[t];
+    const int i = w.sc_off[s] + (t - w.tile_off[s]) * 256 + threadIdx.x;
+    const float inf = __builtin_huge_valf();
+    float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    if (scene_runs(w, s) && i < w.sc_off[s + 1]) {
+        const float4 p = w.pos[i];
+        const float r = w.scale[i].w;
+        const float c[3] = {p.x, p.y, p.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float m = 1e-5f * (fabsf(c[a]) + r);
+            lo[a] = (c[a] - r) - m;
+            hi[a] = (c[a] + r) + m;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
+            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
+        }
+    __shared__ float red[4][6];
+    const int wave = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; ++a) {
+            red[wave][a] = lo[a];
+            red[wave][3 + a] = hi[a];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], red[k][a]);
+                hi[a] = fmaxf(hi[a], red[k][3 + a]);
+            }
+        w.tile_lo[t] = make_float4(lo[0], lo[1], lo[2], 0.f);
+        w.tile_hi[t] = make_float4(hi[0], hi[1], hi[2], 0.f);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_broad(DevWorld w, const int4* items) {
     const int4 it = items[blockIdx.x];
     const int s = it.x;
     if (!scene_runs(w, s)) return;
+    if (it.y != it.z) {  // the same decision for the whole block, before its barrier
+        const float4 alo = w.tile_lo[w.tile_off[s] + it.y], ahi = w.tile_hi[w.tile_off[s] + it.y];
+        const float4 blo = w.tile_lo[w.tile_off[s] + it.z], bhi = w.tile_hi[w.tile_off[s] + it.z];
+        if (ahi.x < blo.x || bhi.x < alo.x || ahi.y < blo.y || bhi.y < alo.y || ahi.z < blo.z || bhi.z < alo.z)
+            return;
+    }
     const int off = w.sc_off[s];
     const int end = w.sc_off[s + 1];
     __shared__ float4 tile[256];
@@ -83,25 +141,43 @@ __global__ __launch_bounds__(256) void k_broad(DevWorld w, const int4* items) {
     const v3 pi = f3(pi4);
     const float ri = w.scale[i].w;
     const int jn = min(256, end - j0);
+    const int lane = threadIdx.x & 63;
     for (int t = 0; t < jn; ++t) {
         const int j = j0 + t;
-        if (j <= i) continue;
         const float4 pj = tile[t];
         v3 d = sub(pi, f3(pj));
         float d2 = dot(d, d);
         float rs = ri + pj.w;
-        if (d2 > (rs * rs) * 1.00001f) continue;  // conservative pre-test
-        float dist = sqrtf(d2);
-        if (dist < rs) {
-            if (!constrained(w, i, j)) {
-                int slot = atomicAdd(&w.g[G_NEW_PAIRS], 1);
-                if (slot < w.pair_cap) {
-                    w.pairs[slot] = make_int2(i, j);
-                    atomicAdd(&w.cnt_i[i], 1);
-                } else {
-                    atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
-                }
-            }
+        bool hit = j > i && !(d2 > (rs * rs) * 1.00001f);  // conservative pre-test
+        if (hit) hit = sqrtf(d2) < rs;
+        // one atomic per wave for its hits at this t
+        const unsigned long long mask = __ballot(hit);
+        if (mask == 0ull) continue;
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&w.g[G_CANDIDATES], __popcll(mask));
+        base = __shfl(base, leader);
+        if (hit) {
+            const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+            if (slot < w.pair_cap) w.pairs_tmp[slot] = make_int2(i, j);
+            else atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
+        }
+    }
+}
+
+// Keep the candidates not yet linked by a force (isConstrainedTo) as new
+// pairs.
+__global__ void k_pair_filter(DevWorld w) {
+    const int C = min(w.g[G_CANDIDATES], w.pair_cap);
+    GRID_STRIDE(p, C) {
+        const int2 pr = w.pairs_tmp[p];
+        if (constrained(w, pr.x, pr.y)) continue;
+        int slot = atomicAdd(&w.g[G_NEW_PAIRS], 1);
+        if (slot < w.pair_cap) {
+            w.pairs[slot] = pr;
+            atomicAdd(&w.cnt_i[pr.x], 1);
+        } else {
+            atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
         }
     }
 }
@@ -454,15 +530,13 @@ AVBD_DEV void for_each_neighbour(const DevWorld& w, int v, F f) {
     }
     for (int k = w.e_off[v]; k < w.e_off[v + 1]; ++k) {
         uint32_t e = w.e_ent[k];
-        uint32_t idx = ENT_INDEX(e);
         int local = EENT_LOCAL(e);
+        const int4 t = w.ev_v[k];
         if (EENT_TRI(e)) {
-            int4 t = w.r_v[idx];
             if (local != 0) f(t.x);
             if (local != 1) f(t.y);
             if (local != 2) f(t.z);
         } else {
-            int4 t = w.t_v[idx];
             if (local != 0) f(t.x);
             if (local != 1) f(t.y);
             if (local != 2) f(t.z);
@@ -921,12 +995,12 @@ AVBD_DEV void accumulate_force(const DevWorld& w, uint32_t e, float alpha, const
     }
 }
 
-AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int projMode, v3& g, float H[6]);
+AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int k, int projMode, v3& g, float H[6]);
 
-AVBD_DEV void accumulate_energy(const DevWorld& w, uint32_t e, int projMode, Acc& A) {
+AVBD_DEV void accumulate_energy(const DevWorld& w, int k, int projMode, Acc& A) {
     v3 g;
     float H[6];
-    energy_terms(w, e, projMode, g, H);
+    energy_terms(w, w.e_ent[k], k, projMode, g, H);
     A.r[0] = A.r[0] + g.x;
     A.r[1] = A.r[1] + g.y;
     A.r[2] = A.r[2] + g.z;
@@ -934,32 +1008,64 @@ AVBD_DEV void accumulate_energy(const DevWorld& w, uint32_t e, int projMode, Acc
     for (int i = 0; i < 6; ++i) A.L[i] = A.L[i] + H[i];
 }
 
-// Gradient and packed lower 3x3 Hessian of one energy entry for its vertex.
-AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int projMode, v3& g, float H[6]) {
-    const uint32_t idx = ENT_INDEX(e);
-    const int local = EENT_LOCAL(e);
-    if (!EENT_TRI(e)) {
-        const int4 tv = w.t_v[idx];
-        const float4 d0 = w.t_d[idx * 3 + 0], d1 = w.t_d[idx * 3 + 1], d2 = w.t_d[idx * 3 + 2];
+// Gradient and packed lower 3x3 Hessian of energy incidence entry k (value
+// e = e_ent[k]) for its vertex; the element's data comes from the
+// incidence-ordered stream.
+// The three stages of one energy entry, split so a loop can overlap the
+// loads of later entries with the arithmetic of the current one.
+struct ElemIn {  // entry value and the element's stream record
+    uint32_t e;
+    float4 a, b, c;
+    int4 v;
+};
+struct ElemPos {  // the element's vertex positions
+    v3 p0, p1, p2, p3;
+};
+
+AVBD_DEV void elem_load(const DevWorld& w, int k, ElemIn& x) {
+    x.e = w.e_ent[k];
+    x.a = w.ev_a[k];
+    x.b = w.ev_b[k];
+    x.c = w.ev_c[k];
+    x.v = w.ev_v[k];
+}
+
+AVBD_DEV void elem_pos(const DevWorld& w, const ElemIn& x, ElemPos& p) {
+    p.p0 = f3(w.pos[x.v.x]);
+    p.p1 = f3(w.pos[x.v.y]);
+    p.p2 = f3(w.pos[x.v.z]);
+    p.p3 = EENT_TRI(x.e) ? p.p2 : f3(w.pos[x.v.w]);
+}
+
+AVBD_DEV void elem_terms(const ElemIn& x, const ElemPos& p, int projMode, v3& g, float H[6]) {
+    const int local = EENT_LOCAL(x.e);
+    if (!EENT_TRI(x.e)) {
+        const float4 d0 = x.a, d1 = x.b, d2 = x.c;
         m3 Dm;
         Dm.m[0][0] = d0.x; Dm.m[0][1] = d0.y; Dm.m[0][2] = d0.z;
         Dm.m[1][0] = d0.w; Dm.m[1][1] = d1.x; Dm.m[1][2] = d1.y;
         Dm.m[2][0] = d1.z; Dm.m[2][1] = d1.w; Dm.m[2][2] = d2.x;
         const v3 g1 = row(Dm, 0), g2 = row(Dm, 1), g3 = row(Dm, 2);
         v3 gN = local == 1 ? g1 : (local == 2 ? g2 : (local == 3 ? g3 : sub(sub(neg(g1), g2), g3)));
-        neohookean_vertex_terms(f3(w.pos[tv.x]), f3(w.pos[tv.y]), f3(w.pos[tv.z]), f3(w.pos[tv.w]), Dm, d2.y, d2.z,
-                                d2.w, gN, projMode, g, H);
+        neohookean_vertex_terms(p.p0, p.p1, p.p2, p.p3, Dm, d2.y, d2.z, d2.w, gN, projMode, g, H);
     } else {
-        const int4 tv = w.r_v[idx];
-        const float4 d0 = w.r_d[idx * 2 + 0], d1 = w.r_d[idx * 2 + 1];
+        const float4 d0 = x.a, d1 = x.b;
         const float Dm[4] = {d0.x, d0.y, d0.z, d0.w};
         float gn0, gn1;
         if (local == 1) { gn0 = Dm[0]; gn1 = Dm[1]; }
         else if (local == 2) { gn0 = Dm[2]; gn1 = Dm[3]; }
         else { gn0 = -Dm[0] - Dm[2]; gn1 = -Dm[1] - Dm[3]; }
-        stvk_vertex_terms(f3(w.pos[tv.x]), f3(w.pos[tv.y]), f3(w.pos[tv.z]), Dm, d1.x, d1.y, d1.z, gn0, gn1,
-                          projMode, g, H);
+        stvk_vertex_terms(p.p0, p.p1, p.p2, Dm, d1.x, d1.y, d1.z, gn0, gn1, projMode, g, H);
     }
+}
+
+AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int k, int projMode, v3& g, float H[6]) {
+    (void)e;
+    ElemIn x;
+    ElemPos p;
+    elem_load(w, k, x);
+    elem_pos(w, x, p);
+    elem_terms(x, p, projMode, g, H);
 }
 
 // One primal block update per body of colour `c` (solver.cpp:392-448 and
@@ -999,16 +1105,30 @@ AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int p
             L[0] = ml; L[2] = ml; L[5] = ml;
             r[0] = ml * dl.x; r[1] = ml * dl.y; r[2] = ml * dl.z;
         }
+        // Entries k = e0 + lane, + G, ... in order; the stream record of the
+        // next entry is loaded while the current one is evaluated.
         const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
-        for (int k = e0 + lane; k < e1; k += G) {
+        int k = e0 + lane;
+        ElemIn in0;
+        bool h0 = k < e1;
+        if (h0) elem_load(w, k, in0);
+        while (h0) {
+            const bool h1 = k + G < e1;
+            ElemPos p0;
+            elem_pos(w, in0, p0);
+            ElemIn in1;
+            if (h1) elem_load(w, k + G, in1);
             v3 g;
             float H[6];
-            energy_terms(w, w.e_ent[k], sc.projection_mode, g, H);
+            elem_terms(in0, p0, sc.projection_mode, g, H);
             r[0] = r[0] + g.x;
             r[1] = r[1] + g.y;
             r[2] = r[2] + g.z;
 #pragma unroll
             for (int i = 0; i < 6; ++i) L[i] = L[i] + H[i];
+            in0 = in1;
+            h0 = h1;
+            k += G;
         }
         if (G > 1) {
 #pragma unroll
@@ -1101,7 +1221,7 @@ __global__ __launch_bounds__(256) void k_primal(DevWorld w, int lo, int hi, int 
                 for (int k = f0 + lane; k < f1; k += G) accumulate_force(w, w.f_ent[k], alpha, dSelf, A);
             }
             const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
-            for (int k = e0 + lane; k < e1; k += G) accumulate_energy(w, w.e_ent[k], sc.projection_mode, A);
+            for (int k = e0 + lane; k < e1; k += G) accumulate_energy(w, k, sc.projection_mode, A);
             if (G > 1) {
 #pragma unroll
                 for (int off = 1; off < G; off <<= 1) {

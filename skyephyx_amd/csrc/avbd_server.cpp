@@ -48,14 +48,15 @@ struct SceneHost {
         k.M = (long long)manifolds.size();
         k.T = (long long)tets.size();
         k.TR = (long long)tris.size();
-        k.items = k.N > 0 ? broad_items(k.N) : 0;
+        k.tiles = broad_tiles(k.N);
+        k.items = broad_items(k.N);
         return k;
     }
 };
 
 void add_counts(WorldCounts& a, const WorldCounts& b) {
     a.S += b.S; a.N += b.N; a.nonpart += b.nonpart; a.J += b.J; a.SP += b.SP;
-    a.M += b.M; a.T += b.T; a.TR += b.TR; a.items += b.items;
+    a.M += b.M; a.T += b.T; a.TR += b.TR; a.tiles += b.tiles; a.items += b.items;
 }
 
 // Renumber the force serials of one scene by rank (stable on ties), keeping

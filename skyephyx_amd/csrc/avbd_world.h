@@ -44,6 +44,7 @@
 #define G_NCOLOURS 7
 #define G_ANY_OOB 8
 #define G_ANY_EMERGENCY 9
+#define G_CANDIDATES 10   // overlapping pairs found by the broad phase
 #define G_WORDS 16
 
 #define AVBD_MAX_COLOURS 256
@@ -134,6 +135,15 @@ struct DevWorld {
     // energy incidence (static)
     const int* e_off;   // [N+1]
     const uint32_t* e_ent;
+    // element data in incidence order (static): entry k of e_ent finds its
+    // element's rest-shape data and vertex ids at index k here, so the
+    // elements of one vertex are one contiguous stream.  Tets: a, b, c = the
+    // three t_d records, v = t_v; triangles: a, b = the two r_d records,
+    // c unused, v = r_v.
+    float4* ev_a;
+    float4* ev_b;
+    float4* ev_c;
+    int4* ev_v;
     // colouring
     int* colour;        // [N]
     int* col_cnt;       // [AVBD_BUCKETS + 1]
@@ -147,6 +157,12 @@ struct DevWorld {
     int* seg_off;       // [N+1]
     int2* pairs;        // [pair_cap]
     int2* pairs_tmp;    // [pair_cap]
+    // broad-phase tiles (256 bodies of a scene in index order): first tile
+    // of each scene, scene of each tile, widened box of each tile's spheres
+    const int* tile_off;    // [S+1]
+    const int* tile_scene;  // [tiles]
+    float4* tile_lo;        // [tiles]
+    float4* tile_hi;        // [tiles]
     int pair_cap;
     int* alive;         // [Mcap]
     int* alive_off;     // [Mcap+1]

@@ -76,6 +76,38 @@ def step_bytes(world):
     return int((per * visits).sum()), int((dual * iters).sum())
 
 
+def fem_only_mask(world):
+    """Bodies the primal sweep puts in its FEM-only bucket: particles with
+    element energies and no contact point, joint or spring."""
+    b = world.bodies
+    N = len(b)
+    forces = np.zeros(N, np.int64)
+    if len(world.manifolds):
+        for side in ("body_a", "body_b"):
+            np.add.at(forces, world.manifolds[side], world.manifolds["n_contacts"])
+    for arr in (world.joints, world.springs):
+        if len(arr):
+            a = arr["body_a"]
+            np.add.at(forces, a[a >= 0], 1)
+            np.add.at(forces, arr["body_b"], 1)
+    n_elem = np.zeros(N, np.int64)
+    if len(world.tets):
+        np.add.at(n_elem, world.tets["v"].ravel(), 1)
+    if len(world.tris):
+        np.add.at(n_elem, world.tris["v"].ravel(), 1)
+    particle = np.all(b["inertia"] == 0, axis=1)
+    return (forces == 0) & (n_elem > 0) & particle
+
+
+def class_step_bytes(world):
+    """Primal algorithmic bytes of one step split by bucket:
+    (bodies with forces, FEM-only particles)."""
+    per, visits, _, _ = sweep_bytes(world)
+    fem = fem_only_mask(world)
+    tot = per * visits
+    return int(tot[~fem].sum()), int(tot[fem].sum())
+
+
 def body_updates_per_step(world):
     """Primal block updates per step (dynamic bodies x (iterations + post-stab))."""
     _, visits, _, _ = sweep_bytes(world)
