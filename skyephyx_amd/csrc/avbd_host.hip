@@ -24,12 +24,25 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-// Temporary storage given to the device-wide scans, sized for n items: an
-// upper bound of the decoupled-lookback scan's needs (one 16-byte state per
-// 256 items plus padding).  The upload takes the larger of this and the
-// library's own query, so the byte count of a world is known without a
-// device.
-size_t cub_tmp_bound(long long n) { return (size_t)(16 * (n / 256 + 64) + 1024); }
+// Temporary storage given to the device-wide scans (n_scan items) and the
+// new-pair sort (n_sort keys, data in double buffers of its own): an upper
+// bound of their needs (a 16-byte lookback state per 256 scan items; digit
+// histograms and lookback states of the radix passes, well under a byte per
+// key).  The upload takes the larger of this and the library's own queries,
+// so the byte count of a world is known without a device.
+size_t cub_tmp_bound(long long n_scan, long long n_sort) {
+    return (size_t)std::max<long long>(16 * (n_scan / 256 + 64) + 1024, n_sort + (1 << 20));
+}
+
+// Capacity of the broad-phase candidate and new-pair buffers.
+long long pair_capacity(long long N) { return std::max<long long>(65536, 4 * N); }
+
+// Significant bits of the new-pair sort keys (i << 32 | j, i < j < N).
+int pair_key_bits(int N) {
+    int b = 0;
+    while (b < 31 && (1ll << b) < N) ++b;
+    return 32 + b;
+}
 
 #define HIP_TRY(expr)                                                                      \
     do {                                                                                   \
@@ -56,6 +69,7 @@ struct avbd_ctx {
     DevWorld w{};
     ManifoldBuf mb[2]{};
     int cur = 0;
+    int* host_word = nullptr;  // pinned: one status word read back mid-step
     int4* items = nullptr;  // broad-phase work items (scene, i-tile, j-tile)
     int n_items = 0;
     int n_tiles = 0;
@@ -285,7 +299,7 @@ int make_dims(const WorldCounts& k, long long mcap_req, WorldDims* d) {
     d->Mcap = int(mcap);
     d->n_tiles = int(k.tiles);
     d->n_items = int(k.items);
-    d->scan_bytes = cub_tmp_bound(std::max<long long>(k.N + 1, mcap + 1));
+    d->scan_bytes = cub_tmp_bound(std::max<long long>(k.N + 1, mcap + 1), pair_capacity(k.N));
     return 0;
 }
 
@@ -325,8 +339,8 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     A(&w.ev_a, (size_t)4 * T + 3 * TR + 1); A(&w.ev_b, (size_t)4 * T + 3 * TR + 1);
     A(&w.ev_c, (size_t)4 * T + 3 * TR + 1); A(&w.ev_v, (size_t)4 * T + 3 * TR + 1);
     A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
-    A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1); A(&w.seg_off, N + 1);
-    w.pair_cap = std::max(65536, 4 * N);
+    A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1);
+    w.pair_cap = (int)pair_capacity(N);
     A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
     int* toff;
     int* tscene;
@@ -334,6 +348,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     w.tile_off = toff;
     w.tile_scene = tscene;
     A(&w.tile_lo, std::max(d.n_tiles, 1)); A(&w.tile_hi, std::max(d.n_tiles, 1));
+    A(&w.chunk_lo, (size_t)8 * std::max(d.n_tiles, 1)); A(&w.chunk_hi, (size_t)8 * std::max(d.n_tiles, 1));
     c->n_tiles = d.n_tiles;
     c->n_items = d.n_items;
     A(&c->items, std::max(d.n_items, 1));
@@ -394,11 +409,13 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     int rc = make_dims(world_counts(in), mcap_req, &dims);
     if (rc) return rc;
     {
-        size_t b1 = 0, b2 = 0;
+        size_t b1 = 0, b2 = 0, b3 = 0;
         int* dummy = nullptr;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, dummy, dummy, N + 1, c->stream));
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, dummy, dummy, dims.Mcap + 1, c->stream));
-        dims.scan_bytes = std::max(dims.scan_bytes, std::max(b1, b2));
+        hipcub::DoubleBuffer<unsigned long long> kb(nullptr, nullptr);
+        HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b3, kb, (int)pair_capacity(N), 0, pair_key_bits(N), c->stream));
+        dims.scan_bytes = std::max(dims.scan_bytes, std::max(b1, std::max(b2, b3)));
     }
     DevWorld& w = c->w;
     avbd_scene* d_sc;
@@ -650,17 +667,24 @@ int do_step(avbd_ctx* c) {
     // ---- 1. broad phase
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_NEW_PAIRS, 0);
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_CANDIDATES, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(N + 1)), dim3(256), 0, st, w.cnt_i, N + 1, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(N + 1)), dim3(256), 0, st, w.cur_i, N + 1, 0);
+    int rc = 0;
     if (c->n_items > 0) {
         hipLaunchKernelGGL(k_tile_bounds, dim3(c->n_tiles), dim3(256), 0, st, w);
         hipLaunchKernelGGL(k_broad, dim3(c->n_items), dim3(256), 0, st, w, (const int4*)c->items);
         hipLaunchKernelGGL(k_pair_filter, dim3(grid_for(N)), dim3(256), 0, st, w);
+        // the new-pair count sizes the sort
+        HIP_TRY(hipMemcpyAsync(c->host_word, w.g + G_NEW_PAIRS, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const int K = std::min(*c->host_word, w.pair_cap);
+        if (K > 0) {
+            hipcub::DoubleBuffer<unsigned long long> kb(reinterpret_cast<unsigned long long*>(w.pairs),
+                                                         reinterpret_cast<unsigned long long*>(w.pairs_tmp));
+            size_t bytes = c->cub_bytes;
+            HIP_TRY(hipcub::DeviceRadixSort::SortKeys(c->cub_tmp, bytes, kb, K, 0, pair_key_bits(N), st));
+            hipLaunchKernelGGL(k_pair_append, dim3(grid_for(K)), dim3(256), 0, st, w,
+                               (const unsigned long long*)kb.Current());
+        }
     }
-    int rc = scan_ints(c, w.cnt_i, w.seg_off, N + 1);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_pair_scatter, dim3(grid_for(w.pair_cap)), dim3(256), 0, st, w);
-    hipLaunchKernelGGL(k_pair_rank_append, dim3(grid_for(w.pair_cap)), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_pair_finish, dim3(1), dim3(1), 0, st, w);
     HIP_TRY(hipEventRecord(c->ev[1], st));
     // ---- 2. force warm start, compaction, incidence, colouring
@@ -807,9 +831,10 @@ int avbd_create(int32_t device, avbd_ctx** out) {
     for (auto& e : c->ev) hipEventCreate(&e);
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void**)&c->host_word, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         avbd_destroy(c);
-        return fail(AVBD_E_HIP, "stream/event creation failed");
+        return fail(AVBD_E_HIP, "stream/event/pinned buffer creation failed");
     }
     *out = c;
     return 0;
@@ -827,6 +852,7 @@ int avbd_destroy(avbd_ctx* c) {
     if (c->join) hipEventDestroy(c->join);
     if (c->side) hipStreamDestroy(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->host_word) hipHostFree(c->host_word);
     delete c;
     return 0;
 }

@@ -1,7 +1,7 @@
 // avbd_kernels.hip -- HIP kernels of the device-resident AVBD step (gfx950).
 //
 // Step pipeline (Solver::Step, solver.cpp:199-627), all scenes at once:
-//   k_tile_bounds, k_broad -> k_pair_scatter -> k_pair_rank_append
+//   k_tile_bounds, k_broad, k_pair_filter -> sort -> k_pair_append
 //     -> k_pair_finish                                                 (phase 1)
 //   k_manifold_init, k_force_init -> compaction                       (phase 2)
 //   incidence lists, greedy colouring, colour lists
@@ -63,15 +63,24 @@ AVBD_DEV bool constrained(const DevWorld& w, int i, int j) {
 
 // Broad phase: the bounding-sphere test of every body pair of a scene
 // (solver.cpp:236-255), tiled.  A scene's bodies form tiles of 256 in index
-// order; one work item = (scene, i-tile, j-tile >= i-tile), the j-tile staged
-// in LDS.  k_tile_bounds gives each tile the box around its bodies' spheres,
-// widened by 1e-5 (|x| + r) per side (far more than the rounding of the
-// bounds), and an item whose two boxes are disjoint is skipped: no pair of
-// it can pass the exact test, so the pair set equals that of the full loop.
-// Overlapping pairs go to a candidate list (pairs_tmp); k_pair_filter then
-// drops those already linked by a force, one thread per candidate, and
-// appends the new pairs unordered (creation order is restored by rank
-// later); cnt_i counts per lower index.
+// order, each of 8 chunks of 32; one work item = (scene, i-tile, j-tile >=
+// i-tile), the j-tile staged in LDS.  k_tile_bounds gives every chunk and
+// tile the box around its bodies' spheres, widened by 1e-5 (|x| + r) per
+// side (far more than the rounding of the bounds).  An item whose tile
+// boxes are disjoint is skipped, and inside an item a wave skips the j
+// chunks disjoint from the box of its 64 i bodies: no pair skipped can pass
+// the exact test, so the pair set equals that of the full loop.
+// Overlapping pairs are gathered per block in LDS and go to a candidate list
+// (pairs_tmp) with one global atomic per block; k_pair_filter then drops
+// those already linked by a force, one thread per candidate, and appends the
+// new pairs unordered (creation order is restored by rank later); cnt_i
+// counts per lower index.
+#define BROAD_HIT_CAP 1024
+
+AVBD_DEV bool boxes_disjoint(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+    return ahi.x < blo.x || bhi.x < alo.x || ahi.y < blo.y || bhi.y < alo.y || ahi.z < blo.z || bhi.z < alo.z;
+}
+
 __global__ __launch_bounds__(256) void k_tile_bounds(DevWorld w) {
     const int t = blockIdx.x;
     const int s = w.tile_scene[t];
@@ -89,23 +98,27 @@ __global__ __launch_bounds__(256) void k_tile_bounds(DevWorld w) {
             hi[a] = (c[a] + r) + m;
         }
     }
+    // chunks of 32 lanes
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
+    for (int off = 16; off > 0; off >>= 1)
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
             hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
         }
-    __shared__ float red[4][6];
-    const int wave = threadIdx.x / 64;
-    if ((threadIdx.x & 63) == 0)
+    __shared__ float red[8][6];
+    const int chunk = threadIdx.x / 32;
+    if ((threadIdx.x & 31) == 0) {
+        w.chunk_lo[t * 8 + chunk] = make_float4(lo[0], lo[1], lo[2], 0.f);
+        w.chunk_hi[t * 8 + chunk] = make_float4(hi[0], hi[1], hi[2], 0.f);
         for (int a = 0; a < 3; ++a) {
-            red[wave][a] = lo[a];
-            red[wave][3 + a] = hi[a];
+            red[chunk][a] = lo[a];
+            red[chunk][3 + a] = hi[a];
         }
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; ++k)
+        for (int k = 1; k < 8; ++k)
             for (int a = 0; a < 3; ++a) {
                 lo[a] = fminf(lo[a], red[k][a]);
                 hi[a] = fmaxf(hi[a], red[k][3 + a]);
@@ -115,104 +128,110 @@ __global__ __launch_bounds__(256) void k_tile_bounds(DevWorld w) {
     }
 }
 
+AVBD_DEV void broad_append_global(const DevWorld& w, int2 pr) {
+    const int slot = atomicAdd(&w.g[G_CANDIDATES], 1);
+    if (slot < w.pair_cap) w.pairs_tmp[slot] = pr;
+    else atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
+}
+
 __global__ __launch_bounds__(256) void k_broad(DevWorld w, const int4* items) {
     const int4 it = items[blockIdx.x];
     const int s = it.x;
     if (!scene_runs(w, s)) return;
-    if (it.y != it.z) {  // the same decision for the whole block, before its barrier
-        const float4 alo = w.tile_lo[w.tile_off[s] + it.y], ahi = w.tile_hi[w.tile_off[s] + it.y];
-        const float4 blo = w.tile_lo[w.tile_off[s] + it.z], bhi = w.tile_hi[w.tile_off[s] + it.z];
-        if (ahi.x < blo.x || bhi.x < alo.x || ahi.y < blo.y || bhi.y < alo.y || ahi.z < blo.z || bhi.z < alo.z)
-            return;
-    }
+    const int ti = w.tile_off[s] + it.y, tj = w.tile_off[s] + it.z;
+    // the same decision for the whole block, before its barriers
+    if (it.y != it.z && boxes_disjoint(w.tile_lo[ti], w.tile_hi[ti], w.tile_lo[tj], w.tile_hi[tj])) return;
     const int off = w.sc_off[s];
     const int end = w.sc_off[s + 1];
     __shared__ float4 tile[256];
+    __shared__ float4 jlo[8], jhi[8];
+    __shared__ int2 hits[BROAD_HIT_CAP];
+    __shared__ int n_hits, base;
     const int j0 = off + it.z * 256;
     const int jl = j0 + threadIdx.x;
     if (jl < end) {
         float4 p = w.pos[jl];
         tile[threadIdx.x] = make_float4(p.x, p.y, p.z, w.scale[jl].w);
     }
+    if (threadIdx.x < 8) {
+        jlo[threadIdx.x] = w.chunk_lo[tj * 8 + threadIdx.x];
+        jhi[threadIdx.x] = w.chunk_hi[tj * 8 + threadIdx.x];
+    }
+    if (threadIdx.x == 0) n_hits = 0;
     __syncthreads();
+    const int wave = threadIdx.x / 64;
     const int i = off + it.y * 256 + threadIdx.x;
-    if (i >= end) return;
-    const float4 pi4 = w.pos[i];
-    const v3 pi = f3(pi4);
-    const float ri = w.scale[i].w;
     const int jn = min(256, end - j0);
-    const int lane = threadIdx.x & 63;
-    for (int t = 0; t < jn; ++t) {
-        const int j = j0 + t;
-        const float4 pj = tile[t];
-        v3 d = sub(pi, f3(pj));
-        float d2 = dot(d, d);
-        float rs = ri + pj.w;
-        bool hit = j > i && !(d2 > (rs * rs) * 1.00001f);  // conservative pre-test
-        if (hit) hit = sqrtf(d2) < rs;
-        // one atomic per wave for its hits at this t
-        const unsigned long long mask = __ballot(hit);
-        if (mask == 0ull) continue;
-        const int leader = __ffsll((long long)mask) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&w.g[G_CANDIDATES], __popcll(mask));
-        base = __shfl(base, leader);
-        if (hit) {
-            const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-            if (slot < w.pair_cap) w.pairs_tmp[slot] = make_int2(i, j);
-            else atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
+    if (off + it.y * 256 + wave * 64 < end) {  // the wave has bodies
+        const bool valid = i < end;
+        const float4 pi4 = valid ? w.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const v3 pi = f3(pi4);
+        const float ri = valid ? w.scale[i].w : 0.f;
+        const float4 c0l = w.chunk_lo[ti * 8 + 2 * wave], c1l = w.chunk_lo[ti * 8 + 2 * wave + 1];
+        const float4 c0h = w.chunk_hi[ti * 8 + 2 * wave], c1h = w.chunk_hi[ti * 8 + 2 * wave + 1];
+        const float4 ilo = make_float4(fminf(c0l.x, c1l.x), fminf(c0l.y, c1l.y), fminf(c0l.z, c1l.z), 0.f);
+        const float4 ihi = make_float4(fmaxf(c0h.x, c1h.x), fmaxf(c0h.y, c1h.y), fmaxf(c0h.z, c1h.z), 0.f);
+        for (int c = 0; c < 8 && c * 32 < jn; ++c) {
+            if (boxes_disjoint(ilo, ihi, jlo[c], jhi[c])) continue;
+            const int t1 = min(c * 32 + 32, jn);
+            for (int t = c * 32; t < t1; ++t) {
+                const int j = j0 + t;
+                const float4 pj = tile[t];
+                v3 d = sub(pi, f3(pj));
+                float d2 = dot(d, d);
+                float rs = ri + pj.w;
+                if (!valid || j <= i || d2 > (rs * rs) * 1.00001f) continue;  // conservative pre-test
+                if (sqrtf(d2) < rs) {
+                    const int k = atomicAdd(&n_hits, 1);
+                    if (k < BROAD_HIT_CAP) hits[k] = make_int2(i, j);
+                    else broad_append_global(w, make_int2(i, j));
+                }
+            }
         }
+    }
+    __syncthreads();
+    const int nh = min(n_hits, BROAD_HIT_CAP);
+    if (nh == 0) return;
+    if (threadIdx.x == 0) base = atomicAdd(&w.g[G_CANDIDATES], nh);
+    __syncthreads();
+    for (int k = threadIdx.x; k < nh; k += 256) {
+        const int slot = base + k;
+        if (slot < w.pair_cap) w.pairs_tmp[slot] = hits[k];
+        else atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
     }
 }
 
 // Keep the candidates not yet linked by a force (isConstrainedTo) as new
-// pairs.
+// pairs, written as sort keys (i << 32 | j) into the pair buffer.
 __global__ void k_pair_filter(DevWorld w) {
     const int C = min(w.g[G_CANDIDATES], w.pair_cap);
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(w.pairs);
     GRID_STRIDE(p, C) {
         const int2 pr = w.pairs_tmp[p];
         if (constrained(w, pr.x, pr.y)) continue;
         int slot = atomicAdd(&w.g[G_NEW_PAIRS], 1);
-        if (slot < w.pair_cap) {
-            w.pairs[slot] = pr;
-            atomicAdd(&w.cnt_i[pr.x], 1);
-        } else {
-            atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
-        }
+        if (slot < w.pair_cap) keys[slot] = ((unsigned long long)(unsigned)pr.x << 32) | (unsigned)pr.y;
+        else atomicExch(&w.g[G_ERR_PAIR_CAP], 1);
     }
 }
 
-// Group the new pairs by first body (seg_off = exclusive scan of cnt_i).
-__global__ void k_pair_scatter(DevWorld w) {
-    int K = min(w.g[G_NEW_PAIRS], w.pair_cap);
-    GRID_STRIDE(p, K) {
-        int2 pr = w.pairs[p];
-        int slot = w.seg_off[pr.x] + atomicAdd(&w.cur_i[pr.x], 1);
-        w.pairs_tmp[slot] = pr;
-    }
-}
-
-// Rank each pair inside its first-body segment by second body: the final
-// position is the pair's lexicographic rank, which is the creation order of
-// the reference's nested loop.  New manifolds are appended in that order.
-__global__ void k_pair_rank_append(DevWorld w) {
+// Append the new pairs as manifolds in lexicographic (i, j) order -- the
+// creation order of the reference's nested loop -- from the sorted keys.
+__global__ void k_pair_append(DevWorld w, const unsigned long long* keys) {
     const int K = min(w.g[G_NEW_PAIRS], w.pair_cap);
     const int M0 = w.g[G_MANIFOLDS];
     const long long base = *w.next_serial;
     GRID_STRIDE(p, K) {
-        int2 pr = w.pairs_tmp[p];
-        int lo = w.seg_off[pr.x], hi = w.seg_off[pr.x + 1];
-        int rank = 0;
-        for (int q = lo; q < hi; ++q) rank += (w.pairs_tmp[q].y < pr.y) ? 1 : 0;
-        int fin = lo + rank;
-        int m = M0 + fin;
+        const unsigned long long key = keys[p];
+        const int a = (int)(key >> 32), b = (int)(key & 0xFFFFFFFFull);
+        const int m = M0 + p;
         if (m < w.Mcap) {
-            w.m.a[m] = pr.x;
-            w.m.b[m] = pr.y;
-            w.m.serial[m] = base + fin;
+            w.m.a[m] = a;
+            w.m.b[m] = b;
+            w.m.serial[m] = base + p;
             w.m.n[m] = 0;
             w.m.fric[m] = 0.f;
-            w.st[w.scene[pr.x] * ST_WORDS + ST_DIRTY] = 1;
+            w.st[w.scene[a] * ST_WORDS + ST_DIRTY] = 1;
         }
     }
 }

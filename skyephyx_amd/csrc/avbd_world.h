@@ -154,7 +154,6 @@ struct DevWorld {
     int* queue;         // [2N]
     int* cnt_i;         // [N]
     int* cur_i;         // [N+1]
-    int* seg_off;       // [N+1]
     int2* pairs;        // [pair_cap]
     int2* pairs_tmp;    // [pair_cap]
     // broad-phase tiles (256 bodies of a scene in index order): first tile
@@ -163,6 +162,8 @@ struct DevWorld {
     const int* tile_scene;  // [tiles]
     float4* tile_lo;        // [tiles]
     float4* tile_hi;        // [tiles]
+    float4* chunk_lo;       // [tiles * 8]: the same for each chunk of 32
+    float4* chunk_hi;       // [tiles * 8]
     int pair_cap;
     int* alive;         // [Mcap]
     int* alive_off;     // [Mcap+1]

@@ -37,6 +37,7 @@ def compare_manifolds(g, o):
     lam_all = [np.abs(m["contacts"]["lambda"][: m["n_contacts"]]).max() for m in o.manifolds if m["n_contacts"]]
     lam_scale = max(lam_all) if lam_all else 1.0
     worst = 0.0
+    d2 = r2 = 0.0
     for key, mo in to.items():
         mg = tg.get(key)
         if mg is None:
@@ -51,7 +52,13 @@ def compare_manifolds(g, o):
         lo = mo["contacts"]["lambda"][:n].astype(np.float64)
         rel = np.abs(lg - lo) / (np.abs(lo) + 1e-3 * lam_scale)
         worst = max(worst, float(rel.max()) if rel.size else 0.0)
+        d2 += float(((lg - lo) ** 2).sum())
+        r2 += float((lo ** 2).sum())
+    # per entry: |dl| / (|l| + 1e-3 max|l|) -- a diagnostic, dominated by
+    # small friction rows whose FP32 rounding sensitivity can exceed 1e-3
     res["lambda_rel"] = worst
+    # norm-wise over every contact row of the step: ||dl|| / ||l||
+    res["lambda_norm_rel"] = float(np.sqrt(d2 / r2)) if r2 > 0 else float(np.sqrt(d2))
     return res
 
 
