@@ -26,6 +26,50 @@ AVBD_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 AVBD_DEV float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 AVBD_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// Vector helpers for the FEM terms with multiply-adds fused (the shared
+// helpers of avbd_math.h keep every product and sum separately rounded for
+// the contact code).
+namespace fc {
+AVBD_DEV float dot(v3 a, v3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+AVBD_DEV float norm(v3 a) { return fsqrt(dot(a, a)); }
+AVBD_DEV v3 lin2(float a, v3 u, float b, v3 v) {
+    return v3{fmaf(b, v.x, a * u.x), fmaf(b, v.y, a * u.y), fmaf(b, v.z, a * u.z)};
+}
+AVBD_DEV v3 lin3(float a, v3 u, float b, v3 v, float c, v3 w) {
+    return v3{fmaf(c, w.x, fmaf(b, v.x, a * u.x)), fmaf(c, w.y, fmaf(b, v.y, a * u.y)), fmaf(c, w.z, fmaf(b, v.z, a * u.z))};
+}
+AVBD_DEV v3 mv(const m3& a, v3 v) { return v3{dot(row(a, 0), v), dot(row(a, 1), v), dot(row(a, 2), v)}; }
+AVBD_DEV m3 mm(const m3& a, const m3& b) {
+    m3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            r.m[i][j] = fmaf(a.m[i][2], b.m[2][j], fmaf(a.m[i][1], b.m[1][j], a.m[i][0] * b.m[0][j]));
+    return r;
+}
+AVBD_DEV float det2(float a, float b, float c, float d) { return fmaf(a, d, -(b * c)); }  // a d - b c
+AVBD_DEV m3 cofactor(const m3& F) {
+    m3 c;
+    c.m[0][0] = det2(F.m[1][1], F.m[1][2], F.m[2][1], F.m[2][2]);
+    c.m[0][1] = -det2(F.m[1][0], F.m[1][2], F.m[2][0], F.m[2][2]);
+    c.m[0][2] = det2(F.m[1][0], F.m[1][1], F.m[2][0], F.m[2][1]);
+    c.m[1][0] = -det2(F.m[0][1], F.m[0][2], F.m[2][1], F.m[2][2]);
+    c.m[1][1] = det2(F.m[0][0], F.m[0][2], F.m[2][0], F.m[2][2]);
+    c.m[1][2] = -det2(F.m[0][0], F.m[0][1], F.m[2][0], F.m[2][1]);
+    c.m[2][0] = det2(F.m[0][1], F.m[0][2], F.m[1][1], F.m[1][2]);
+    c.m[2][1] = -det2(F.m[0][0], F.m[0][2], F.m[1][0], F.m[1][2]);
+    c.m[2][2] = det2(F.m[0][0], F.m[0][1], F.m[1][0], F.m[1][1]);
+    return c;
+}
+// det F from its first row and cofactors
+AVBD_DEV float det_cof(const m3& F, const m3& cof) {
+    return fmaf(F.m[0][2], cof.m[0][2], fmaf(F.m[0][1], cof.m[0][1], F.m[0][0] * cof.m[0][0]));
+}
+AVBD_DEV float det3(const m3& a) { return det_cof(a, cofactor(a)); }
+AVBD_DEV v3 cross(v3 a, v3 b) { return v3{det2(a.y, a.z, b.y, b.z), det2(a.z, a.x, b.z, b.x), det2(a.x, a.y, b.x, b.y)}; }
+}  // namespace fc
+
 AVBD_DEV void swap_cols(m3& a, int p, int q) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -127,7 +171,7 @@ AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
             const int p = (k == 2) ? 1 : 0;
             const int q = (k == 0) ? 1 : 2;
             v3 ap = col(A, p), aq = col(A, q);
-            float alpha = dot(ap, ap), beta = dot(aq, aq), gamma = dot(ap, aq);
+            float alpha = fc::dot(ap, ap), beta = fc::dot(aq, aq), gamma = fc::dot(ap, aq);
             if (gamma * gamma <= (4.f * AVBD_FLT_EPS * AVBD_FLT_EPS) * (alpha * beta) || gamma == 0.f) continue;
             rotated = true;
             float zeta = (beta - alpha) * frcp(2.f * gamma);
@@ -146,7 +190,7 @@ AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
         }
         if (!rotated) break;
     }
-    float s0 = norm(col(A, 0)), s1 = norm(col(A, 1)), s2 = norm(col(A, 2));
+    float s0 = fc::norm(col(A, 0)), s1 = fc::norm(col(A, 1)), s2 = fc::norm(col(A, 2));
     // stable descending sort
     if (s1 > s0) { float t = s0; s0 = s1; s1 = t; swap_cols(A, 0, 1); swap_cols(W, 0, 1); }
     if (s2 > s1) {
@@ -159,10 +203,10 @@ AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
     if (s1 > tiny) u1 = mul(u1, frcp(s1));
     else {
         v3 e = fabsf(u0.x) < 0.9f ? v3{1.f, 0.f, 0.f} : v3{0.f, 1.f, 0.f};
-        u1 = cross(u0, e);
-        u1 = mul(u1, frsq(dot(u1, u1)));
+        u1 = fc::cross(u0, e);
+        u1 = mul(u1, frsq(fc::dot(u1, u1)));
     }
-    if (s2 > tiny) u2 = mul(u2, frcp(s2)); else u2 = cross(u0, u1);
+    if (s2 > tiny) u2 = mul(u2, frcp(s2)); else u2 = fc::cross(u0, u1);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         U.m[r][0] = comp(u0, r);
@@ -170,12 +214,12 @@ AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
         U.m[r][2] = comp(u2, r);
     }
     V = W;
-    if (det3(U) < 0.f) {
+    if (fc::det3(U) < 0.f) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) U.m[r][2] = -U.m[r][2];
         s2 = -s2;
     }
-    if (det3(V) < 0.f) {
+    if (fc::det3(V) < 0.f) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) V.m[r][2] = -V.m[r][2];
         s2 = -s2;
@@ -195,12 +239,13 @@ AVBD_DEV float project_ev(float e, int mode) {
 
 AVBD_DEV void add_outer(float Hv[6], float l, v3 w) {
     // packed lower 3x3: (0,0) (1,0) (1,1) (2,0) (2,1) (2,2)
-    Hv[0] = Hv[0] + l * (w.x * w.x);
-    Hv[1] = Hv[1] + l * (w.y * w.x);
-    Hv[2] = Hv[2] + l * (w.y * w.y);
-    Hv[3] = Hv[3] + l * (w.z * w.x);
-    Hv[4] = Hv[4] + l * (w.z * w.y);
-    Hv[5] = Hv[5] + l * (w.z * w.z);
+    const v3 lw = smul(l, w);
+    Hv[0] = fmaf(lw.x, w.x, Hv[0]);
+    Hv[1] = fmaf(lw.y, w.x, Hv[1]);
+    Hv[2] = fmaf(lw.y, w.y, Hv[2]);
+    Hv[3] = fmaf(lw.z, w.x, Hv[3]);
+    Hv[4] = fmaf(lw.z, w.y, Hv[4]);
+    Hv[5] = fmaf(lw.z, w.z, Hv[5]);
 }
 
 // Gradient g and projected Hessian H (packed lower 3x3) of tet energy with
@@ -213,35 +258,65 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
     Ds.m[0][0] = e1.x; Ds.m[1][0] = e1.y; Ds.m[2][0] = e1.z;
     Ds.m[0][1] = e2.x; Ds.m[1][1] = e2.y; Ds.m[2][1] = e2.z;
     Ds.m[0][2] = e3.x; Ds.m[1][2] = e3.y; Ds.m[2][2] = e3.z;
-    m3 F = mm(Ds, DmInv);
-    float J = det3(F);
+    const m3 F = fc::mm(Ds, DmInv);
+    const m3 cof = fc::cofactor(F);
+    const float J = fc::det_cof(F, cof);
+    const v3 cg = fc::mv(cof, gradN);  // dJ/dx of the vertex
     if (J <= 1e-6f) {  // inverted element (neohookeanfem.cpp:86-110)
         float stiff = mu + lam;
-        v3 dJ = mv(cofactor(F), gradN);
         float pm = stiff * (1e-6f - J);
-        g = smul(-V0 * pm, dJ);
+        g = smul(-V0 * pm, cg);
         float d = V0 * stiff;
         H[0] = d; H[1] = 0.f; H[2] = d; H[3] = 0.f; H[4] = 0.f; H[5] = d;
         return;
     }
-    float logJ = logf(J);
+    const float logJ = logf(J);
     const float invJ = frcp(J);
-    float scal = (lam * logJ - mu) * invJ;
-    m3 cof = cofactor(F);
-    m3 P;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) P.m[i][j] = mu * F.m[i][j] + scal * cof.m[i][j];
-    g = mul(mv(P, gradN), V0);
+    const float dPhi = (lam * logJ - mu) * invJ;
+    // P gradN = mu F gradN + dPhi cof gradN
+    g = smul(V0, fc::lin2(mu, fc::mv(F, gradN), dPhi, cg));
+    const float diagExtra = mu + lam * (1.f - logJ);
+    // Whole Hessian positive definite (every eigenvalue >= 1e-6, so the
+    // projection leaves it unchanged).  With dPhi <= 0 (lam log J <= mu) the
+    // flip modes mu - dPhi s_k are >= mu, the scaling block
+    // mu I + (mu - lam log J) S^-2 + lam S^-1 1 1^T S^-1 is >= mu, and the
+    // twist modes mu + dPhi s_k are >= 1e-6 iff s_max < T = (mu - 1e-6) / -dPhi,
+    // i.e. T^2 I - F^T F is positive definite.  The vertex block of the
+    // unprojected Hessian mu I + c1 cof cof^T + dPhi d2J/dF2 is then
+    // mu |gradN|^2 I + c1 (cof gradN)(cof gradN)^T, c1 = diagExtra / J^2:
+    // the d2J term vanishes on a single vertex (dF = dx gradN^T is rank one
+    // and det(F + t dx gradN^T) is affine in t).
+    if (mu > 1e-6f && dPhi <= 0.f) {
+        bool pd = dPhi == 0.f;
+        if (!pd) {
+            const float T = (mu - 1e-6f) * frcp(-dPhi);
+            const float T2 = T * T;
+            const v3 f0 = col(F, 0), f1 = col(F, 1), f2 = col(F, 2);
+            const float m00 = T2 - fc::dot(f0, f0), m11 = T2 - fc::dot(f1, f1), m22 = T2 - fc::dot(f2, f2);
+            const float m01 = -fc::dot(f0, f1), m02 = -fc::dot(f0, f2), m12 = -fc::dot(f1, f2);
+            const float d01 = m00 * m11 - m01 * m01;
+            const float dM = m00 * (m11 * m22 - m12 * m12) - m01 * (m01 * m22 - m12 * m02) + m02 * (m01 * m12 - m11 * m02);
+            pd = m00 > 0.f && d01 > 0.f && dM > 0.f;
+        }
+        if (pd) {
+            const float dg = mu * fc::dot(gradN, gradN);
+            const float c1 = diagExtra * invJ * invJ;
+            const v3 cc = smul(c1, cg);
+            H[0] = V0 * fmaf(cc.x, cg.x, dg);
+            H[1] = V0 * (cc.y * cg.x);
+            H[2] = V0 * fmaf(cc.y, cg.y, dg);
+            H[3] = V0 * (cc.z * cg.x);
+            H[4] = V0 * (cc.z * cg.y);
+            H[5] = V0 * fmaf(cc.z, cg.z, dg);
+            return;
+        }
+    }
     m3 U, V;
     v3 S;
     svd3(F, U, S, V);
     const float s0 = S.x, s1 = S.y, s2 = S.z;
-    float dPhi = scal;
     float ev3 = mu + dPhi * s2, ev4 = mu + dPhi * s1, ev5 = mu + dPhi * s0;
     float ev6 = mu - dPhi * s2, ev7 = mu - dPhi * s1, ev8 = mu - dPhi * s0;
-    float diagExtra = mu + lam * (1.f - logJ);
     m3 a;
     a.m[0][0] = mu + diagExtra * frcp(s0 * s0);
     a.m[1][1] = mu + diagExtra * frcp(s1 * s1);
@@ -254,7 +329,7 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
 #pragma unroll
     for (int i = 3; i < 9; ++i) ev[i] = project_ev(ev[i], projMode);
     v3 u0 = col(U, 0), u1 = col(U, 1), u2 = col(U, 2);
-    float vg0 = dot(col(V, 0), gradN), vg1 = dot(col(V, 1), gradN), vg2 = dot(col(V, 2), gradN);
+    float vg0 = fc::dot(col(V, 0), gradN), vg1 = fc::dot(col(V, 1), gradN), vg2 = fc::dot(col(V, 2), gradN);
     float Hv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     // Scaling modes: with a = Va diag(aw) Va^T and the columns vg_j u_j of
     // Ut, their part of the vertex block is Ut Va proj(aw) Va^T Ut^T.  Both
@@ -271,9 +346,9 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
     if (b00 > 0.f && m01 > 0.f && dB > 0.f) {
         const v3 t0 = smul(vg0, u0), t1 = smul(vg1, u1), t2 = smul(vg2, u2);
         // rows of Ut a: r_j = sum_k a_jk t_k; block = sum_j t_j r_j^T
-        const v3 r0 = add(add(smul(a.m[0][0], t0), smul(a.m[0][1], t1)), smul(a.m[0][2], t2));
-        const v3 r1 = add(add(smul(a.m[1][0], t0), smul(a.m[1][1], t1)), smul(a.m[1][2], t2));
-        const v3 r2 = add(add(smul(a.m[2][0], t0), smul(a.m[2][1], t1)), smul(a.m[2][2], t2));
+        const v3 r0 = fc::lin3(a.m[0][0], t0, a.m[0][1], t1, a.m[0][2], t2);
+        const v3 r1 = fc::lin3(a.m[1][0], t0, a.m[1][1], t1, a.m[1][2], t2);
+        const v3 r2 = fc::lin3(a.m[2][0], t0, a.m[2][1], t1, a.m[2][2], t2);
         Hv[0] = (t0.x * r0.x + t1.x * r1.x) + t2.x * r2.x;
         Hv[1] = (t0.y * r0.x + t1.y * r1.x) + t2.y * r2.x;
         Hv[2] = (t0.y * r0.y + t1.y * r1.y) + t2.y * r2.y;
@@ -288,17 +363,19 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             v3 c = col(av, k);
-            v3 w = add(add(smul(c.x * vg0, u0), smul(c.y * vg1, u1)), smul(c.z * vg2, u2));
+            v3 w = fc::lin3(c.x * vg0, u0, c.y * vg1, u1, c.z * vg2, u2);
             add_outer(Hv, pw[k], w);
         }
     }
+    // twist (ev3-5) and flip (ev6-8) modes: w = (vg_j u_i -+ vg_i u_j) / sqrt 2
     const float is2 = 0.70710678118f;
-    add_outer(Hv, ev[3], smul(is2, sub(smul(vg1, u0), smul(vg0, u1))));
-    add_outer(Hv, ev[4], smul(is2, sub(smul(vg2, u0), smul(vg0, u2))));
-    add_outer(Hv, ev[5], smul(is2, sub(smul(vg2, u1), smul(vg1, u2))));
-    add_outer(Hv, ev[6], smul(is2, add(smul(vg1, u0), smul(vg0, u1))));
-    add_outer(Hv, ev[7], smul(is2, add(smul(vg2, u0), smul(vg0, u2))));
-    add_outer(Hv, ev[8], smul(is2, add(smul(vg2, u1), smul(vg1, u2))));
+    const float a0 = is2 * vg0, a1 = is2 * vg1, a2 = is2 * vg2;
+    add_outer(Hv, ev[3], fc::lin2(a1, u0, -a0, u1));
+    add_outer(Hv, ev[4], fc::lin2(a2, u0, -a0, u2));
+    add_outer(Hv, ev[5], fc::lin2(a2, u1, -a1, u2));
+    add_outer(Hv, ev[6], fc::lin2(a1, u0, a0, u1));
+    add_outer(Hv, ev[7], fc::lin2(a2, u0, a0, u2));
+    add_outer(Hv, ev[8], fc::lin2(a2, u1, a1, u2));
 #pragma unroll
     for (int i = 0; i < 6; ++i) H[i] = Hv[i] * V0;
 }

@@ -186,6 +186,52 @@ def test_neohookean_vertex_terms_match_fd_hessian(vtx):
         np.testing.assert_allclose(H, H_ref, rtol=2e-3, atol=2e-3 * np.abs(H_ref).max())
 
 
+@pytest.mark.parametrize("vtx", [0, 1, 2, 3])
+def test_neohookean_positive_definite_vertex_block_closed_form(vtx):
+    """When every eigenvalue of the Neo-Hookean Hessian is >= 1e-6 the
+    projection is the identity and the vertex block is
+    V0 (mu |gN|^2 I + (mu + lam (1 - log J)) / J^2 (cof F gN)(cof F gN)^T)
+    (the d2J/dF2 term vanishes on one vertex).  In the common case
+    lam log J <= mu that holds iff s_max(F) < (mu - 1e-6) / -dPhi.  The
+    oracle's eigen-based projected block must equal the closed form there."""
+    mu, lam = 38461.5, 57692.3
+    rng = np.random.default_rng(5 + vtx)
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64) * 0.5
+    Dm = np.stack([base[1] - base[0], base[2] - base[0], base[3] - base[0]], axis=1)
+    Dminv = np.linalg.inv(Dm)
+    V0 = abs(np.linalg.det(Dm)) / 6.0
+    gN = np.concatenate([[-Dminv.sum(axis=0)], Dminv])[vtx]
+    n_pd = n_not = 0
+    for k in range(40):
+        R, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        # half near-uniform expansions (positive definite), half general
+        s = rng.uniform(1.02, 1.25) * rng.uniform(0.97, 1.03, 3) if k % 2 else rng.uniform(0.75, 1.1, 3)
+        F = R @ np.diag(s) @ Q.T
+        if np.linalg.det(F) < 0:
+            F[:, 0] = -F[:, 0]
+        p = base @ F.T
+        J = np.linalg.det(F)
+        dphi = (lam * np.log(J) - mu) / J
+        if dphi > 0:  # the closed form is only used for lam log J <= mu
+            continue
+        T = (mu - 1e-6) / -dphi
+        pd = np.linalg.eigvalsh(F.T @ F).max() < T * T
+        g, H = O.neohookean_terms(p, Dminv, V0, mu, lam, vtx)
+        cof = J * np.linalg.inv(F).T
+        cg = cof @ gN
+        c1 = (mu + lam * (1.0 - np.log(J))) / (J * J)
+        H_closed = V0 * (mu * (gN @ gN) * np.eye(3) + c1 * np.outer(cg, cg))
+        # the oracle's Jacobi tolerances are FLT_EPS-based even in FP64
+        if pd:
+            n_pd += 1
+            np.testing.assert_allclose(H, H_closed, rtol=2e-6, atol=2e-6 * np.abs(H_closed).max())
+        else:
+            n_not += 1
+            assert np.abs(H - H_closed).max() > 1e-4 * np.abs(H_closed).max()
+    assert n_pd >= 5 and n_not >= 5, (n_pd, n_not)
+
+
 def test_neohookean_rest_state():
     base, _, Dminv, V0 = random_tet()
     g, H = O.neohookean_terms(base, Dminv, V0, 1.0, 2.0, 1)
