@@ -83,6 +83,7 @@ struct avbd_ctx {
     // host mirrors of the static parts
     std::vector<avbd_scene> scenes;
     std::vector<int> sc_off;
+    std::vector<int32_t> tet_serial, tri_serial;  // creation serials of the device's elements
     std::vector<int32_t> host_colours;
     hipEvent_t ev[8]{};
     avbd_timings timings{};
@@ -364,6 +365,106 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     return 0;
 }
 
+// Energy records -> device: the element arrays, the per-vertex incidence
+// lists in creation (serial) order and the incidence-ordered element stream.
+// Vertex indices must be < N.
+int upload_energies(avbd_ctx* c, int N, const avbd_tet* tets, int T, const avbd_tri* tris, int TR) {
+    DevWorld& w = c->w;
+    struct ERef { int serial, kind, idx; };
+    std::vector<ERef> refs;
+    refs.reserve(T + TR);
+    for (int i = 0; i < T; ++i) refs.push_back({tets[i].serial, 0, i});
+    for (int i = 0; i < TR; ++i) refs.push_back({tris[i].serial, 1, i});
+    std::stable_sort(refs.begin(), refs.end(), [](const ERef& x, const ERef& y) { return x.serial < y.serial; });
+    std::vector<int> cnt(N + 1, 0);
+    for (const ERef& r : refs) {
+        if (r.kind == 0) for (int k = 0; k < 4; ++k) cnt[tets[r.idx].v[k]]++;
+        else for (int k = 0; k < 3; ++k) cnt[tris[r.idx].v[k]]++;
+    }
+    std::vector<int> eoff(N + 1, 0);
+    for (int i = 0; i < N; ++i) eoff[i + 1] = eoff[i] + cnt[i];
+    std::vector<uint32_t> ent(std::max(1, eoff[N]));
+    std::vector<int> cur(eoff.begin(), eoff.end() - 1);
+    for (const ERef& r : refs) {
+        if (r.kind == 0) {
+            for (int k = 0; k < 4; ++k) ent[cur[tets[r.idx].v[k]]++] = MAKE_EENT(0, k, r.idx);
+        } else {
+            for (int k = 0; k < 3; ++k) ent[cur[tris[r.idx].v[k]]++] = MAKE_EENT(1, k, r.idx);
+        }
+    }
+    std::vector<int4> tv(T), rv(TR);
+    std::vector<float4> td((size_t)T * 3), rd((size_t)TR * 2);
+    for (int i = 0; i < T; ++i) {
+        const avbd_tet& t = tets[i];
+        tv[i] = make_int4(t.v[0], t.v[1], t.v[2], t.v[3]);
+        const float* d = t.dm_inv;
+        td[i * 3 + 0] = make_float4(d[0], d[1], d[2], d[3]);
+        td[i * 3 + 1] = make_float4(d[4], d[5], d[6], d[7]);
+        td[i * 3 + 2] = make_float4(d[8], t.rest_volume, t.mu, t.lambda);
+    }
+    for (int i = 0; i < TR; ++i) {
+        const avbd_tri& t = tris[i];
+        rv[i] = make_int4(t.v[0], t.v[1], t.v[2], 0);
+        rd[i * 2 + 0] = make_float4(t.dm_inv[0], t.dm_inv[1], t.dm_inv[2], t.dm_inv[3]);
+        rd[i * 2 + 1] = make_float4(t.rest_area, t.mu, t.lambda, 0.f);
+    }
+    // the element stream in incidence order
+    const size_t nE = ent.size();
+    std::vector<float4> ea(nE), eb(nE), ec(nE, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<int4> evv(nE, make_int4(0, 0, 0, 0));
+    for (int k = 0; k < eoff[N]; ++k) {
+        const uint32_t e = ent[k];
+        const uint32_t idx = e & 0x1FFFFFFFu;
+        if (e >> 31) {
+            ea[k] = rd[idx * 2 + 0];
+            eb[k] = rd[idx * 2 + 1];
+            evv[k] = rv[idx];
+        } else {
+            ea[k] = td[idx * 3 + 0];
+            eb[k] = td[idx * 3 + 1];
+            ec[k] = td[idx * 3 + 2];
+            evv[k] = tv[idx];
+        }
+    }
+    int rc;
+    if ((rc = upload_vec(c, const_cast<int*>(w.e_off), eoff)) || (rc = upload_vec(c, const_cast<uint32_t*>(w.e_ent), ent)) ||
+        (rc = upload_vec(c, w.t_v, tv)) || (rc = upload_vec(c, w.t_d, td)) || (rc = upload_vec(c, w.r_v, rv)) ||
+        (rc = upload_vec(c, w.r_d, rd)) || (rc = upload_vec(c, w.ev_a, ea)) || (rc = upload_vec(c, w.ev_b, eb)) ||
+        (rc = upload_vec(c, w.ev_c, ec)) || (rc = upload_vec(c, w.ev_v, evv)))
+        return rc;
+    w.T = T;
+    w.TR = TR;
+    c->tet_serial.resize(T);
+    c->tri_serial.resize(TR);
+    for (int i = 0; i < T; ++i) c->tet_serial[i] = tets[i].serial;
+    for (int i = 0; i < TR; ++i) c->tri_serial[i] = tris[i].serial;
+    return 0;
+}
+
+// Broad-phase tiles and work items (scene, i-tile, j-tile >= i-tile) of the
+// scenes with body offsets `off`.
+int upload_tiles(avbd_ctx* c, const std::vector<int>& off) {
+    DevWorld& w = c->w;
+    const int S = (int)off.size() - 1;
+    std::vector<int> toff(S + 1, 0), tscene;
+    std::vector<int4> items;
+    for (int s = 0; s < S; ++s) {
+        const int tiles = (int)broad_tiles(off[s + 1] - off[s]);
+        toff[s + 1] = toff[s] + tiles;
+        for (int ti = 0; ti < tiles; ++ti) {
+            tscene.push_back(s);
+            for (int tj = ti; tj < tiles; ++tj) items.push_back(make_int4(s, ti, tj, 0));
+        }
+    }
+    int rc;
+    if ((rc = upload_vec(c, const_cast<int*>(w.tile_off), toff)) ||
+        (rc = upload_vec(c, const_cast<int*>(w.tile_scene), tscene)) || (rc = upload_vec(c, c->items, items)))
+        return rc;
+    c->n_tiles = (int)tscene.size();
+    c->n_items = (int)items.size();
+    return 0;
+}
+
 int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     const int S = in->n_scenes, N = in->n_bodies, J = in->n_joints, SP = in->n_springs;
     const int M = in->n_manifolds, T = in->n_tets, TR = in->n_tris;
@@ -541,70 +642,7 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
             (rc = upload_vec(c, m0.cold, cold)))
             return rc;
     }
-    {
-        // energy incidence: per vertex, energies in creation (serial) order
-        struct ERef { int serial, kind, idx; };
-        std::vector<ERef> refs;
-        refs.reserve(T + TR);
-        for (int i = 0; i < T; ++i) refs.push_back({in->tets[i].serial, 0, i});
-        for (int i = 0; i < TR; ++i) refs.push_back({in->tris[i].serial, 1, i});
-        std::stable_sort(refs.begin(), refs.end(), [](const ERef& x, const ERef& y) { return x.serial < y.serial; });
-        std::vector<int> cnt(N + 1, 0);
-        for (const ERef& r : refs) {
-            if (r.kind == 0) for (int k = 0; k < 4; ++k) cnt[in->tets[r.idx].v[k]]++;
-            else for (int k = 0; k < 3; ++k) cnt[in->tris[r.idx].v[k]]++;
-        }
-        std::vector<int> eoff(N + 1, 0);
-        for (int i = 0; i < N; ++i) eoff[i + 1] = eoff[i] + cnt[i];
-        std::vector<uint32_t> ent(std::max(1, eoff[N]));
-        std::vector<int> cur(eoff.begin(), eoff.end() - 1);
-        for (const ERef& r : refs) {
-            if (r.kind == 0) {
-                for (int k = 0; k < 4; ++k) ent[cur[in->tets[r.idx].v[k]]++] = MAKE_EENT(0, k, r.idx);
-            } else {
-                for (int k = 0; k < 3; ++k) ent[cur[in->tris[r.idx].v[k]]++] = MAKE_EENT(1, k, r.idx);
-            }
-        }
-        std::vector<int4> tv(T), rv(TR);
-        std::vector<float4> td((size_t)T * 3), rd((size_t)TR * 2);
-        for (int i = 0; i < T; ++i) {
-            const avbd_tet& t = in->tets[i];
-            tv[i] = make_int4(t.v[0], t.v[1], t.v[2], t.v[3]);
-            const float* d = t.dm_inv;
-            td[i * 3 + 0] = make_float4(d[0], d[1], d[2], d[3]);
-            td[i * 3 + 1] = make_float4(d[4], d[5], d[6], d[7]);
-            td[i * 3 + 2] = make_float4(d[8], t.rest_volume, t.mu, t.lambda);
-        }
-        for (int i = 0; i < TR; ++i) {
-            const avbd_tri& t = in->tris[i];
-            rv[i] = make_int4(t.v[0], t.v[1], t.v[2], 0);
-            rd[i * 2 + 0] = make_float4(t.dm_inv[0], t.dm_inv[1], t.dm_inv[2], t.dm_inv[3]);
-            rd[i * 2 + 1] = make_float4(t.rest_area, t.mu, t.lambda, 0.f);
-        }
-        // the element stream in incidence order
-        const size_t nE = ent.size();
-        std::vector<float4> ea(nE), eb(nE), ec(nE, make_float4(0.f, 0.f, 0.f, 0.f));
-        std::vector<int4> evv(nE, make_int4(0, 0, 0, 0));
-        for (int k = 0; k < eoff[N]; ++k) {
-            const uint32_t e = ent[k];
-            const uint32_t idx = e & 0x1FFFFFFFu;
-            if (e >> 31) {
-                ea[k] = rd[idx * 2 + 0];
-                eb[k] = rd[idx * 2 + 1];
-                evv[k] = rv[idx];
-            } else {
-                ea[k] = td[idx * 3 + 0];
-                eb[k] = td[idx * 3 + 1];
-                ec[k] = td[idx * 3 + 2];
-                evv[k] = tv[idx];
-            }
-        }
-        if ((rc = upload_vec(c, e_off, eoff)) || (rc = upload_vec(c, e_ent, ent)) || (rc = upload_vec(c, w.t_v, tv)) ||
-            (rc = upload_vec(c, w.t_d, td)) || (rc = upload_vec(c, w.r_v, rv)) || (rc = upload_vec(c, w.r_d, rd)) ||
-            (rc = upload_vec(c, w.ev_a, ea)) || (rc = upload_vec(c, w.ev_b, eb)) || (rc = upload_vec(c, w.ev_c, ec)) ||
-            (rc = upload_vec(c, w.ev_v, evv)))
-            return rc;
-    }
+    if ((rc = upload_energies(c, N, in->tets, T, in->tris, TR))) return rc;
     {
         std::vector<int> g(G_WORDS, 0);
         g[G_MANIFOLDS] = M;
@@ -612,26 +650,8 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
         std::vector<long long> ns(1, max_serial + 1);
         if ((rc = upload_vec(c, w.next_serial, ns))) return rc;
     }
-    // broad-phase tiles and work items (scene, i-tile, j-tile >= i-tile)
-    {
-        std::vector<int> toff(S + 1, 0), tscene;
-        std::vector<int4> items;
-        tscene.reserve(dims.n_tiles);
-        items.reserve(dims.n_items);
-        for (int s = 0; s < S; ++s) {
-            const int tiles = (int)broad_tiles(off[s + 1] - off[s]);
-            toff[s + 1] = toff[s] + tiles;
-            for (int ti = 0; ti < tiles; ++ti) {
-                tscene.push_back(s);
-                for (int tj = ti; tj < tiles; ++tj) items.push_back(make_int4(s, ti, tj, 0));
-            }
-        }
-        if ((int)tscene.size() != dims.n_tiles || (int)items.size() != dims.n_items)
-            return fail(AVBD_E_ARG, "broad-phase tile count mismatch");
-        if ((rc = upload_vec(c, const_cast<int*>(w.tile_off), toff)) ||
-            (rc = upload_vec(c, const_cast<int*>(w.tile_scene), tscene)) || (rc = upload_vec(c, c->items, items)))
-            return rc;
-    }
+    if ((rc = upload_tiles(c, off))) return rc;
+    if (c->n_tiles != dims.n_tiles || c->n_items != dims.n_items) return fail(AVBD_E_ARG, "broad-phase tile count mismatch");
     c->have_world = true;
     if ((rc = build_incidence(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -651,6 +671,215 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     hipEventElapsedTime(&ms, a, b);
     return ms;
+}
+
+// Greedy colouring of the dirty running scenes and the sweep bucket lists;
+// reads back the status words (g) and the bucket offsets (c->bucket_off).
+int colour_and_buckets(avbd_ctx* c, int* g) {
+    DevWorld& w = c->w;
+    hipStream_t st = c->stream;
+    const int N = w.N;
+    hipLaunchKernelGGL(k_colour, dim3(w.S), dim3(1024), 0, st, w);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, w.col_cnt, AVBD_BUCKETS + 1, 0);
+    hipLaunchKernelGGL(k_colour_hist, dim3(grid_for(N)), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_colour_scan, dim3(1), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_colour_fill, dim3(grid_for(N)), dim3(256), 0, st, w);
+    HIP_TRY(hipMemcpyAsync(g, w.g, G_WORDS * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(c->bucket_off.data(), w.col_off, (AVBD_BUCKETS + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return check_errors(c, g);
+}
+
+// Post-stabilisation sweep (solver.cpp:536-597) of the scenes marked in
+// `only` (the others are made inactive meanwhile), with their colouring
+// recomputed first.
+int poststab_scenes(avbd_ctx* c, const std::vector<char>& only) {
+    DevWorld& w = c->w;
+    const int S = w.S;
+    std::vector<int> st;
+    int rc;
+    if ((rc = download_vec(c, st, w.st, (size_t)S * ST_WORDS))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<int> active(S);
+    bool any_post = false;
+    for (int s = 0; s < S; ++s) {
+        active[s] = st[s * ST_WORDS + ST_ACTIVE];
+        st[s * ST_WORDS + ST_ACTIVE] = only[s] ? active[s] : 0;
+        any_post = any_post || (only[s] && c->scenes[s].post_stabilization);
+    }
+    if ((rc = upload_vec(c, w.st, st))) return rc;
+    int g[G_WORDS];
+    if ((rc = colour_and_buckets(c, g))) return rc;
+    if (any_post) {
+        for (int col = 0; col < g[G_NCOLOURS]; ++col)
+            if ((rc = launch_colour(c, c->bucket_off.data(), col, -1, 1))) return rc;
+        hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, c->stream, w);
+    }
+    // restore the active set
+    if ((rc = download_vec(c, st, w.st, (size_t)S * ST_WORDS))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < S; ++s) st[s * ST_WORDS + ST_ACTIVE] = active[s];
+    if ((rc = upload_vec(c, w.st, st))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Out-of-bounds removal (solver.cpp:520-534) for the scenes k_velocity
+// flagged, then their post-stabilisation, which the step skipped for them.
+// The removal follows Solver::RemoveBody: the bodies that left the bounds, in
+// index order, each replaced by the scene's last body (swap with last,
+// solver.cpp:104-116); their forces and energies go with them (~Mesh,
+// geometry.cpp:11-24).  The device state is renumbered in place, so the
+// step's own state (inertial targets, poses at step start, contact rows)
+// carries into the post-stabilisation.  A scene whose explosion check fired
+// removes nothing (the reference returns before, solver.cpp:528-529).
+int remove_out_of_bounds(avbd_ctx* c) {
+    DevWorld& w = c->w;
+    hipStream_t st = c->stream;
+    const int S = w.S, N = w.N;
+    std::vector<int> stw;
+    std::vector<float4> pos;
+    std::vector<uint32_t> flags;
+    int rc;
+    if ((rc = download_vec(c, stw, w.st, (size_t)S * ST_WORDS)) || (rc = download_vec(c, pos, w.pos, N)) ||
+        (rc = download_vec(c, flags, w.flags, N)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<char> aff(S, 0);
+    bool any = false;
+    for (int s = 0; s < S; ++s) {
+        int* sw = &stw[s * ST_WORDS];
+        if (sw[ST_OOB] > 0 && !sw[ST_EMERGENCY] && sw[ST_ACTIVE]) {
+            aff[s] = 1;
+            any = true;
+        }
+        sw[ST_OOB] = 0;
+    }
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_ANY_OOB, 0);
+    if (!any) {
+        if ((rc = upload_vec(c, w.st, stw))) return rc;
+        HIP_TRY(hipStreamSynchronize(st));
+        return 0;
+    }
+    // surviving order per scene
+    std::vector<int> perm, map(N, -1), noff(S + 1, 0);
+    perm.reserve(N);
+    for (int s = 0; s < S; ++s) {
+        const int a = c->sc_off[s], e = c->sc_off[s + 1];
+        std::vector<int> order(e - a), where(e - a);
+        for (int i = a; i < e; ++i) {
+            order[i - a] = i;
+            where[i - a] = i - a;
+        }
+        if (aff[s]) {
+            const avbd_scene& sc = c->scenes[s];
+            int removed = 0;
+            for (int i = a; i < e; ++i) {
+                if (flags[i] & AVBD_BODY_STATIC) continue;
+                const float4 p = pos[i];
+                if (p.x < sc.bounds_min[0] || p.x > sc.bounds_max[0] || p.y < sc.bounds_min[1] ||
+                    p.y > sc.bounds_max[1] || p.z < sc.bounds_min[2] || p.z > sc.bounds_max[2]) {
+                    const int k = where[i - a], last = (int)order.size() - 1;
+                    if (k != last) {
+                        order[k] = order[last];
+                        where[order[k] - a] = k;
+                    }
+                    order.pop_back();
+                    where[i - a] = -1;
+                    ++removed;
+                }
+            }
+            stw[s * ST_WORDS + ST_REMOVED] += removed;
+            stw[s * ST_WORDS + ST_DIRTY] = 1;
+        }
+        for (int o : order) {
+            map[o] = (int)perm.size();
+            perm.push_back(o);
+        }
+        noff[s + 1] = (int)perm.size();
+    }
+    const int N2 = (int)perm.size();
+    // energies that keep all their vertices, renumbered, in creation order
+    std::vector<int4> tv, rv;
+    if ((rc = download_vec(c, tv, w.t_v, w.T)) || (rc = download_vec(c, rv, w.r_v, w.TR))) return rc;
+    std::vector<float4> td, rd;
+    if ((rc = download_vec(c, td, w.t_d, (size_t)w.T * 3)) || (rc = download_vec(c, rd, w.r_d, (size_t)w.TR * 2)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<avbd_tet> tets;
+    std::vector<avbd_tri> tris;
+    std::vector<int32_t> tser, rser;
+    for (int i = 0; i < w.T; ++i) {
+        const int v[4] = {map[tv[i].x], map[tv[i].y], map[tv[i].z], map[tv[i].w]};
+        if (v[0] < 0 || v[1] < 0 || v[2] < 0 || v[3] < 0) continue;
+        avbd_tet t{};
+        for (int k = 0; k < 4; ++k) t.v[k] = v[k];
+        const float d[12] = {td[i * 3].x, td[i * 3].y, td[i * 3].z, td[i * 3].w, td[i * 3 + 1].x, td[i * 3 + 1].y,
+                             td[i * 3 + 1].z, td[i * 3 + 1].w, td[i * 3 + 2].x, td[i * 3 + 2].y, td[i * 3 + 2].z,
+                             td[i * 3 + 2].w};
+        for (int q = 0; q < 9; ++q) t.dm_inv[q] = d[q];
+        t.rest_volume = d[9]; t.mu = d[10]; t.lambda = d[11];
+        t.serial = c->tet_serial[i];
+        tets.push_back(t);
+    }
+    for (int i = 0; i < w.TR; ++i) {
+        const int v[3] = {map[rv[i].x], map[rv[i].y], map[rv[i].z]};
+        if (v[0] < 0 || v[1] < 0 || v[2] < 0) continue;
+        avbd_tri t{};
+        for (int k = 0; k < 3; ++k) t.v[k] = v[k];
+        t.dm_inv[0] = rd[i * 2].x; t.dm_inv[1] = rd[i * 2].y; t.dm_inv[2] = rd[i * 2].z; t.dm_inv[3] = rd[i * 2].w;
+        t.rest_area = rd[i * 2 + 1].x; t.mu = rd[i * 2 + 1].y; t.lambda = rd[i * 2 + 1].z;
+        t.serial = c->tri_serial[i];
+        tris.push_back(t);
+    }
+    // device renumbering
+    struct Scratch {  // freed on every return path
+        void* p[3] = {nullptr, nullptr, nullptr};
+        ~Scratch() { for (void* q : p) if (q) hipFree(q); }
+    } scratch;
+    HIP_TRY(hipMalloc(&scratch.p[0], sizeof(int) * std::max(N2, 1)));
+    HIP_TRY(hipMalloc(&scratch.p[1], sizeof(int) * std::max(N, 1)));
+    HIP_TRY(hipMalloc(&scratch.p[2], sizeof(float4) * std::max(N2, 1)));
+    int* d_perm = static_cast<int*>(scratch.p[0]);
+    int* d_map = static_cast<int*>(scratch.p[1]);
+    float4* d_tmp = static_cast<float4*>(scratch.p[2]);
+    if ((rc = upload_vec(c, d_perm, perm)) || (rc = upload_vec(c, d_map, map))) return rc;
+    if (N2 > 0) {
+        float4* f4[] = {w.pos, w.rot, w.scale, w.vel, w.angvel, w.pvel, w.pangvel, w.inertia,
+                        w.last_pos, w.last_rot, w.inert_pos, w.inert_rot, w.diff0, w.diff1};
+        for (float4* a : f4) {
+            hipLaunchKernelGGL(k_gather_f4, dim3(grid_for(N2)), dim3(256), 0, st, d_tmp, a, d_perm, N2);
+            HIP_TRY(hipMemcpyAsync(a, d_tmp, sizeof(float4) * N2, hipMemcpyDeviceToDevice, st));
+        }
+        uint32_t* u32[] = {w.flags, reinterpret_cast<uint32_t*>(w.scene)};
+        for (uint32_t* a : u32) {
+            hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(N2)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(d_tmp), a,
+                               d_perm, N2);
+            HIP_TRY(hipMemcpyAsync(a, d_tmp, sizeof(uint32_t) * N2, hipMemcpyDeviceToDevice, st));
+        }
+    }
+    if (w.J + w.SP > 0)
+        hipLaunchKernelGGL(k_remap_forces, dim3(grid_for(w.J + w.SP)), dim3(256), 0, st, w, (const int*)d_map);
+    hipLaunchKernelGGL(k_remap_manifolds, dim3(grid_for(w.Mcap)), dim3(256), 0, st, w, (const int*)d_map);
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.alive + w.Mcap, 0);
+    if ((rc = scan_ints(c, w.alive, w.alive_off, w.Mcap + 1))) return rc;
+    ManifoldBuf& dst = c->mb[c->cur ^ 1];
+    hipLaunchKernelGGL(k_compact, dim3(grid_for(w.Mcap)), dim3(256), 0, st, w, dst);
+    hipLaunchKernelGGL(k_compact_finish, dim3(1), dim3(1), 0, st, w);
+    c->cur ^= 1;
+    w.m = c->mb[c->cur];
+    HIP_TRY(hipStreamSynchronize(st));
+    // sizes, offsets, energies, tiles, incidence
+    w.N = N2;
+    c->sc_off = noff;
+    if ((rc = upload_vec(c, const_cast<int*>(w.sc_off), noff))) return rc;
+    if ((rc = upload_energies(c, N2, tets.data(), (int)tets.size(), tris.data(), (int)tris.size()))) return rc;
+    if ((rc = upload_tiles(c, noff))) return rc;
+    if ((rc = upload_vec(c, w.st, stw))) return rc;
+    if ((rc = build_incidence(c))) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    return poststab_scenes(c, aff);
 }
 
 int do_step(avbd_ctx* c) {
@@ -698,18 +927,9 @@ int do_step(avbd_ctx* c) {
     c->cur ^= 1;
     w.m = c->mb[c->cur];
     if ((rc = build_incidence(c))) return rc;
-    hipLaunchKernelGGL(k_colour, dim3(w.S), dim3(1024), 0, st, w);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, w.col_cnt, AVBD_BUCKETS + 1, 0);
-    hipLaunchKernelGGL(k_colour_hist, dim3(grid_for(N)), dim3(256), 0, st, w);
-    hipLaunchKernelGGL(k_colour_scan, dim3(1), dim3(256), 0, st, w);
-    hipLaunchKernelGGL(k_colour_fill, dim3(grid_for(N)), dim3(256), 0, st, w);
     int g[G_WORDS];
+    if ((rc = colour_and_buckets(c, g))) return rc;
     int* off = c->bucket_off.data();
-    HIP_TRY(hipMemcpyAsync(g, w.g, sizeof(g), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(off, w.col_off, (AVBD_BUCKETS + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    HIP_TRY(hipGetLastError());
-    if ((rc = check_errors(c, g))) return rc;
     const int ncol = g[G_NCOLOURS];
     c->last_ncol = ncol;
     HIP_TRY(hipEventRecord(c->ev[2], st));
@@ -933,11 +1153,7 @@ static int step_impl(avbd_ctx* c, int32_t n_steps) {
         HIP_TRY(hipStreamSynchronize(c->stream));
         c->collect_kernel_times();
         c->tuner.collect();
-        if (g[G_ANY_OOB]) {
-            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, c->stream, c->w.g + G_ANY_OOB, 0);
-            hipStreamSynchronize(c->stream);
-            return fail(AVBD_E_UNSUPPORTED, "a body left the world bounds; out-of-bounds removal is not supported on the device yet");
-        }
+        if (g[G_ANY_OOB] && (rc = remove_out_of_bounds(c))) return rc;
         acc.broad_phase_ms = elapsed(c->ev[0], c->ev[1]);
         acc.warmstart_ms = elapsed(c->ev[1], c->ev[2]);
         acc.prediction_ms = elapsed(c->ev[2], c->ev[3]);
@@ -1121,14 +1337,14 @@ int avbd_download(avbd_ctx* c, avbd_world* out) {
                              td[i * 3 + 2].w};
         for (int q = 0; q < 9; ++q) t.dm_inv[q] = d[q];
         t.rest_volume = d[9]; t.mu = d[10]; t.lambda = d[11];
-        t.serial = i;
+        t.serial = c->tet_serial[i];
     }
     for (int i = 0; i < w.TR; ++i) {
         avbd_tri& t = out->tris[i];
         t.v[0] = rv[i].x; t.v[1] = rv[i].y; t.v[2] = rv[i].z;
         t.dm_inv[0] = rd[i * 2].x; t.dm_inv[1] = rd[i * 2].y; t.dm_inv[2] = rd[i * 2].z; t.dm_inv[3] = rd[i * 2].w;
         t.rest_area = rd[i * 2 + 1].x; t.mu = rd[i * 2 + 1].y; t.lambda = rd[i * 2 + 1].z;
-        t.serial = w.T + i;
+        t.serial = c->tri_serial[i];
     }
     out->n_scenes = cnt.n_scenes;
     out->n_bodies = cnt.n_bodies;

@@ -1113,7 +1113,7 @@ AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int p
         const int s = w.scene[b];
         const avbd_scene& sc = w.sc[s];
         if (w.st[s * ST_WORDS + ST_EMERGENCY]) continue;
-        if (poststab ? !sc.post_stabilization : iter >= sc.iterations) continue;
+        if (poststab ? (!sc.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) : iter >= sc.iterations) continue;
         const float dt2 = sc.dt * sc.dt;
         float L[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         float r[3] = {0.f, 0.f, 0.f};
@@ -1194,7 +1194,8 @@ __global__ __launch_bounds__(256) void k_primal(DevWorld w, int lo, int hi, int 
             s = w.scene[b];
             const avbd_scene& sc0 = w.sc[s];
             if (w.st[s * ST_WORDS + ST_EMERGENCY]) run = false;
-            if (poststab) { if (!sc0.post_stabilization) run = false; }
+            // a scene with bodies out of bounds post-stabilises after their removal
+            if (poststab) { if (!sc0.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) run = false; }
             else if (iter >= sc0.iterations) run = false;
         }
         // all lanes of a group agree on `run`; groups never straddle a wave
@@ -1448,7 +1449,7 @@ __global__ void k_stick(DevWorld w) {
         const int a = w.m.a[m], b = w.m.b[m];
         const int s = w.scene[a];
         if (!scene_runs(w, s)) continue;
-        if (!w.sc[s].post_stabilization) continue;
+        if (!w.sc[s].post_stabilization || w.st[s * ST_WORDS + ST_OOB]) continue;
         if (is_static(w, a) && is_static(w, b)) continue;
         const float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
         const float4 h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4], h5 = h[5];
@@ -1456,5 +1457,56 @@ __global__ void k_stick(DevWorld w) {
         float tl = sqrtf(h4.w * h4.w + h5.x * h5.x);
         float tc = sqrtf(h1.w * h1.w + h2.w * h2.w);
         w.m.cold[((size_t)m * 8 + c) * CONTACT_COLD_F4].y = (float)((tl < bound) && (tc < 0.01f));
+    }
+}
+
+// ------------------------------------------------ out-of-bounds removal
+// Solver::RemoveBody for the bodies that left the world bounds
+// (solver.cpp:520-534, 104-116): the host computes the surviving order
+// (perm: new index -> old, map: old -> new or -1) and these kernels move the
+// device state to it.
+__global__ void k_gather_f4(float4* dst, const float4* src, const int* perm, int n) {
+    GRID_STRIDE(i, n) dst[i] = src[perm[i]];
+}
+__global__ void k_gather_u32(uint32_t* dst, const uint32_t* src, const int* perm, int n) {
+    GRID_STRIDE(i, n) dst[i] = src[perm[i]];
+}
+
+// Joints and springs: endpoints renumbered; a force that loses a body is
+// removed with it (~Mesh deletes the body's forces, geometry.cpp:11-24).
+__global__ void k_remap_forces(DevWorld w, const int* map) {
+    GRID_STRIDE(t, w.J + w.SP) {
+        int *pa, *pb;
+        uint32_t* fl;
+        if (t < w.J) { pa = &w.j_a[t]; pb = &w.j_b[t]; fl = &w.j_flags[t]; }
+        else { pa = &w.s_a[t - w.J]; pb = &w.s_b[t - w.J]; fl = &w.s_flags[t - w.J]; }
+        if (*fl & FORCE_DEAD) continue;
+        const int a = *pa, b = *pb;
+        const int na = a >= 0 ? map[a] : -1, nb = map[b];
+        if ((a >= 0 && na < 0) || nb < 0) {
+            *fl |= FORCE_DEAD;
+            continue;
+        }
+        *pa = na;
+        *pb = nb;
+    }
+}
+
+// Manifolds: endpoints renumbered (the bodies keep their roles A and B);
+// those that lose a body are dropped by the compaction that follows.
+__global__ void k_remap_manifolds(DevWorld w, const int* map) {
+    const int M = w.g[G_MANIFOLDS];
+    GRID_STRIDE(m, w.Mcap) {
+        if (m >= M) {
+            w.alive[m] = 0;
+            continue;
+        }
+        const int na = map[w.m.a[m]], nb = map[w.m.b[m]];
+        const bool keep = na >= 0 && nb >= 0;
+        w.alive[m] = keep ? 1 : 0;
+        if (keep) {
+            w.m.a[m] = na;
+            w.m.b[m] = nb;
+        }
     }
 }

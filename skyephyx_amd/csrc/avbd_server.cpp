@@ -146,17 +146,20 @@ struct avbd_server {
         for (int i = 0; i < w.n_bodies; ++i) off[bodies[i].scene + 1]++;
         for (int s = 0; s < S; ++s) off[s + 1] += off[s];
         for (int s = 0; s < S; ++s) {
+            // (bodies that left the world bounds were removed on the device)
             SceneHost& h = scenes[resident[s]];
-            if (off[s + 1] - off[s] != (int)h.bodies.size())
-                return avbd_set_error(AVBD_E_STATE, "device body count of a scene changed");
             h.sc = sc[s];
-            for (int i = off[s]; i < off[s + 1]; ++i) {
-                h.bodies[i - off[s]] = bodies[i];
-                h.bodies[i - off[s]].scene = 0;
+            h.bodies.assign(bodies.begin() + off[s], bodies.begin() + off[s + 1]);
+            h.nonpart = 0;
+            for (avbd_body& b : h.bodies) {
+                b.scene = 0;
+                if (!(b.flags & AVBD_BODY_PARTICLE)) ++h.nonpart;
             }
             h.joints.clear();
             h.springs.clear();
             h.manifolds.clear();
+            h.tets.clear();
+            h.tris.clear();
         }
         auto local = [&](int b, int s) { return b < 0 ? b : b - off[s]; };
         for (int i = 0; i < w.n_joints; ++i) {
@@ -180,7 +183,22 @@ struct avbd_server {
             m.body_b = local(m.body_b, s);
             scenes[resident[s]].manifolds.push_back(m);
         }
-        for (int id : resident) renumber_forces(scenes[id]);
+        for (int i = 0; i < w.n_tets; ++i) {
+            avbd_tet t = tets[i];
+            const int s = bodies[t.v[0]].scene;
+            for (int k = 0; k < 4; ++k) t.v[k] = local(t.v[k], s);
+            scenes[resident[s]].tets.push_back(t);
+        }
+        for (int i = 0; i < w.n_tris; ++i) {
+            avbd_tri t = tris[i];
+            const int s = bodies[t.v[0]].scene;
+            for (int k = 0; k < 3; ++k) t.v[k] = local(t.v[k], s);
+            scenes[resident[s]].tris.push_back(t);
+        }
+        for (int id : resident) {
+            renumber_forces(scenes[id]);
+            renumber_energies(scenes[id]);
+        }
         return 0;
     }
 

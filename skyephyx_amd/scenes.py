@@ -246,6 +246,28 @@ def mass_springs(n=4, **scene):
     return b.build()
 
 
+def out_of_bounds(**scene):
+    """Bodies leaving the world bounds (x in [-6, 6]) in the first step:
+    the bounds removal (solver.cpp:520-534) takes a fast box with the spring
+    that ties it to a resting box, and the fast half of a small Neo-Hookean
+    block with the tets that touch it; the scene's last body, a box resting
+    on another, moves into a freed index (swap with last)."""
+    params = dict(bounds_min=(-6.0, -20.0, -6.0), bounds_max=(6.0, 20.0, 6.0))
+    params.update(scene)
+    b = WorldBuilder(**params)
+    add_ground(b)
+    rest = b.add_body(MODEL_CUBE, 1.0, 0.5, (0.0, -9.0, 0.0), (1.0, 1.0, 1.0))
+    fast = b.add_body(MODEL_CUBE, 1.0, 0.5, (5.5, -5.0, 0.0), (1.0, 1.0, 1.0), velocity=(60.0, 0.0, 0.0))
+    anchor = b.add_body(MODEL_CUBE, 1.0, 0.5, (-2.0, -9.0, 0.0), (1.0, 1.0, 1.0))
+    b.add_spring(anchor, (0.0, 0.0, 0.0), fast, (0.0, 0.0, 0.0), 100.0)
+    idx = tet_volume(b, (5.0, -6.0, -0.5), (3, 2, 2), 0.5, 1e4, 0.3, 0.1)
+    for p in idx[2].ravel():  # the x = 6.0 column leaves
+        b._bodies[int(p)]["vel"] = (60.0, 0.0, 0.0)
+    b.add_body(MODEL_CUBE, 1.0, 0.5, (0.0, -7.98, 0.0), (1.0, 1.0, 1.0))
+    del rest
+    return b.build()
+
+
 def batched_mix(n_scenes=4096, seed=7, box_range=(8, 2048), tet_range=(1000, 50000), fem_fraction=0.5):
     """Independent scenes of uneven size: rigid box lattices/pyramids with box
     counts log-uniform in box_range, and Neo-Hookean grids with tet counts
