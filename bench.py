@@ -12,7 +12,7 @@ value = whole-job scene-steps/s.  The line also carries the roofline of the
 dominant kernel (the primal block-solve sweep, timed per launch with HIP
 events on the library's stream), the served form over the same scenes
 (request server with every scene resident under a 200 GB HBM budget,
-Poisson arrivals at --serve-load fractions of the batched rate: request
+Poisson arrivals at --serve-load fractions of its own closed-loop saturation: request
 latency p50/p99 and tick p99) and a CPU baseline: the FP32 oracle (a
 restatement of the reference's single-threaded Solver::Step) timed on one
 host core over a bounded, size-stratified sample of the same scenes.
@@ -137,13 +137,14 @@ def plain_gpu_rate(world, steps=3, lanes=8, fem_lanes=16):
     return 1.0 / (t_w / total_w)
 
 
-def served(world, saturation, fractions, duration, budget=200e9):
+def served(world, fractions, duration, budget=200e9):
     """Served form over the same scenes: every scene registered and resident
-    under an HBM budget, Poisson request arrivals at fractions of the batched
-    saturation rate, each request U{1..8} steps of a random scene.  The
+    under an HBM budget, requests of U{1..8} steps of a random scene.  The
+    served saturation is measured first (closed loop: every scene always has
+    a request in flight), then Poisson arrivals at fractions of it.  The
     server keeps its default pinned lane widths (a scene's result does not
     depend on the other scenes in its ticks)."""
-    from skyephyx_amd.serve import Server, poisson_run
+    from skyephyx_amd.serve import Server, closed_loop, poisson_run
 
     mean_k = 4.5
     out = []
@@ -154,17 +155,18 @@ def served(world, saturation, fractions, duration, budget=200e9):
         st = srv.stats()
         log(f"[serve] {len(ids)} scenes registered and resident ({st['resident_bytes'] / 1e9:.2f} GB of "
             f"{budget / 1e9:.0f} GB budget) in {time.perf_counter() - t0:.1f}s")
-        # untimed warm-up
-        warm = poisson_run(srv, ids, 0.5 * saturation / mean_k, 2.0, steps=(1, 8), seed=99)
-        log(f"[serve] warm-up: {warm['requests']} requests")
+        closed_loop(srv, ids, 1.0, seed=98)  # untimed warm-up
+        sat = closed_loop(srv, ids, 4.0, seed=99)
+        log(f"[serve] saturation (closed loop): {json.dumps(sat)}")
         for f in fractions:
-            rate = f * saturation / mean_k
+            rate = f * sat["scene_steps_per_s"] / mean_k
             r = poisson_run(srv, ids, rate, duration, steps=(1, 8), seed=int(f * 1000), log=log)
             r["load_fraction"] = f
             log(f"[serve] load {f:.2f}: {json.dumps(r)}")
             out.append(r)
+        timings = srv.context.timings()
         st = srv.stats()
-    return out, st
+    return out, st, sat, timings
 
 
 def main():
@@ -179,7 +181,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson request) measurement")
     ap.add_argument("--serve-load", type=float, nargs="+", default=[0.8],
-                    help="served load points as fractions of the batched saturation rate")
+                    help="served load points as fractions of the served (closed-loop) saturation")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per served load point")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -290,15 +292,16 @@ def main():
     res["device_bytes"] = ctx_bytes
     if not args.no_serve and S > 1:
         log("[bench] served form")
-        points, sst = served(state, scene_steps, args.serve_load, args.serve_seconds)
+        points, sst, sat, stim = served(state, args.serve_load, args.serve_seconds)
         res["served"] = {
             "hbm_budget_bytes": int(200e9),
             "resident_bytes": int(sst["resident_bytes"]),
             "resident_scenes": int(sst["resident_scenes"]),
             "lanes": "pinned 4/4 (result independent of the mix)",
             "request": "U{1..8} steps of a uniformly drawn scene, Poisson arrivals",
-            "saturation_scene_steps_per_s": round(scene_steps, 3),
+            "saturation": sat,
             "points": points,
+            "phase_ms_last_tick": {k: round(v, 4) for k, v in stim.items() if k.endswith("_ms")},
         }
         main_pt = points[0]
         res["served_p50_latency_ms"] = main_pt["latency_ms_p50"]

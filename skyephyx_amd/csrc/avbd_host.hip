@@ -84,6 +84,10 @@ struct avbd_ctx {
     std::vector<avbd_scene> scenes;
     std::vector<int> sc_off;
     std::vector<int32_t> tet_serial, tri_serial;  // creation serials of the device's elements
+    // Removed bodies stay on the device as dead slots at the end of their
+    // scene's index range (static, no broad-phase sphere, no energies);
+    // removed elements keep their index.  Downloads skip both.
+    std::vector<char> body_dead, tet_dead, tri_dead;
     std::vector<int32_t> host_colours;
     hipEvent_t ev[8]{};
     avbd_timings timings{};
@@ -436,6 +440,8 @@ int upload_energies(avbd_ctx* c, int N, const avbd_tet* tets, int T, const avbd_
     w.TR = TR;
     c->tet_serial.resize(T);
     c->tri_serial.resize(TR);
+    c->tet_dead.assign(T, 0);
+    c->tri_dead.assign(TR, 0);
     for (int i = 0; i < T; ++i) c->tet_serial[i] = tets[i].serial;
     for (int i = 0; i < TR; ++i) c->tri_serial[i] = tris[i].serial;
     return 0;
@@ -528,6 +534,7 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     // ---- host staging (records -> device SoA) ----
     c->scenes.assign(in->scenes, in->scenes + S);
     c->sc_off = off;
+    c->body_dead.assign(N, 0);
     std::vector<int> st((size_t)S * ST_WORDS, 0);
     for (int s = 0; s < S; ++s) {
         st[s * ST_WORDS + ST_EMERGENCY] = in->scenes[s].emergency_stop;
@@ -762,105 +769,164 @@ int remove_out_of_bounds(avbd_ctx* c) {
         HIP_TRY(hipStreamSynchronize(st));
         return 0;
     }
-    // surviving order per scene
-    std::vector<int> perm, map(N, -1), noff(S + 1, 0);
-    perm.reserve(N);
+    // New order inside each affected scene: its live bodies (earlier dead
+    // slots are at the end of its index range), the removals applied one by
+    // one as swap with last, then the removed bodies and the earlier dead
+    // slots as the scene's trailing dead slots.  Other scenes keep every
+    // index.  perm: new -> old; map: old -> new, -1 for the removed.
+    std::vector<int> perm(N), map(N);
+    for (int i = 0; i < N; ++i) perm[i] = map[i] = i;
+    std::vector<int> dead_new;  // new dead slots
+    struct Seg { int a, e, n_live; };
+    std::vector<Seg> segs;
     for (int s = 0; s < S; ++s) {
+        if (!aff[s]) continue;
         const int a = c->sc_off[s], e = c->sc_off[s + 1];
-        std::vector<int> order(e - a), where(e - a);
-        for (int i = a; i < e; ++i) {
-            order[i - a] = i;
-            where[i - a] = i - a;
-        }
-        if (aff[s]) {
-            const avbd_scene& sc = c->scenes[s];
-            int removed = 0;
-            for (int i = a; i < e; ++i) {
-                if (flags[i] & AVBD_BODY_STATIC) continue;
-                const float4 p = pos[i];
-                if (p.x < sc.bounds_min[0] || p.x > sc.bounds_max[0] || p.y < sc.bounds_min[1] ||
-                    p.y > sc.bounds_max[1] || p.z < sc.bounds_min[2] || p.z > sc.bounds_max[2]) {
-                    const int k = where[i - a], last = (int)order.size() - 1;
-                    if (k != last) {
-                        order[k] = order[last];
-                        where[order[k] - a] = k;
-                    }
-                    order.pop_back();
-                    where[i - a] = -1;
-                    ++removed;
-                }
+        std::vector<int> order, where(e - a, -1), gone;
+        for (int i = a; i < e; ++i)
+            if (!c->body_dead[i]) {
+                where[i - a] = (int)order.size();
+                order.push_back(i);
             }
-            stw[s * ST_WORDS + ST_REMOVED] += removed;
-            stw[s * ST_WORDS + ST_DIRTY] = 1;
+        const avbd_scene& sc = c->scenes[s];
+        for (int i = a; i < e; ++i) {
+            if (c->body_dead[i] || (flags[i] & AVBD_BODY_STATIC)) continue;
+            const float4 p = pos[i];
+            if (p.x < sc.bounds_min[0] || p.x > sc.bounds_max[0] || p.y < sc.bounds_min[1] || p.y > sc.bounds_max[1] ||
+                p.z < sc.bounds_min[2] || p.z > sc.bounds_max[2]) {
+                const int k = where[i - a], last = (int)order.size() - 1;
+                if (k != last) {
+                    order[k] = order[last];
+                    where[order[k] - a] = k;
+                }
+                order.pop_back();
+                where[i - a] = -1;
+                gone.push_back(i);
+            }
         }
+        stw[s * ST_WORDS + ST_REMOVED] += (int)gone.size();
+        stw[s * ST_WORDS + ST_DIRTY] = 1;
+        segs.push_back({a, e, (int)order.size()});
+        int k = a;
         for (int o : order) {
-            map[o] = (int)perm.size();
-            perm.push_back(o);
+            perm[k] = o;
+            map[o] = k++;
         }
-        noff[s + 1] = (int)perm.size();
+        for (int o : gone) {
+            perm[k] = o;
+            map[o] = -1;
+            dead_new.push_back(k++);
+        }
+        for (int i = a; i < e; ++i)
+            if (c->body_dead[i]) {
+                perm[k] = i;
+                map[i] = -1;
+                ++k;
+            }
     }
-    const int N2 = (int)perm.size();
-    // energies that keep all their vertices, renumbered, in creation order
-    std::vector<int4> tv, rv;
-    if ((rc = download_vec(c, tv, w.t_v, w.T)) || (rc = download_vec(c, rv, w.r_v, w.TR))) return rc;
-    std::vector<float4> td, rd;
-    if ((rc = download_vec(c, td, w.t_d, (size_t)w.T * 3)) || (rc = download_vec(c, rd, w.r_d, (size_t)w.TR * 2)))
-        return rc;
+    // energy lists of the affected scenes: each is the contiguous range
+    // [e_off[a], e_off[e]) of the incidence arrays (an element's vertices
+    // share its scene).  Rebuilt on the host in the new vertex order without
+    // the elements that lost a vertex; the trailing dead slots get empty
+    // lists (the last one keeps the rest of the range, never read).
+    struct SegData {
+        std::vector<int> off;
+        std::vector<uint32_t> ent;
+        std::vector<float4> ea, eb, ec;
+        std::vector<int4> ev;
+    };
+    std::vector<SegData> sd(segs.size());
+    for (size_t q = 0; q < segs.size(); ++q) {
+        const Seg& g = segs[q];
+        if ((rc = download_vec(c, sd[q].off, w.e_off + g.a, (size_t)(g.e - g.a + 1)))) return rc;
+    }
     HIP_TRY(hipStreamSynchronize(st));
-    std::vector<avbd_tet> tets;
-    std::vector<avbd_tri> tris;
-    std::vector<int32_t> tser, rser;
-    for (int i = 0; i < w.T; ++i) {
-        const int v[4] = {map[tv[i].x], map[tv[i].y], map[tv[i].z], map[tv[i].w]};
-        if (v[0] < 0 || v[1] < 0 || v[2] < 0 || v[3] < 0) continue;
-        avbd_tet t{};
-        for (int k = 0; k < 4; ++k) t.v[k] = v[k];
-        const float d[12] = {td[i * 3].x, td[i * 3].y, td[i * 3].z, td[i * 3].w, td[i * 3 + 1].x, td[i * 3 + 1].y,
-                             td[i * 3 + 1].z, td[i * 3 + 1].w, td[i * 3 + 2].x, td[i * 3 + 2].y, td[i * 3 + 2].z,
-                             td[i * 3 + 2].w};
-        for (int q = 0; q < 9; ++q) t.dm_inv[q] = d[q];
-        t.rest_volume = d[9]; t.mu = d[10]; t.lambda = d[11];
-        t.serial = c->tet_serial[i];
-        tets.push_back(t);
+    for (size_t q = 0; q < segs.size(); ++q) {
+        const Seg& g = segs[q];
+        SegData& d = sd[q];
+        const int E0 = d.off[0], E1 = d.off[g.e - g.a];
+        if ((rc = download_vec(c, d.ent, w.e_ent + E0, (size_t)(E1 - E0))) ||
+            (rc = download_vec(c, d.ea, w.ev_a + E0, (size_t)(E1 - E0))) ||
+            (rc = download_vec(c, d.eb, w.ev_b + E0, (size_t)(E1 - E0))) ||
+            (rc = download_vec(c, d.ec, w.ev_c + E0, (size_t)(E1 - E0))) ||
+            (rc = download_vec(c, d.ev, w.ev_v + E0, (size_t)(E1 - E0))))
+            return rc;
     }
-    for (int i = 0; i < w.TR; ++i) {
-        const int v[3] = {map[rv[i].x], map[rv[i].y], map[rv[i].z]};
-        if (v[0] < 0 || v[1] < 0 || v[2] < 0) continue;
-        avbd_tri t{};
-        for (int k = 0; k < 3; ++k) t.v[k] = v[k];
-        t.dm_inv[0] = rd[i * 2].x; t.dm_inv[1] = rd[i * 2].y; t.dm_inv[2] = rd[i * 2].z; t.dm_inv[3] = rd[i * 2].w;
-        t.rest_area = rd[i * 2 + 1].x; t.mu = rd[i * 2 + 1].y; t.lambda = rd[i * 2 + 1].z;
-        t.serial = c->tri_serial[i];
-        tris.push_back(t);
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t q = 0; q < segs.size(); ++q) {
+        const Seg& g = segs[q];
+        SegData& d = sd[q];
+        const int E0 = d.off[0], E1 = d.off[g.e - g.a];
+        std::vector<int> noff(g.e - g.a, 0);
+        std::vector<uint32_t> nent;
+        std::vector<float4> na, nb, nc;
+        std::vector<int4> nv;
+        for (int k = 0; k < g.e - g.a; ++k) {
+            noff[k] = E0 + (int)nent.size();
+            if (k >= g.n_live) continue;
+            const int o = perm[g.a + k] - g.a;  // old local index
+            for (int x = d.off[o] - E0; x < d.off[o + 1] - E0; ++x) {
+                const uint32_t ent = d.ent[x];
+                const int4 v = d.ev[x];
+                const bool tri = (ent >> 31) != 0;
+                if (map[v.x] < 0 || map[v.y] < 0 || map[v.z] < 0 || (!tri && map[v.w] < 0)) {
+                    const uint32_t idx = ent & 0x1FFFFFFFu;
+                    if (tri) c->tri_dead[idx] = 1;
+                    else c->tet_dead[idx] = 1;
+                    continue;
+                }
+                nent.push_back(ent);
+                na.push_back(d.ea[x]);
+                nb.push_back(d.eb[x]);
+                nc.push_back(d.ec[x]);
+                nv.push_back(make_int4(map[v.x], map[v.y], map[v.z], tri ? 0 : map[v.w]));
+            }
+        }
+        if (E0 + (int)nent.size() > E1) return fail(AVBD_E_STATE, "energy list grew during removal");
+        if ((rc = upload_vec(c, const_cast<int*>(w.e_off) + g.a, noff)) ||
+            (rc = upload_vec(c, const_cast<uint32_t*>(w.e_ent) + E0, nent)) || (rc = upload_vec(c, w.ev_a + E0, na)) ||
+            (rc = upload_vec(c, w.ev_b + E0, nb)) || (rc = upload_vec(c, w.ev_c + E0, nc)) ||
+            (rc = upload_vec(c, w.ev_v + E0, nv)))
+            return rc;
+        HIP_TRY(hipStreamSynchronize(st));
     }
-    // device renumbering
+    // device renumbering of the bodies, forces and element vertex ids
     struct Scratch {  // freed on every return path
         void* p[3] = {nullptr, nullptr, nullptr};
         ~Scratch() { for (void* q : p) if (q) hipFree(q); }
     } scratch;
-    HIP_TRY(hipMalloc(&scratch.p[0], sizeof(int) * std::max(N2, 1)));
+    HIP_TRY(hipMalloc(&scratch.p[0], sizeof(int) * std::max(N, 1)));
     HIP_TRY(hipMalloc(&scratch.p[1], sizeof(int) * std::max(N, 1)));
-    HIP_TRY(hipMalloc(&scratch.p[2], sizeof(float4) * std::max(N2, 1)));
+    HIP_TRY(hipMalloc(&scratch.p[2], sizeof(float4) * std::max(N, 1)));
     int* d_perm = static_cast<int*>(scratch.p[0]);
     int* d_map = static_cast<int*>(scratch.p[1]);
     float4* d_tmp = static_cast<float4*>(scratch.p[2]);
     if ((rc = upload_vec(c, d_perm, perm)) || (rc = upload_vec(c, d_map, map))) return rc;
-    if (N2 > 0) {
+    {
         float4* f4[] = {w.pos, w.rot, w.scale, w.vel, w.angvel, w.pvel, w.pangvel, w.inertia,
                         w.last_pos, w.last_rot, w.inert_pos, w.inert_rot, w.diff0, w.diff1};
         for (float4* a : f4) {
-            hipLaunchKernelGGL(k_gather_f4, dim3(grid_for(N2)), dim3(256), 0, st, d_tmp, a, d_perm, N2);
-            HIP_TRY(hipMemcpyAsync(a, d_tmp, sizeof(float4) * N2, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_gather_f4, dim3(grid_for(N)), dim3(256), 0, st, d_tmp, a, d_perm, N);
+            HIP_TRY(hipMemcpyAsync(a, d_tmp, sizeof(float4) * N, hipMemcpyDeviceToDevice, st));
         }
         uint32_t* u32[] = {w.flags, reinterpret_cast<uint32_t*>(w.scene)};
         for (uint32_t* a : u32) {
-            hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(N2)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(d_tmp), a,
-                               d_perm, N2);
-            HIP_TRY(hipMemcpyAsync(a, d_tmp, sizeof(uint32_t) * N2, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(N)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(d_tmp), a,
+                               d_perm, N);
+            HIP_TRY(hipMemcpyAsync(a, d_tmp, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
         }
+    }
+    if (!dead_new.empty()) {
+        // the new dead slots: static, no broad-phase sphere, at rest (d_perm
+        // is free again once the gathers above have run: same stream)
+        if ((rc = upload_vec(c, d_perm, dead_new))) return rc;
+        hipLaunchKernelGGL(k_kill_bodies, dim3(grid_for((int)dead_new.size())), dim3(256), 0, st, w, (const int*)d_perm,
+                           (int)dead_new.size());
     }
     if (w.J + w.SP > 0)
         hipLaunchKernelGGL(k_remap_forces, dim3(grid_for(w.J + w.SP)), dim3(256), 0, st, w, (const int*)d_map);
+    if (w.T + w.TR > 0)
+        hipLaunchKernelGGL(k_remap_elements, dim3(grid_for(w.T + w.TR)), dim3(256), 0, st, w, (const int*)d_map);
     hipLaunchKernelGGL(k_remap_manifolds, dim3(grid_for(w.Mcap)), dim3(256), 0, st, w, (const int*)d_map);
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.alive + w.Mcap, 0);
     if ((rc = scan_ints(c, w.alive, w.alive_off, w.Mcap + 1))) return rc;
@@ -869,13 +935,8 @@ int remove_out_of_bounds(avbd_ctx* c) {
     hipLaunchKernelGGL(k_compact_finish, dim3(1), dim3(1), 0, st, w);
     c->cur ^= 1;
     w.m = c->mb[c->cur];
-    HIP_TRY(hipStreamSynchronize(st));
-    // sizes, offsets, energies, tiles, incidence
-    w.N = N2;
-    c->sc_off = noff;
-    if ((rc = upload_vec(c, const_cast<int*>(w.sc_off), noff))) return rc;
-    if ((rc = upload_energies(c, N2, tets.data(), (int)tets.size(), tris.data(), (int)tris.size()))) return rc;
-    if ((rc = upload_tiles(c, noff))) return rc;
+    for (const Seg& g : segs)
+        for (int i = g.a; i < g.e; ++i) c->body_dead[i] = (i - g.a) >= g.n_live;
     if ((rc = upload_vec(c, w.st, stw))) return rc;
     if ((rc = build_incidence(c))) return rc;
     HIP_TRY(hipStreamSynchronize(st));
@@ -963,6 +1024,16 @@ int do_step(avbd_ctx* c) {
 }
 
 }  // namespace
+
+// Device index -> index among the live bodies (dead slots skipped), -1 for
+// a dead slot; negative ids (the world anchor) pass through.
+static std::vector<int> live_index(const avbd_ctx* c) {
+    std::vector<int> idx(c->w.N, -1);
+    int k = 0;
+    for (int i = 0; i < c->w.N; ++i)
+        if (!c->body_dead[i]) idx[i] = k++;
+    return idx;
+}
 
 int avbd_set_error(int code, const std::string& msg) { return fail(code, msg); }
 
@@ -1210,12 +1281,12 @@ int avbd_get_counts(avbd_ctx* c, avbd_world* out) {
     if ((rc = download_vec(c, jf, w.j_flags, w.J)) || (rc = download_vec(c, sf, w.s_flags, w.SP))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     out->n_scenes = w.S;
-    out->n_bodies = w.N;
+    out->n_bodies = w.N - (int)std::count(c->body_dead.begin(), c->body_dead.end(), 1);
     out->n_joints = (int)std::count_if(jf.begin(), jf.end(), [](uint32_t f) { return !(f & FORCE_DEAD); });
     out->n_springs = (int)std::count_if(sf.begin(), sf.end(), [](uint32_t f) { return !(f & FORCE_DEAD); });
     out->n_manifolds = g[G_MANIFOLDS];
-    out->n_tets = w.T;
-    out->n_tris = w.TR;
+    out->n_tets = w.T - (int)std::count(c->tet_dead.begin(), c->tet_dead.end(), 1);
+    out->n_tris = w.TR - (int)std::count(c->tri_dead.begin(), c->tri_dead.end(), 1);
     return 0;
 }
 
@@ -1268,8 +1339,11 @@ int avbd_download(avbd_ctx* c, avbd_world* out) {
         out->scenes[s].n_removed = st[s * ST_WORDS + ST_REMOVED];
     }
     auto put3 = [](float* d, float4 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; };
+    const std::vector<int> live = live_index(c);
+    auto L = [&](int b) { return b < 0 ? b : live[b]; };
     for (int i = 0; i < N; ++i) {
-        avbd_body& b = out->bodies[i];
+        if (live[i] < 0) continue;
+        avbd_body& b = out->bodies[live[i]];
         std::memset(&b, 0, sizeof(b));
         put3(b.pos, pos[i]);
         b.rot[0] = rot[i].x; b.rot[1] = rot[i].y; b.rot[2] = rot[i].z; b.rot[3] = rot[i].w;
@@ -1290,7 +1364,7 @@ int avbd_download(avbd_ctx* c, avbd_world* out) {
         if (jf[i] & FORCE_DEAD) continue;
         avbd_joint& j = out->joints[k++];
         std::memset(&j, 0, sizeof(j));
-        j.body_a = ja[i]; j.body_b = jb[i]; j.serial = jser[i]; j.flags = jf[i] & AVBD_FORCE_DISABLED;
+        j.body_a = L(ja[i]); j.body_b = L(jb[i]); j.serial = jser[i]; j.flags = jf[i] & AVBD_FORCE_DISABLED;
         put3(j.r_a, jra[i]); j.torque_arm = jra[i].w;
         put3(j.r_b, jrb[i]);
         j.rest_rot[0] = jrest[i].x; j.rest_rot[1] = jrest[i].y; j.rest_rot[2] = jrest[i].z; j.rest_rot[3] = jrest[i].w;
@@ -1305,7 +1379,7 @@ int avbd_download(avbd_ctx* c, avbd_world* out) {
         if (sf[i] & FORCE_DEAD) continue;
         avbd_spring& s = out->springs[k++];
         std::memset(&s, 0, sizeof(s));
-        s.body_a = sa[i]; s.body_b = sb[i]; s.serial = sser[i]; s.flags = sf[i] & AVBD_FORCE_DISABLED;
+        s.body_a = L(sa[i]); s.body_b = L(sb[i]); s.serial = sser[i]; s.flags = sf[i] & AVBD_FORCE_DISABLED;
         put3(s.r_a, sra[i]); s.rest_length = sra[i].w;
         put3(s.r_b, srb[i]);
         const float* row = &srow[(size_t)i * 6];
@@ -1314,7 +1388,7 @@ int avbd_download(avbd_ctx* c, avbd_world* out) {
     for (int i = 0; i < M; ++i) {
         avbd_manifold& m = out->manifolds[i];
         std::memset(&m, 0, sizeof(m));
-        m.body_a = ma[i]; m.body_b = mbv[i]; m.serial = mser[i]; m.n_contacts = mn[i]; m.friction = mfr[i];
+        m.body_a = L(ma[i]); m.body_b = L(mbv[i]); m.serial = mser[i]; m.n_contacts = mn[i]; m.friction = mfr[i];
         for (int ci = 0; ci < mn[i]; ++ci) {
             const float4* h = &hot[((size_t)i * 8 + ci) * CONTACT_HOT_F4];
             const float4* cd = &cold[((size_t)i * 8 + ci) * CONTACT_COLD_F4];
@@ -1329,9 +1403,11 @@ int avbd_download(avbd_ctx* c, avbd_world* out) {
             ct.penalty[0] = h[5].y; ct.penalty[1] = h[5].z; ct.penalty[2] = h[5].w;
         }
     }
+    k = 0;
     for (int i = 0; i < w.T; ++i) {
-        avbd_tet& t = out->tets[i];
-        t.v[0] = tv[i].x; t.v[1] = tv[i].y; t.v[2] = tv[i].z; t.v[3] = tv[i].w;
+        if (c->tet_dead[i]) continue;
+        avbd_tet& t = out->tets[k++];
+        t.v[0] = L(tv[i].x); t.v[1] = L(tv[i].y); t.v[2] = L(tv[i].z); t.v[3] = L(tv[i].w);
         const float d[12] = {td[i * 3].x, td[i * 3].y, td[i * 3].z, td[i * 3].w, td[i * 3 + 1].x, td[i * 3 + 1].y,
                              td[i * 3 + 1].z, td[i * 3 + 1].w, td[i * 3 + 2].x, td[i * 3 + 2].y, td[i * 3 + 2].z,
                              td[i * 3 + 2].w};
@@ -1339,9 +1415,11 @@ int avbd_download(avbd_ctx* c, avbd_world* out) {
         t.rest_volume = d[9]; t.mu = d[10]; t.lambda = d[11];
         t.serial = c->tet_serial[i];
     }
+    k = 0;
     for (int i = 0; i < w.TR; ++i) {
-        avbd_tri& t = out->tris[i];
-        t.v[0] = rv[i].x; t.v[1] = rv[i].y; t.v[2] = rv[i].z;
+        if (c->tri_dead[i]) continue;
+        avbd_tri& t = out->tris[k++];
+        t.v[0] = L(rv[i].x); t.v[1] = L(rv[i].y); t.v[2] = L(rv[i].z);
         t.dm_inv[0] = rd[i * 2].x; t.dm_inv[1] = rd[i * 2].y; t.dm_inv[2] = rd[i * 2].z; t.dm_inv[3] = rd[i * 2].w;
         t.rest_area = rd[i * 2 + 1].x; t.mu = rd[i * 2 + 1].y; t.lambda = rd[i * 2 + 1].z;
         t.serial = c->tri_serial[i];
@@ -1381,7 +1459,9 @@ int avbd_get_colours(avbd_ctx* c, int32_t* out, int32_t n) {
     int rc = download_vec(c, col, c->w.colour, c->w.N);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < n && i < c->w.N; ++i) out[i] = col[i];
+    const std::vector<int> live = live_index(c);
+    for (int i = 0; i < c->w.N; ++i)
+        if (live[i] >= 0 && live[i] < n) out[live[i]] = col[i];
     return 0;
 }
 

@@ -1492,6 +1492,34 @@ __global__ void k_remap_forces(DevWorld w, const int* map) {
     }
 }
 
+// Removed bodies left as dead slots: static (no sweep, no colour, no
+// bounds or explosion check), a negative radius so no broad-phase test
+// passes, at rest.  Mass and friction (the w components) are kept.
+__global__ void k_kill_bodies(DevWorld w, const int* list, int n) {
+    GRID_STRIDE(k, n) {
+        const int b = list[k];
+        w.flags[b] |= AVBD_BODY_STATIC;
+        const float4 sc = w.scale[b];
+        w.scale[b] = make_float4(sc.x, sc.y, sc.z, -1e30f);
+        const float4 v = w.vel[b], av = w.angvel[b];
+        w.vel[b] = make_float4(0.f, 0.f, 0.f, v.w);
+        w.angvel[b] = make_float4(0.f, 0.f, 0.f, av.w);
+        w.pvel[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        w.pangvel[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// Element vertex ids renumbered (-1 for a removed body, kept by later
+// removals; such an element is removed with its body and no longer read).
+__global__ void k_remap_elements(DevWorld w, const int* map) {
+    GRID_STRIDE(t, w.T + w.TR) {
+        int4* v = t < w.T ? &w.t_v[t] : &w.r_v[t - w.T];
+        const int4 o = *v;
+        auto r = [&](int x) { return x >= 0 ? map[x] : -1; };
+        *v = make_int4(r(o.x), r(o.y), r(o.z), t < w.T ? r(o.w) : 0);
+    }
+}
+
 // Manifolds: endpoints renumbered (the bodies keep their roles A and B);
 // those that lose a body are dropped by the compaction that follows.
 __global__ void k_remap_manifolds(DevWorld w, const int* map) {

@@ -118,35 +118,91 @@ class Server:
         return k
 
 
+def _pct(x, q):
+    return round(float(np.percentile(x, q)), 3) if len(x) else None
+
+
+def _drain(server: Server, limit_s: float):
+    """Tick until nothing is queued or running, for at most limit_s seconds."""
+    t0 = server.now()
+    while server.now() - t0 < limit_s:
+        st = server.stats()
+        if st["queued"] == 0 and st["running"] == 0:
+            return
+        server.tick()
+
+
+def closed_loop(server: Server, scene_ids, duration: float, steps=(1, 8), seed: int = 0) -> dict:
+    """Saturation of the served form: every scene always has a request in
+    flight (a finished request is replaced at once by a new one for the same
+    scene, U{steps} steps), for `duration` seconds.  Returns the served
+    scene-steps/s and the tick time."""
+    rng = np.random.default_rng(seed)
+    ids = [int(s) for s in scene_ids]
+    _drain(server, 600.0)
+    server.poll()
+    for s in ids:
+        server.submit(s, int(rng.integers(steps[0], steps[1] + 1)))
+    s0 = server.stats()
+    t0 = server.now()
+    tick_ms, width = [], []
+    while server.now() - t0 < duration:
+        a = server.now()
+        width.append(server.tick())
+        tick_ms.append((server.now() - a) * 1e3)
+        for d in server.poll():
+            server.submit(d["scene"], int(rng.integers(steps[0], steps[1] + 1)))
+    elapsed = server.now() - t0
+    s1 = server.stats()
+    _drain(server, 600.0)
+    server.poll()
+    return {
+        "scene_steps_per_s": round((s1["scene_steps"] - s0["scene_steps"]) / elapsed, 1),
+        "tick_ms_p50": _pct(tick_ms, 50),
+        "scenes_per_tick_mean": round(float(np.mean(width)), 1) if width else 0.0,
+        "elapsed_s": round(elapsed, 3),
+    }
+
+
 def poisson_run(server: Server, scene_ids, rate: float, duration: float, steps=(1, 8), seed: int = 0,
-                log=None) -> dict:
+                log=None, drain_limit: float | None = None) -> dict:
     """Open-loop load: requests arrive as a Poisson process of `rate` per
     second for `duration` seconds, each for a uniformly drawn scene and
     U{steps[0]..steps[1]} steps.  A request's arrival time is its scheduled
-    time, so time spent waiting for a tick to end counts as queueing.  Runs
-    until every request has finished; returns latency percentiles (ms) and
-    the served scene-steps/s."""
+    time, so time spent waiting for a tick to end counts as queueing.  After
+    the arrivals the server drains for at most drain_limit seconds (default
+    2 x duration); requests still queued or running then are reported as the
+    backlog (an overloaded server never drains).  Returns latency percentiles
+    (ms) of the finished requests and the served scene-steps/s over the
+    arrival window."""
     rng = np.random.default_rng(seed)
     n_max = int(rate * duration * 1.5) + 64
     arr = np.cumsum(rng.exponential(1.0 / rate, n_max))
     arr = arr[arr < duration]
     sc = rng.choice(np.asarray(scene_ids), len(arr))
     ks = rng.integers(steps[0], steps[1] + 1, len(arr))
+    drain_limit = 2.0 * duration if drain_limit is None else drain_limit
+    server.poll()
     s0 = server.stats()
     t0 = server.now()
     i, tick_ms, width = 0, [], []
     next_log = 5.0
+    s_window = None
     while True:
         now = server.now() - t0
         while i < len(arr) and arr[i] <= now:
             server.submit(int(sc[i]), int(ks[i]), t0 + float(arr[i]))
             i += 1
+        if s_window is None and now >= duration:
+            s_window = server.stats()
         st = server.stats()
         if st["queued"] == 0 and st["running"] == 0:
             if i >= len(arr):
                 break
             time.sleep(max(0.0, float(arr[i]) - (server.now() - t0)))
             continue
+        if now > duration + drain_limit:
+            break
         a = server.now()
         n = server.tick()
         tick_ms.append((server.now() - a) * 1e3)
@@ -154,25 +210,29 @@ def poisson_run(server: Server, scene_ids, rate: float, duration: float, steps=(
         if log is not None and now > next_log:
             log(f"[serve] t={now:.1f}s submitted {i}/{len(arr)} queued {st['queued']} running {st['running']}")
             next_log += 5.0
-    elapsed = server.now() - t0
+    if s_window is None:
+        s_window = server.stats()
     done = server.poll()
+    end = server.stats()
+    backlog = end["queued"] + end["running"]
+    _drain(server, 600.0)  # leave the server idle for the next run
+    server.poll()
     lat = np.array([(d["t_done"] - d["t_arrival"]) * 1e3 for d in done])
     wait = np.array([(d["t_start"] - d["t_arrival"]) * 1e3 for d in done])
-    s1 = server.stats()
-    steps_served = s1["scene_steps"] - s0["scene_steps"]
-    pct = lambda x, q: round(float(np.percentile(x, q)), 3) if len(x) else None
+    steps_window = s_window["scene_steps"] - s0["scene_steps"]
     return {
-        "rate_req_per_s": rate,
+        "rate_req_per_s": round(rate, 2),
         "offered_scene_steps_per_s": round(rate * (steps[0] + steps[1]) / 2.0, 1),
-        "requests": int(len(done)),
-        "served_scene_steps_per_s": round(steps_served / elapsed, 1) if elapsed > 0 else 0.0,
-        "elapsed_s": round(elapsed, 3),
-        "latency_ms_p50": pct(lat, 50),
-        "latency_ms_p99": pct(lat, 99),
-        "queue_ms_p50": pct(wait, 50),
-        "queue_ms_p99": pct(wait, 99),
-        "tick_ms_p50": pct(tick_ms, 50),
-        "tick_ms_p99": pct(tick_ms, 99),
+        "requests": int(len(arr)),
+        "finished": int(len(done)),
+        "backlog": int(backlog),
+        "served_scene_steps_per_s": round(steps_window / duration, 1),
+        "latency_ms_p50": _pct(lat, 50),
+        "latency_ms_p99": _pct(lat, 99),
+        "queue_ms_p50": _pct(wait, 50),
+        "queue_ms_p99": _pct(wait, 99),
+        "tick_ms_p50": _pct(tick_ms, 50),
+        "tick_ms_p99": _pct(tick_ms, 99),
         "scenes_per_tick_mean": round(float(np.mean(width)), 1) if width else 0.0,
         "ticks": int(len(tick_ms)),
     }
