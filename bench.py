@@ -177,6 +177,9 @@ def main():
     ap.add_argument("--workload", default="mix")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body with forces (0 = autotuned)")
     ap.add_argument("--fem-lanes", type=int, default=0, help="lanes per FEM-only body (0 = autotuned)")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="time the steps without per-launch events (step graphs replay; no roofline figures)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel of a step one by one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson request) measurement")
@@ -197,14 +200,18 @@ def main():
     log(f"[bench] workload {args.workload}: {len(world.scenes)} scenes, {len(world.bodies)} bodies, "
         f"{len(world.tets)} tets, built in {time.perf_counter() - t0:.1f}s")
     ctx = Context(lanes=args.lanes, fem_lanes=args.fem_lanes)
+    if args.no_graphs:
+        ctx.set_option("step_graphs", 0)
     ctx.upload(world)
     log("[bench] uploaded")
     for i in range(args.warmup):
         ctx.step(1)
     ctx.synchronize()
     log(f"[bench] warmup {args.warmup} done")
-    ctx.set_option("kernel_timing", 1)
-    for k in KERNEL_CLASSES:
+    # per-launch events on the library's stream over the timed region (a
+    # step then launches its kernels one by one instead of replaying a graph)
+    ctx.set_option("kernel_timing", 0 if args.no_kernel_timing else 1)
+    for k in KERNEL_CLASSES + ("step_graph",):
         ctx.kernel_stats(k, reset=True)
     step_ms = []
     ctx.synchronize()
@@ -216,6 +223,7 @@ def main():
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
     ks = {k: ctx.kernel_stats(k) for k in KERNEL_CLASSES}
+    graph_replays = ctx.kernel_stats("step_graph")[0]
     ctx.set_option("kernel_timing", 0)
     timings = ctx.timings()
     state = ctx.download()
@@ -283,7 +291,10 @@ def main():
         "body_vertex_updates_per_s": round(updates * args.steps / elapsed, 1),
         "phase_ms_last_step": {k: round(v, 4) for k, v in timings.items() if k.endswith("_ms")},
         "dual_avg_launch_us": round(ms_dual * 1e3 / max(n_dual, 1), 3),
+        "step_graph_replays": graph_replays,
     }
+    if args.no_kernel_timing:
+        res["roofline"] = None
     if not args.no_plain and S > 1:
         log("[bench] plain per-scene comparison")
         plain = plain_gpu_rate(state, lanes=args.lanes, fem_lanes=args.fem_lanes)

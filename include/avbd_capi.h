@@ -209,7 +209,10 @@ typedef struct avbd_world {
     avbd_tri* tris;
 } avbd_world;
 
-/* Per-phase wall time of the last step, milliseconds (SolverTimings). */
+/* Per-phase wall time of the last step, milliseconds (SolverTimings).  When
+ * the step ran phases 4-7 as a replayed step graph (option "step_graphs"),
+ * their time is reported together in primal_dual_ms and prediction,
+ * velocity and post-stabilisation read 0. */
 typedef struct avbd_timings {
     float broad_phase_ms;
     float warmstart_ms;
@@ -272,7 +275,14 @@ int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
  *   "fem_lanes"      the same for particles whose only terms are FEM
  *                    energies
  *   "kernel_timing"  1 = bracket every primal-sweep and dual launch with HIP
- *                    events on the context's stream (see avbd_kernel_stats) */
+ *                    events on the context's stream (see avbd_kernel_stats);
+ *                    steps then run without graphs
+ *   "step_graphs"    1 (default) = phases 4-7 of a step (prediction, the
+ *                    iterations, velocities, post-stabilisation) are captured
+ *                    into a HIP graph the second time their launch shape
+ *                    (colours, iterations, lane widths and grids) occurs and
+ *                    replayed from then on; 0 = always launched one by one.
+ *                    Results are identical either way. */
 int avbd_set_option(avbd_ctx* ctx, const char* name, int64_t value);
 
 /* Autotuner table (no reference counterpart): the lane width chosen per
@@ -286,7 +296,8 @@ int avbd_tune_load(avbd_ctx* ctx, const char* json);
  * the last reset: kernel = "primal_forces" / "primal_fem" (each primal
  * bucket launch: bodies with forces / FEM-only particles, timed on the
  * stream it runs on), "primal" (both together) or "dual".  reset != 0 clears
- * after reading.  Requires the "kernel_timing" option. */
+ * after reading.  Requires the "kernel_timing" option.  kernel =
+ * "step_graph": launches = step graph replays, total_ms = graphs captured. */
 int avbd_kernel_stats(avbd_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms, int32_t reset);
 
 /* Device memory (no reference counterpart).  avbd_footprint: the bytes an

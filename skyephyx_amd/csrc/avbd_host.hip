@@ -35,7 +35,9 @@ size_t cub_tmp_bound(long long n_scan, long long n_sort) {
 }
 
 // Capacity of the broad-phase candidate and new-pair buffers.
-long long pair_capacity(long long N) { return std::max<long long>(65536, 4 * N); }
+// Broad-phase candidate buffer: every overlapping pair is a candidate,
+// those already in contact (at most the manifold capacity) included.
+long long pair_capacity(long long N, long long mcap) { return std::max<long long>(65536, 4 * N) + mcap; }
 
 // Significant bits of the new-pair sort keys (i << 32 | j, i < j < N).
 int pair_key_bits(int N) {
@@ -142,6 +144,26 @@ struct avbd_ctx {
         kmarks.clear();
     }
 
+    // Step graphs: phases 4-7 of a step (prediction, iterations, velocities,
+    // post-stabilisation) captured once per launch shape and replayed.  A
+    // shape is captured the second time it occurs, so shapes that never
+    // repeat (a served tick's changing scene set) stay eager.
+    struct StepGraph {
+        std::vector<int> key;
+        hipGraphExec_t exec = nullptr;
+        long long used = 0;
+    };
+    static constexpr size_t kMaxGraphs = 16;
+    bool use_graphs = true;
+    std::vector<StepGraph> graphs;
+    long long graph_clock = 0;
+    long long graph_launches = 0, graph_captures = 0;
+    void drop_graphs() {
+        for (StepGraph& g : graphs)
+            if (g.exec) hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+
     // dry: allocations are only counted (world_device_bytes)
     bool dry = false;
     size_t alloc_bytes = 0;
@@ -165,6 +187,7 @@ struct avbd_ctx {
     }
     void release() {
         if (stream) hipStreamSynchronize(stream);
+        drop_graphs();
         for (void* p : allocs) hipFree(p);
         allocs.clear();
         alloc_bytes = 0;
@@ -232,30 +255,47 @@ int build_incidence(avbd_ctx* c) {
     return 0;
 }
 
+// Blocks of a bucket launch: enough for n bodies at G lanes, rounded up to a
+// power of two (the kernel strides over the bucket, whose size it reads from
+// device memory), so that a step graph keyed on it survives small changes.
+int bucket_grid(int n, int G) {
+    const int g = grid_for((long long)n * G, 256, 16384);
+    int p = 1;
+    while (p < g) p <<= 1;
+    return std::min(p, 16384);
+}
+
 template <bool FEM>
-void launch_primal_g(avbd_ctx* c, int G, int lo, int hi, int iter, int poststab) {
+void launch_primal_g(avbd_ctx* c, int G, int grid, int bucket, int iter, int poststab) {
     const DevWorld& w = c->w;
-    const int grid = grid_for((long long)(hi - lo) * G, 256, 16384);
     switch (G) {
-        case 1: hipLaunchKernelGGL((k_primal<1, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
-        case 2: hipLaunchKernelGGL((k_primal<2, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
-        case 4: hipLaunchKernelGGL((k_primal<4, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
-        case 8: hipLaunchKernelGGL((k_primal<8, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
-        case 16: hipLaunchKernelGGL((k_primal<16, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
-        default: hipLaunchKernelGGL((k_primal<32, FEM>), dim3(grid), dim3(256), 0, c->stream, w, lo, hi, iter, poststab); break;
+        case 1: hipLaunchKernelGGL((k_primal<1, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
+        case 2: hipLaunchKernelGGL((k_primal<2, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
+        case 4: hipLaunchKernelGGL((k_primal<4, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
+        case 8: hipLaunchKernelGGL((k_primal<8, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
+        case 16: hipLaunchKernelGGL((k_primal<16, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
+        default: hipLaunchKernelGGL((k_primal<32, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
     }
 }
 
+// Lane width of a bucket launch of n bodies (class 0 = bodies with forces,
+// 1 = FEM-only) without starting a tuning sample: the fixed width, the tuned
+// one, or 0 while the tuner still samples this size.
+int bucket_lanes(const avbd_ctx* c, bool fem, int n) {
+    const int fixed = fem ? c->fem_lanes : c->lanes;
+    return fixed > 0 ? fixed : c->tuner.peek(fem ? 1 : 0, n);
+}
+
 // One bucket launch on c->stream with the lane width fixed by the user or
-// picked by the tuner (class 0 = bodies with forces, 1 = FEM-only).
+// picked by the tuner.
 template <bool FEM>
-void launch_bucket(avbd_ctx* c, int lo, int hi, int iter, int poststab) {
+void launch_bucket(avbd_ctx* c, int bucket, int n, int iter, int poststab) {
     int fixed = FEM ? c->fem_lanes : c->lanes;
     int measure = -1;
-    int G = fixed > 0 ? fixed : c->tuner.choose(FEM ? 1 : 0, hi - lo, &measure);
+    int G = fixed > 0 ? fixed : c->tuner.choose(FEM ? 1 : 0, n, &measure);
     const int mark = c->mark_begin(FEM ? avbd_ctx::K_FEM : avbd_ctx::K_FORCES);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].a, c->stream);
-    launch_primal_g<FEM>(c, G, lo, hi, iter, poststab);
+    launch_primal_g<FEM>(c, G, bucket_grid(n, G), bucket, iter, poststab);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].b, c->stream);
     c->mark_end(mark);
 }
@@ -265,22 +305,21 @@ void launch_bucket(avbd_ctx* c, int lo, int hi, int iter, int poststab) {
 // forked from and joined back into the main stream with events, so the next
 // colour still sees both).  Empty buckets are not launched.
 int launch_colour(avbd_ctx* c, const int* off, int colour, int iter, int poststab) {
-    const int g0 = off[2 * colour], g1 = off[2 * colour + 1], g2 = off[2 * colour + 2];
-    const bool gen = g1 > g0, fem = g2 > g1;
-    if (!gen && !fem) return 0;
-    if (gen && fem) {
+    const int n_gen = off[2 * colour + 1] - off[2 * colour], n_fem = off[2 * colour + 2] - off[2 * colour + 1];
+    if (n_gen == 0 && n_fem == 0) return 0;
+    if (n_gen > 0 && n_fem > 0) {
         HIP_TRY(hipEventRecord(c->fork, c->stream));
         HIP_TRY(hipStreamWaitEvent(c->side, c->fork, 0));
         std::swap(c->stream, c->side);
-        launch_bucket<true>(c, g1, g2, iter, poststab);
+        launch_bucket<true>(c, 2 * colour + 1, n_fem, iter, poststab);
         std::swap(c->stream, c->side);
-        launch_bucket<false>(c, g0, g1, iter, poststab);
+        launch_bucket<false>(c, 2 * colour, n_gen, iter, poststab);
         HIP_TRY(hipEventRecord(c->join, c->side));
         HIP_TRY(hipStreamWaitEvent(c->stream, c->join, 0));
-    } else if (gen) {
-        launch_bucket<false>(c, g0, g1, iter, poststab);
+    } else if (n_gen > 0) {
+        launch_bucket<false>(c, 2 * colour, n_gen, iter, poststab);
     } else {
-        launch_bucket<true>(c, g1, g2, iter, poststab);
+        launch_bucket<true>(c, 2 * colour + 1, n_fem, iter, poststab);
     }
     return 0;
 }
@@ -289,6 +328,7 @@ int launch_colour(avbd_ctx* c, const int* off, int colour, int iter, int poststa
 struct WorldDims {
     int S = 0, N = 0, J = 0, SP = 0, T = 0, TR = 0, Mcap = 0;
     int n_tiles = 0, n_items = 0;
+    long long pair_cap = 0;
     size_t scan_bytes = 0;
 };
 
@@ -304,7 +344,9 @@ int make_dims(const WorldCounts& k, long long mcap_req, WorldDims* d) {
     d->Mcap = int(mcap);
     d->n_tiles = int(k.tiles);
     d->n_items = int(k.items);
-    d->scan_bytes = cub_tmp_bound(std::max<long long>(k.N + 1, mcap + 1), pair_capacity(k.N));
+    d->pair_cap = pair_capacity(k.N, mcap);
+    if (d->pair_cap > (1ll << 31) - 1) return fail(AVBD_E_CAPACITY, "broad-phase pair buffer too large");
+    d->scan_bytes = cub_tmp_bound(std::max<long long>(k.N + 1, mcap + 1), d->pair_cap);
     return 0;
 }
 
@@ -345,7 +387,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     A(&w.ev_c, (size_t)4 * T + 3 * TR + 1); A(&w.ev_v, (size_t)4 * T + 3 * TR + 1);
     A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
     A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1);
-    w.pair_cap = (int)pair_capacity(N);
+    w.pair_cap = (int)d.pair_cap;
     A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
     int* toff;
     int* tscene;
@@ -521,7 +563,7 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, dummy, dummy, N + 1, c->stream));
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, dummy, dummy, dims.Mcap + 1, c->stream));
         hipcub::DoubleBuffer<unsigned long long> kb(nullptr, nullptr);
-        HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b3, kb, (int)pair_capacity(N), 0, pair_key_bits(N), c->stream));
+        HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b3, kb, (int)dims.pair_cap, 0, pair_key_bits(N), c->stream));
         dims.scan_bytes = std::max(dims.scan_bytes, std::max(b1, std::max(b2, b3)));
     }
     DevWorld& w = c->w;
@@ -745,6 +787,7 @@ int remove_out_of_bounds(avbd_ctx* c) {
     DevWorld& w = c->w;
     hipStream_t st = c->stream;
     const int S = w.S, N = w.N;
+    c->drop_graphs();
     std::vector<int> stw;
     std::vector<float4> pos;
     std::vector<uint32_t> flags;
@@ -943,6 +986,55 @@ int remove_out_of_bounds(avbd_ctx* c) {
     return poststab_scenes(c, aff);
 }
 
+// Phases 4-7 of a step for the bucket offsets off: prediction, the
+// iterations (per colour the primal buckets, then the dual and the fracture
+// reset), velocities and post-stabilisation.  marks: record the phase events
+// (not under stream capture).
+int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_post, int dgrid, bool marks) {
+    DevWorld& w = c->w;
+    hipStream_t st = c->stream;
+    const int N = w.N;
+    hipLaunchKernelGGL(k_predict, dim3(grid_for(N)), dim3(256), 0, st, w);
+    if (marks) HIP_TRY(hipEventRecord(c->ev[3], st));
+    for (int it = 0; it < max_iters; ++it) {
+        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, it, 0);
+        const int mark = c->mark_begin(avbd_ctx::K_DUAL);
+        hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
+        c->mark_end(mark);
+        if (w.J + w.SP > 0) {
+            hipLaunchKernelGGL(k_fracture_reset, dim3(grid_for(2 * (w.J + w.SP) + 2)), dim3(256), 0, st, w);
+            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_FRACTURES, 0);
+        }
+    }
+    if (marks) HIP_TRY(hipEventRecord(c->ev[4], st));
+    hipLaunchKernelGGL(k_velocity, dim3(grid_for(N)), dim3(256), 0, st, w);
+    if (marks) HIP_TRY(hipEventRecord(c->ev[5], st));
+    if (any_post) {
+        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, -1, 1);
+        hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, st, w);
+    }
+    return 0;
+}
+
+// Launch shape of phases 4-7: everything a captured graph fixes (manifold
+// buffer, colours, iterations, per bucket the lane width and grid, the dual
+// grid).  False while a bucket's lane width is still being tuned.
+bool step_graph_key(const avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_post, int dgrid,
+                    std::vector<int>* key) {
+    key->assign({c->cur, ncol, max_iters, any_post ? 1 : 0, dgrid});
+    for (int k = 0; k < 2 * ncol; ++k) {
+        const int n = off[k + 1] - off[k];
+        if (n == 0) {
+            key->push_back(0);
+            continue;
+        }
+        const int G = bucket_lanes(c, (k & 1) != 0, n);
+        if (G == 0) return false;
+        key->push_back(G * 65536 + bucket_grid(n, G));
+    }
+    return true;
+}
+
 int do_step(avbd_ctx* c) {
     DevWorld& w = c->w;
     hipStream_t st = c->stream;
@@ -994,30 +1086,63 @@ int do_step(avbd_ctx* c) {
     const int ncol = g[G_NCOLOURS];
     c->last_ncol = ncol;
     HIP_TRY(hipEventRecord(c->ev[2], st));
-    // ---- 4. prediction
-    hipLaunchKernelGGL(k_predict, dim3(grid_for(N)), dim3(256), 0, st, w);
-    HIP_TRY(hipEventRecord(c->ev[3], st));
-    // ---- 5. iterations
-    const int dgrid = grid_for((long long)g[G_MANIFOLDS] * 8 + w.J + w.SP, 256, 8192);
-    for (int it = 0; it < max_iters; ++it) {
-        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, it, 0);
-        const int mark = c->mark_begin(avbd_ctx::K_DUAL);
-        hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
-        c->mark_end(mark);
-        if (w.J + w.SP > 0) {
-            hipLaunchKernelGGL(k_fracture_reset, dim3(grid_for(2 * (w.J + w.SP) + 2)), dim3(256), 0, st, w);
-            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_FRACTURES, 0);
+    int dgrid = grid_for((long long)g[G_MANIFOLDS] * 8 + w.J + w.SP, 256, 8192);
+    {
+        int p = 1;
+        while (p < dgrid) p <<= 1;
+        dgrid = std::min(p, 8192);
+    }
+    // ---- 4.-7. as one graph when this launch shape has been seen before
+    std::vector<int> key;
+    if (c->use_graphs && !c->ktiming && step_graph_key(c, off, ncol, max_iters, any_post, dgrid, &key)) {
+        avbd_ctx::StepGraph* hit = nullptr;
+        for (avbd_ctx::StepGraph& sg : c->graphs)
+            if (sg.key == key) hit = &sg;
+        if (hit && !hit->exec) {
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = sweep_phases(c, off, ncol, max_iters, any_post, dgrid, false);
+            hipError_t e = hipStreamEndCapture(st, &graph);
+            if (rc) {
+                if (graph) hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(AVBD_E_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&hit->exec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                hit->exec = nullptr;
+                return fail(AVBD_E_HIP, std::string("step graph instantiate: ") + hipGetErrorString(e));
+            }
+            ++c->graph_captures;
         }
+        if (hit) {
+            hit->used = ++c->graph_clock;
+            // phases 4-7 run inside the graph: reported together as primal_dual
+            HIP_TRY(hipEventRecord(c->ev[3], st));
+            HIP_TRY(hipGraphLaunch(hit->exec, st));
+            ++c->graph_launches;
+            HIP_TRY(hipEventRecord(c->ev[4], st));
+            HIP_TRY(hipEventRecord(c->ev[5], st));
+            HIP_TRY(hipEventRecord(c->ev[6], st));
+            HIP_TRY(hipGetLastError());
+            return 0;
+        }
+        // first sighting: remember the shape, run eagerly
+        if (c->graphs.size() >= avbd_ctx::kMaxGraphs) {
+            auto lru = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                        [](const avbd_ctx::StepGraph& a, const avbd_ctx::StepGraph& b) {
+                                            return a.used < b.used;
+                                        });
+            if (lru->exec) hipGraphExecDestroy(lru->exec);
+            c->graphs.erase(lru);
+        }
+        avbd_ctx::StepGraph sg;
+        sg.key = key;
+        sg.used = ++c->graph_clock;
+        c->graphs.push_back(std::move(sg));
     }
-    HIP_TRY(hipEventRecord(c->ev[4], st));
-    // ---- 6. velocities
-    hipLaunchKernelGGL(k_velocity, dim3(grid_for(N)), dim3(256), 0, st, w);
-    HIP_TRY(hipEventRecord(c->ev[5], st));
-    // ---- 7. post-stabilisation
-    if (any_post) {
-        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, -1, 1);
-        hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, st, w);
-    }
+    if ((rc = sweep_phases(c, off, ncol, max_iters, any_post, dgrid, true))) return rc;
     HIP_TRY(hipEventRecord(c->ev[6], st));
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1161,6 +1286,11 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
         c->ktiming = value != 0;
         return 0;
     }
+    if (n == "step_graphs") {
+        c->use_graphs = value != 0;
+        if (!c->use_graphs) c->drop_graphs();
+        return 0;
+    }
     return fail(AVBD_E_ARG, "unknown option " + n);
 }
 
@@ -1183,6 +1313,12 @@ int avbd_tune_load(avbd_ctx* c, const char* json) {
 int avbd_kernel_stats(avbd_ctx* c, const char* kernel, int64_t* launches, double* total_ms, int32_t reset) {
     if (!c || !kernel) return fail(AVBD_E_ARG, "null argument");
     std::string k(kernel);
+    if (k == "step_graph") {  // replays of captured steps (launches), captures (total_ms field: count)
+        if (launches) *launches = c->graph_launches;
+        if (total_ms) *total_ms = double(c->graph_captures);
+        if (reset) c->graph_launches = c->graph_captures = 0;
+        return 0;
+    }
     // "primal" = both bucket classes together
     int lo = k == "primal" ? avbd_ctx::K_FORCES
            : k == "dual" ? avbd_ctx::K_DUAL

@@ -1176,8 +1176,12 @@ AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int p
     }
 }
 
+// One sweep bucket (bodies col_list[col_off[bucket] .. col_off[bucket+1])),
+// read from device memory so that a launch does not depend on this step's
+// bucket sizes (a captured step graph replays with the sizes of its step).
 template <int G, bool FEM>
-__global__ __launch_bounds__(256) void k_primal(DevWorld w, int lo, int hi, int iter, int poststab) {
+__global__ __launch_bounds__(256) void k_primal(DevWorld w, int bucket, int iter, int poststab) {
+    const int lo = w.col_off[bucket], hi = w.col_off[bucket + 1];
     if constexpr (FEM) {
         primal_fem_body<G>(w, lo, hi, iter, poststab);
         return;
@@ -1269,8 +1273,8 @@ __global__ __launch_bounds__(256) void k_primal(DevWorld w, int lo, int hi, int 
 }
 
 #define AVBD_PRIMAL_INST(G)                                                        \
-    template __global__ void k_primal<G, false>(DevWorld, int, int, int, int); \
-    template __global__ void k_primal<G, true>(DevWorld, int, int, int, int);
+    template __global__ void k_primal<G, false>(DevWorld, int, int, int); \
+    template __global__ void k_primal<G, true>(DevWorld, int, int, int);
 AVBD_PRIMAL_INST(1)
 AVBD_PRIMAL_INST(2)
 AVBD_PRIMAL_INST(4)

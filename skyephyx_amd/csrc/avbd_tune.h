@@ -78,6 +78,15 @@ struct LaneTuner {
         return kCands[ci];
     }
 
+    // The width choose() would return without sampling: 0 while the key is
+    // still being tuned.
+    int peek(int cls, int n) const {
+        auto it = table.find(key(cls, n));
+        const int best = it == table.end() ? 0 : it->second.best;
+        if (!enabled) return best ? best : 8;
+        return best;
+    }
+
     // After the stream(s) have been synchronised.
     void collect() {
         for (const Pending& p : pending) {
