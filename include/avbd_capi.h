@@ -25,6 +25,11 @@
  *                                force.hpp:9-24,167-198)
  *   avbd_get_status           -> Solver::emergencyStop (solver.hpp:86),
  *                                CheckExplosion (solver.cpp:182-196)
+ *   avbd_last_removed         -> the RemoveBody calls of Step's bounds check
+ *                                (solver.cpp:520-534)
+ *   avbd_download_bodies,
+ *   avbd_upload_bodies        -> per-frame reads/writes of Mesh state
+ *                                (geometry.hpp:148-256)
  *   avbd_get_timings          -> Solver::timings / SolverTimings
  *                                (solver.hpp:10-19, solver.cpp:599-626)
  *   avbd_step_scenes          -> (no counterpart: batched/served stepping of a
@@ -262,6 +267,28 @@ int avbd_download(avbd_ctx* ctx, avbd_world* out);
 
 /* Per-scene status words: emergency_stop and n_removed for `n` scenes. */
 int avbd_get_status(avbd_ctx* ctx, int32_t* emergency_stop, int32_t* n_removed, int32_t n);
+
+/* Bodies the last avbd_step / avbd_step_scenes call removed for leaving the
+ * world bounds (Solver::RemoveBody from Step, solver.cpp:520-534,
+ * 104-116), in removal order: the scene and the body's index within its
+ * scene at the moment of its removal (its solverIndex then).  Applying
+ * RemoveBody (swap with last) in this order to a host copy of the scene
+ * reproduces the device's body order.  Writes at most cap entries; *n_out
+ * receives the full count. */
+int avbd_last_removed(avbd_ctx* ctx, int32_t* scene, int32_t* index, int32_t cap, int32_t* n_out);
+
+/* Body state only, n = the live body count (avbd_get_counts), in body order:
+ * the per-step read of bodyPtrs[i]->transform/velocity by the renderer and
+ * game loop (GameManager.cpp:92-99, RenderEngine.cpp:951-1029) without the
+ * forces and contacts. */
+int avbd_download_bodies(avbd_ctx* ctx, avbd_body* out, int32_t n);
+
+/* Overwrite the dynamic state of every live body (pos, rot, vel, ang_vel,
+ * prev_vel, prev_ang_vel; other fields of `in` are ignored): a host facade's
+ * direct writes to Mesh::transform/velocity between steps (e.g. dragging,
+ * geometry.hpp:193-194).  Mass, inertia, scale and flags change only
+ * through avbd_upload. */
+int avbd_upload_bodies(avbd_ctx* ctx, const avbd_body* in, int32_t n);
 
 int avbd_get_timings(avbd_ctx* ctx, avbd_timings* out);
 

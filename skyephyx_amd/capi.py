@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from .abi import AvbdRequestInfo, AvbdServerStats, AvbdTimings, AvbdWorld, alloc_world, world_to_c
-from .world import World
+from .world import BODY_DTYPE, World
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libavbd_hip.so")
@@ -47,6 +47,10 @@ def lib():
         L.avbd_download.argtypes = [vp, C.POINTER(AvbdWorld)]
         L.avbd_get_status.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32]
         L.avbd_get_timings.argtypes = [vp, C.POINTER(AvbdTimings)]
+        L.avbd_last_removed.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                                         C.POINTER(C.c_int32)]
+        L.avbd_download_bodies.argtypes = [vp, vp, C.c_int32]
+        L.avbd_upload_bodies.argtypes = [vp, vp, C.c_int32]
         L.avbd_get_colours.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.avbd_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
         L.avbd_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int32]
@@ -163,6 +167,27 @@ class Context:
         _check(self.L.avbd_get_status(self.h, em.ctypes.data_as(C.POINTER(C.c_int32)),
                                       rem.ctypes.data_as(C.POINTER(C.c_int32)), n_scenes))
         return em, rem
+
+    def last_removed(self):
+        """(scene, index at removal) of the bodies the last step call removed, in order."""
+        n = C.c_int32()
+        _check(self.L.avbd_last_removed(self.h, None, None, 0, C.byref(n)))
+        sc = np.zeros(n.value, np.int32)
+        ix = np.zeros(n.value, np.int32)
+        _check(self.L.avbd_last_removed(self.h, sc.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        ix.ctypes.data_as(C.POINTER(C.c_int32)), n.value, C.byref(n)))
+        return list(zip(sc.tolist(), ix.tolist()))
+
+    def download_bodies(self) -> np.ndarray:
+        cnt = AvbdWorld()
+        _check(self.L.avbd_get_counts(self.h, C.byref(cnt)))
+        out = np.zeros(cnt.n_bodies, BODY_DTYPE)
+        _check(self.L.avbd_download_bodies(self.h, out.ctypes.data, cnt.n_bodies))
+        return out
+
+    def upload_bodies(self, bodies: np.ndarray):
+        b = np.ascontiguousarray(bodies, BODY_DTYPE)
+        _check(self.L.avbd_upload_bodies(self.h, b.ctypes.data, len(b)))
 
     def timings(self) -> dict:
         t = AvbdTimings()

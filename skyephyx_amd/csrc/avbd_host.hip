@@ -90,6 +90,9 @@ struct avbd_ctx {
     // scene's index range (static, no broad-phase sphere, no energies);
     // removed elements keep their index.  Downloads skip both.
     std::vector<char> body_dead, tet_dead, tri_dead;
+    // bodies removed by the current avbd_step call: scene, index in the scene
+    // at the moment of removal (RemoveBody order)
+    std::vector<int32_t> removed_scene, removed_index;
     std::vector<int32_t> host_colours;
     hipEvent_t ev[8]{};
     avbd_timings timings{};
@@ -838,6 +841,8 @@ int remove_out_of_bounds(avbd_ctx* c) {
             if (p.x < sc.bounds_min[0] || p.x > sc.bounds_max[0] || p.y < sc.bounds_min[1] || p.y > sc.bounds_max[1] ||
                 p.z < sc.bounds_min[2] || p.z > sc.bounds_max[2]) {
                 const int k = where[i - a], last = (int)order.size() - 1;
+                c->removed_scene.push_back(s);
+                c->removed_index.push_back(k);
                 if (k != last) {
                     order[k] = order[last];
                     where[order[k] - a] = k;
@@ -1352,6 +1357,8 @@ static int step_impl(avbd_ctx* c, int32_t n_steps) {
     if (n_steps < 0) return fail(AVBD_E_ARG, "negative step count");
     HIP_TRY(hipSetDevice(c->device));
     avbd_timings acc{};
+    c->removed_scene.clear();
+    c->removed_index.clear();
     for (int k = 0; k < n_steps; ++k) {
         int rc = do_step(c);
         if (rc) return rc;
@@ -1579,6 +1586,88 @@ int avbd_get_status(avbd_ctx* c, int32_t* emergency, int32_t* removed, int32_t n
         if (emergency) emergency[s] = st[s * ST_WORDS + ST_EMERGENCY];
         if (removed) removed[s] = st[s * ST_WORDS + ST_REMOVED];
     }
+    return 0;
+}
+
+int avbd_last_removed(avbd_ctx* c, int32_t* scene, int32_t* index, int32_t cap, int32_t* n_out) {
+    if (!c || !n_out) return fail(AVBD_E_ARG, "null argument");
+    const int n = (int)c->removed_index.size();
+    *n_out = n;
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (scene) scene[i] = c->removed_scene[i];
+        if (index) index[i] = c->removed_index[i];
+    }
+    return 0;
+}
+
+int avbd_download_bodies(avbd_ctx* c, avbd_body* out, int32_t n) {
+    if (!c || (!out && n > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    const DevWorld& w = c->w;
+    const std::vector<int> live = live_index(c);
+    const int n_live = w.N - (int)std::count(c->body_dead.begin(), c->body_dead.end(), 1);
+    if (n != n_live) return fail(AVBD_E_ARG, "body count differs from the device's live bodies");
+    std::vector<float4> pos, rot, scale, vel, av, pv, pav, inert;
+    std::vector<uint32_t> flags;
+    std::vector<int> scene;
+    int rc;
+    if ((rc = download_vec(c, pos, w.pos, w.N)) || (rc = download_vec(c, rot, w.rot, w.N)) ||
+        (rc = download_vec(c, scale, w.scale, w.N)) || (rc = download_vec(c, vel, w.vel, w.N)) ||
+        (rc = download_vec(c, av, w.angvel, w.N)) || (rc = download_vec(c, pv, w.pvel, w.N)) ||
+        (rc = download_vec(c, pav, w.pangvel, w.N)) || (rc = download_vec(c, inert, w.inertia, w.N)) ||
+        (rc = download_vec(c, flags, w.flags, w.N)) || (rc = download_vec(c, scene, w.scene, w.N)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    auto put3 = [](float* d, float4 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; };
+    for (int i = 0; i < w.N; ++i) {
+        if (live[i] < 0) continue;
+        avbd_body& b = out[live[i]];
+        std::memset(&b, 0, sizeof(b));
+        put3(b.pos, pos[i]);
+        b.rot[0] = rot[i].x; b.rot[1] = rot[i].y; b.rot[2] = rot[i].z; b.rot[3] = rot[i].w;
+        put3(b.scale, scale[i]);
+        b.radius = scale[i].w;
+        put3(b.vel, vel[i]);
+        b.mass = vel[i].w;
+        put3(b.ang_vel, av[i]);
+        b.friction = av[i].w;
+        put3(b.prev_vel, pv[i]);
+        put3(b.prev_ang_vel, pav[i]);
+        put3(b.inertia, inert[i]);
+        b.flags = flags[i];
+        b.scene = scene[i];
+    }
+    return 0;
+}
+
+int avbd_upload_bodies(avbd_ctx* c, const avbd_body* in, int32_t n) {
+    if (!c || (!in && n > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    DevWorld& w = c->w;
+    const std::vector<int> live = live_index(c);
+    const int n_live = w.N - (int)std::count(c->body_dead.begin(), c->body_dead.end(), 1);
+    if (n != n_live) return fail(AVBD_E_ARG, "body count differs from the device's live bodies");
+    std::vector<float4> pos, rot, vel, av, pv, pav;
+    int rc;
+    if ((rc = download_vec(c, pos, w.pos, w.N)) || (rc = download_vec(c, rot, w.rot, w.N)) ||
+        (rc = download_vec(c, vel, w.vel, w.N)) || (rc = download_vec(c, av, w.angvel, w.N)) ||
+        (rc = download_vec(c, pv, w.pvel, w.N)) || (rc = download_vec(c, pav, w.pangvel, w.N)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < w.N; ++i) {
+        if (live[i] < 0) continue;
+        const avbd_body& b = in[live[i]];
+        pos[i] = make_float4(b.pos[0], b.pos[1], b.pos[2], pos[i].w);
+        rot[i] = make_float4(b.rot[0], b.rot[1], b.rot[2], b.rot[3]);
+        vel[i] = make_float4(b.vel[0], b.vel[1], b.vel[2], vel[i].w);
+        av[i] = make_float4(b.ang_vel[0], b.ang_vel[1], b.ang_vel[2], av[i].w);
+        pv[i] = make_float4(b.prev_vel[0], b.prev_vel[1], b.prev_vel[2], 0.f);
+        pav[i] = make_float4(b.prev_ang_vel[0], b.prev_ang_vel[1], b.prev_ang_vel[2], 0.f);
+    }
+    if ((rc = upload_vec(c, w.pos, pos)) || (rc = upload_vec(c, w.rot, rot)) || (rc = upload_vec(c, w.vel, vel)) ||
+        (rc = upload_vec(c, w.angvel, av)) || (rc = upload_vec(c, w.pvel, pv)) || (rc = upload_vec(c, w.pangvel, pav)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 

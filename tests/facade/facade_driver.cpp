@@ -1,0 +1,166 @@
+// facade_driver.cpp -- test driver of the C++ solver facade
+// (skyephyx_amd/physics).  Built by tests/test_facade.py.
+//
+//   facade_driver records <scene> <out>     world records as built (no GPU)
+//   facade_driver step <scene> <steps> <lanes> <fem_lanes> <out>
+//                                           records after <steps> Step() calls
+//   facade_driver semantics                 host-side API checks (no GPU)
+//
+// <out> holds int32 counts (scenes, bodies, joints, springs, manifolds,
+// tets, tris) followed by the record arrays of include/avbd_capi.h in that
+// order.  `step` also prints the counts after every step and the names of
+// the bodies left, in order.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+
+#include "levels.hpp"
+#include "solver.hpp"
+
+namespace {
+
+bool build(Solver& s, const std::string& scene) {
+    if (scene == "pyramid") Levels::pyramid2D(&s);
+    else if (scene == "joint_chain") Levels::jointChain(&s, 5);
+    else if (scene == "spring_bridge") Levels::springBridge(&s, 6, 3);
+    else if (scene == "cloth") Levels::clothSheet(&s, 6);
+    else if (scene == "fem_block") {
+        const int n = 5;
+        const double cell = 0.25, o = -(n - 1) * cell * 0.5;
+        const float mass = float(1000.0 * cell * cell * cell);
+        GeometryHelpers::makeTetVolume(&s, o, 0.0, o, n, n, n, cell, 1e5f, 0.3f, mass, 0.5f,
+                                       [n](int, int j, int) { return j == n - 1; });
+    } else if (scene == "out_of_bounds") Levels::outOfBounds(&s);
+    else return false;
+    return true;
+}
+
+template <class T>
+void write_array(FILE* f, const std::vector<T>& v) {
+    if (!v.empty()) std::fwrite(v.data(), sizeof(T), v.size(), f);
+}
+
+int write_records(Solver& s, const char* path) {
+    WorldRecords r;
+    s.ExportWorld(r);
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return 1;
+    const int32_t counts[7] = {1,
+                               (int32_t)r.bodies.size(),
+                               (int32_t)r.joints.size(),
+                               (int32_t)r.springs.size(),
+                               (int32_t)r.manifolds.size(),
+                               (int32_t)r.tets.size(),
+                               (int32_t)r.tris.size()};
+    std::fwrite(counts, sizeof(counts), 1, f);
+    std::fwrite(&r.scene, sizeof(r.scene), 1, f);
+    write_array(f, r.bodies);
+    write_array(f, r.joints);
+    write_array(f, r.springs);
+    write_array(f, r.manifolds);
+    write_array(f, r.tets);
+    write_array(f, r.tris);
+    std::fclose(f);
+    return 0;
+}
+
+int failures = 0;
+void expect(bool ok, const char* what) {
+    if (!ok) {
+        std::printf("FAIL %s\n", what);
+        ++failures;
+    }
+}
+
+Mesh* unit_box(Solver& s, float x, bool isStatic = false) {
+    return s.AddBody(ModelType_Cube, 1.f, 0.5f, Vector3f(x, 0.f, 0.f), Vector3f(1.f, 1.f, 1.f), Vector3f::Zero(),
+                     Quaternionf::Identity(), Vector3f::Zero(), isStatic);
+}
+
+// The reference's ownership rules (solver.cpp:50-164, geometry.cpp:11-24,
+// force.cpp:5-19, energy.cpp:5-19) on host objects only.
+int semantics() {
+    Solver s;
+    Mesh* b[5];
+    for (int i = 0; i < 5; ++i) b[i] = unit_box(s, 2.f * i);
+    expect(s.bodyPtrs.size() == 5 && b[3]->solverIndex == 3, "AddBody appends with solverIndex");
+    expect(std::fabs(b[0]->mass - 1.f) < 1e-7f && std::fabs(b[0]->inertiaTensorBody(0, 0) - 1.f / 6.f) < 1e-7f,
+           "box mass and inertia");
+    expect(std::fabs(b[0]->detectionRadius - 0.5f * std::sqrt(3.f)) < 1e-6f, "detection radius |scale|/2");
+    Force* j = s.AddForce(std::make_unique<Joint>(&s, b[0], Vector3f(0.5f, 0.f, 0.f), b[2], Vector3f(-0.5f, 0.f, 0.f)));
+    Force* sp = s.AddForce(std::make_unique<Spring>(&s, b[1], Vector3f::Zero(), b[2], Vector3f::Zero(), 100.f));
+    expect(b[2]->forces.size() == 2 && b[2]->forces[0] == j && b[2]->forces[1] == sp, "Mesh::forces in creation order");
+    expect(std::fabs(static_cast<Spring*>(sp)->restLength - 2.f) < 1e-6f, "spring rest length = current distance");
+    expect(isConstrainedTo(b[0], b[2]) && !isConstrainedTo(b[0], b[1]), "isConstrainedTo");
+    expect(j->serial < sp->serial, "creation serials increase");
+    {
+        Manifold probe(&s, b[3], b[4]);
+        expect(probe.constraintPoints[0].fmaxMagnitude == 0.f && std::isinf(probe.constraintPoints[0].fminMagnitude) &&
+                   !probe.includeHessian && probe.isManifold && b[3]->forces.size() == 1,
+               "manifold normal rows compressive only");
+    }
+    expect(b[3]->forces.empty(), "~Force leaves the bodies' lists");
+    // RemoveBody: swap with last, the body's forces go with it
+    s.RemoveBody(b[0]);
+    expect(s.bodyPtrs.size() == 4 && s.bodyPtrs[0] == b[4] && b[4]->solverIndex == 0, "RemoveBody swaps with last");
+    expect(s.solverForces.size() == 1 && s.solverForces[0].get() == sp && sp->solverIndex == 0,
+           "removing a body removes its forces (swap with last)");
+    expect(b[2]->forces.size() == 1 && b[2]->forces[0] == sp, "~Force leaves the other body's list");
+    // energies
+    Mesh* p[4];
+    for (int i = 0; i < 4; ++i)
+        p[i] = s.AddParticle(0.1f, 0.5f, Vector3f(i == 1 ? 1.f : 0.f, i == 2 ? 1.f : 0.f, i == 3 ? 1.f : 0.f),
+                             Vector3f::Zero(), false);
+    expect(p[0]->isParticle && p[0]->inertiaTensorBody(0, 0) == 0.f, "particles have no inertia");
+    auto* tet = static_cast<NeoHookeanFEM*>(
+        s.AddEnergy(std::make_unique<NeoHookeanFEM>(&s, p[0], p[1], p[2], p[3], 1e5f, 0.3f)));
+    expect(std::fabs(tet->restVolume - 1.f / 6.f) < 1e-7f && tet->DmInv[0] == 1.f && tet->DmInv[4] == 1.f,
+           "tet rest volume and Dm^-1 of the unit corner");
+    expect(std::fabs(tet->lameMu - 1e5f / 2.6f) < 1e-2f, "Lame mu");
+    expect(p[3]->energies.size() == 1, "Energy registers with its vertices");
+    s.RemoveBody(p[1]);
+    expect(s.solverEnergies.empty() && p[0]->energies.empty(), "removing a vertex removes its energies");
+    s.Clear();
+    expect(s.bodyPtrs.empty() && s.solverForces.empty() && s.solverEnergies.empty(), "Clear empties the world");
+    std::printf("%d failures\n", failures);
+    return failures ? 1 : 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    try {
+        if (mode == "semantics") return semantics();
+        if (mode == "records" && argc == 4) {
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            return write_records(s, argv[3]);
+        }
+        if (mode == "step" && argc == 7) {
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            avbd_ctx* ctx = s.DeviceContext();
+            if (avbd_set_option(ctx, "lanes", std::atoi(argv[4])) || avbd_set_option(ctx, "fem_lanes", std::atoi(argv[5])))
+                return 3;
+            std::vector<std::string> names;
+            const int steps = std::atoi(argv[3]);
+            for (int k = 0; k < steps; ++k) {
+                s.Step();
+                std::printf("step %d bodies %zu forces %zu energies %zu emergency %d avg_ms %.4f\n", k, s.bodyPtrs.size(),
+                            s.forcePtrs.size(), s.energyPtrs.size(), int(s.emergencyStop), s.averageStepTime);
+            }
+            for (Mesh* m : s.bodyPtrs) std::printf("body %s\n", m->name.c_str());
+            return write_records(s, argv[6]);
+        }
+    } catch (const std::exception& e) {
+        std::printf("error: %s\n", e.what());
+        return 4;
+    }
+    return 2;
+}

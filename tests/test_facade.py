@@ -1,0 +1,163 @@
+"""The C++ solver facade (skyephyx_amd/physics): the reference's Solver API
+(src/physics/solver.hpp) over the C ABI.
+
+* Host side (no GPU): the ownership rules of the reference -- RemoveBody
+  swaps with the last body and takes the body's forces and energies with it,
+  forces and energies register in their bodies' lists in creation order
+  (solver.cpp:50-164, geometry.cpp:11-24, force.cpp:5-19, energy.cpp:5-19)
+  -- and the world records the facade builds: the same scenes built with
+  the C++ API and with the Python builder (which follows the same
+  constructors) give the same records.
+* GPU: Step() through the facade moves the world exactly as the C ABI
+  driven directly does (same lane widths, so bit for bit), with the
+  contacts the device creates mirrored into solverForces and the bodies it
+  removes for leaving the bounds removed from the host world.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from skyephyx_amd import scenes
+from skyephyx_amd.world import (BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, SCENE_DTYPE, SPRING_DTYPE, TET_DTYPE,
+                                TRI_DTYPE, World)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "skyephyx_amd", "lib")
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    if not os.path.exists(os.path.join(LIB, "libskyephyx_physics.so")):
+        pytest.fail("libskyephyx_physics.so is missing: build first (__graft_entry__.build())")
+    exe = str(tmp_path_factory.mktemp("facade") / "facade_driver")
+    cmd = ["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "skyephyx_amd", "physics"), "-I",
+           os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "facade", "facade_driver.cpp"), "-L", LIB,
+           "-lskyephyx_physics", "-lavbd_hip", f"-Wl,-rpath,{LIB}", "-o", exe]
+    subprocess.run(cmd, check=True)
+    return exe
+
+
+def read_records(path) -> World:
+    raw = open(path, "rb").read()
+    counts = np.frombuffer(raw[:28], np.int32)
+    off = 28
+    arrays = []
+    for n, dt in zip(counts, (SCENE_DTYPE, BODY_DTYPE, JOINT_DTYPE, SPRING_DTYPE, MANIFOLD_DTYPE, TET_DTYPE,
+                              TRI_DTYPE)):
+        a = np.frombuffer(raw[off:off + int(n) * dt.itemsize], dt).copy()
+        off += int(n) * dt.itemsize
+        arrays.append(a)
+    assert off == len(raw)
+    return World(*arrays)
+
+
+def python_scene(name) -> World:
+    return {
+        "pyramid": lambda: scenes.pyramid_2d(),
+        "joint_chain": lambda: scenes.joint_chain(5),
+        "spring_bridge": lambda: scenes.spring_bridge(6, 3),
+        "cloth": lambda: scenes.cloth_sheet(n=6, spacing=0.25),
+        "fem_block": lambda: scenes.fem_block(n=5, cell=0.25),
+        "out_of_bounds": lambda: scenes.out_of_bounds(),
+    }[name]()
+
+
+SCENES = ["pyramid", "joint_chain", "spring_bridge", "cloth", "fem_block", "out_of_bounds"]
+
+
+def test_host_semantics(driver):
+    r = subprocess.run([driver, "semantics"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def _same(a, b, fields, where, rtol=0.0):
+    for f in fields:
+        if rtol:
+            np.testing.assert_allclose(a[f], b[f], rtol=rtol, atol=0, err_msg=f"{where}.{f}")
+        else:
+            np.testing.assert_array_equal(a[f], b[f], err_msg=f"{where}.{f}")
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_records_match_the_python_builder(driver, tmp_path, name):
+    out = str(tmp_path / "w.bin")
+    subprocess.run([driver, "records", name, out], check=True)
+    c, p = read_records(out), python_scene(name)
+    for arr in ("bodies", "joints", "springs", "manifolds", "tets", "tris"):
+        assert len(getattr(c, arr)) == len(getattr(p, arr)), arr
+    _same(c.scenes, p.scenes, SCENE_DTYPE.names, "scene")
+    _same(c.bodies, p.bodies, BODY_DTYPE.names, "bodies")
+    _same(c.joints, p.joints, JOINT_DTYPE.names, "joints")
+    _same(c.springs, p.springs, SPRING_DTYPE.names, "springs")
+    # rest shapes: both sides invert in double and round once; the inverse
+    # methods differ (cofactors vs LU), so the last float bit may differ
+    _same(c.tets, p.tets, ("v", "serial", "mu", "lambda"), "tets")
+    _same(c.tets, p.tets, ("dm_inv", "rest_volume"), "tets", rtol=1e-6)
+    _same(c.tris, p.tris, ("v", "serial", "mu", "lambda"), "tris")
+    _same(c.tris, p.tris, ("dm_inv", "rest_area"), "tris", rtol=1e-6)
+
+
+def _run_steps(driver, tmp_path, name, steps, lanes):
+    out = str(tmp_path / "s.bin")
+    r = subprocess.run([driver, "step", name, str(steps), str(lanes[0]), str(lanes[1]), out], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    return read_records(out), r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["pyramid", "joint_chain", "spring_bridge", "cloth", "fem_block"])
+def test_step_matches_the_c_abi(driver, tmp_path, name):
+    from skyephyx_amd.capi import Context
+
+    steps, lanes = 12, (1, 1)
+    fac, log = _run_steps(driver, tmp_path, name, steps, lanes)
+    # the facade's initial records through the C ABI directly
+    subprocess.run([driver, "records", name, str(tmp_path / "w.bin")], check=True)
+    built = read_records(str(tmp_path / "w.bin"))
+    with Context(lanes=lanes[0], fem_lanes=lanes[1]) as ctx:
+        ctx.upload(built)
+        for _ in range(steps):
+            ctx.step(1)
+        ref = ctx.download()
+    assert len(fac.bodies) == len(ref.bodies)
+    _same(fac.bodies, ref.bodies, ("pos", "rot", "vel", "ang_vel", "prev_vel", "prev_ang_vel"), "bodies")
+    # contacts the device created, mirrored as Manifold objects
+    assert len(fac.manifolds) == len(ref.manifolds), (len(fac.manifolds), len(ref.manifolds))
+    _same(fac.manifolds, ref.manifolds, ("body_a", "body_b", "serial", "n_contacts", "friction"), "manifolds")
+    for f in ("id", "stick", "lambda", "penalty", "normal", "penetration", "r_a", "r_b"):
+        np.testing.assert_array_equal(fac.manifolds["contacts"][f], ref.manifolds["contacts"][f], err_msg=f)
+    _same(fac.joints, ref.joints, ("body_a", "body_b", "serial", "flags", "lambda", "penalty"), "joints")
+    _same(fac.springs, ref.springs, ("body_a", "body_b", "serial", "flags", "lambda", "penalty"), "springs")
+    assert len(fac.tets) == len(ref.tets) and len(fac.tris) == len(ref.tris)
+
+
+@pytest.mark.gpu
+def test_step_mirrors_bounds_removal(driver, tmp_path):
+    """Bodies leaving the bounds are removed from the host world in the
+    device's order: the resting box stacked last takes a freed index."""
+    from skyephyx_amd.capi import Context
+
+    steps = 3
+    fac, log = _run_steps(driver, tmp_path, "out_of_bounds", steps, (1, 1))
+    subprocess.run([driver, "records", "out_of_bounds", str(tmp_path / "w.bin")], check=True)
+    built = read_records(str(tmp_path / "w.bin"))
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(built)
+        for _ in range(steps):
+            ctx.step(1)
+        ref = ctx.download()
+    n_removed = len(built.bodies) - len(ref.bodies)
+    assert n_removed == 9, n_removed
+    assert len(fac.bodies) == len(ref.bodies)
+    _same(fac.bodies, ref.bodies, ("pos", "rot", "vel", "flags", "mass"), "bodies")
+    assert len(fac.springs) == len(ref.springs) == 0
+    assert len(fac.tets) == len(ref.tets)
+    np.testing.assert_array_equal(np.sort(fac.tets["v"], axis=1), np.sort(ref.tets["v"], axis=1))
+    names = [ln.split()[1] for ln in log.splitlines() if ln.startswith("body ")]
+    assert "fast" not in names and "top" in names
+    # swap with last: "top" (the last body) now sits where the first removed body was
+    assert names.index("top") < len(names) - 1
