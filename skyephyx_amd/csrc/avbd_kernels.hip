@@ -1056,6 +1056,17 @@ AVBD_DEV void elem_pos(const DevWorld& w, const ElemIn& x, ElemPos& p) {
     p.p3 = EENT_TRI(x.e) ? p.p2 : f3(w.pos[x.v.w]);
 }
 
+// The same, with the visiting vertex's own position (in registers, and not
+// written by anyone else during its colour's sweep) in its slot: three
+// gathers per tet instead of four.
+AVBD_DEV void elem_pos_self(const DevWorld& w, const ElemIn& x, v3 self, ElemPos& p) {
+    const int local = EENT_LOCAL(x.e);
+    p.p0 = local == 0 ? self : f3(w.pos[x.v.x]);
+    p.p1 = local == 1 ? self : f3(w.pos[x.v.y]);
+    p.p2 = local == 2 ? self : f3(w.pos[x.v.z]);
+    p.p3 = EENT_TRI(x.e) ? p.p2 : (local == 3 ? self : f3(w.pos[x.v.w]));
+}
+
 AVBD_DEV void elem_terms(const ElemIn& x, const ElemPos& p, int projMode, v3& g, float H[6]) {
     const int local = EENT_LOCAL(x.e);
     if (!EENT_TRI(x.e)) {
@@ -1134,7 +1145,7 @@ AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int p
         while (h0) {
             const bool h1 = k + G < e1;
             ElemPos p0;
-            elem_pos(w, in0, p0);
+            elem_pos_self(w, in0, p, p0);
             ElemIn in1;
             if (h1) elem_load(w, k + G, in1);
             v3 g;
