@@ -165,23 +165,27 @@ def cpu_all_cores(world, ids, weights, seconds=8.0):
     return threads / (t_w / w_sum), threads, sum(len(t) for t in times)
 
 
-def plain_gpu_rate(world, steps=3, lanes=8, fem_lanes=16):
-    """Plain per-scene GPU step: each sampled scene alone in its own context."""
+def plain_gpu_rate(world, steps=3, configs=((8, 16), (4, 4))):
+    """Plain per-scene GPU step: each sampled scene alone in its own context,
+    at the faster of two lane settings (8/16 lanes suit large scenes, 4/4
+    small ones)."""
     from skyephyx_amd.capi import Context
 
     ids, weights = stratified_sample(world, 16, seed=5)
     t_w, total_w = 0.0, 0
     for sid, wgt in zip(ids, weights):
         sub = world.extract_scene(sid)
-        with Context(lanes=lanes, fem_lanes=fem_lanes) as ctx:
-            ctx.upload(sub)
-            ctx.step(1)
-            ctx.synchronize()
-            t0 = time.perf_counter()
-            ctx.step(steps)
-            ctx.synchronize()
-            dt = (time.perf_counter() - t0) / steps
-        t_w += wgt * dt
+        best = math.inf
+        for lanes, fem_lanes in configs:
+            with Context(lanes=lanes, fem_lanes=fem_lanes) as ctx:
+                ctx.upload(sub)
+                ctx.step(2)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                ctx.step(steps)
+                ctx.synchronize()
+                best = min(best, (time.perf_counter() - t0) / steps)
+        t_w += wgt * best
         total_w += wgt
     return 1.0 / (t_w / total_w)
 
@@ -229,6 +233,7 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without per-launch events (step graphs replay; no roofline figures)")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel of a step one by one")
+    ap.add_argument("--fused", action="store_true", help="small scenes in the per-scene kernel (option fused_scenes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson request) measurement")
@@ -251,6 +256,8 @@ def main():
     ctx = Context(lanes=args.lanes, fem_lanes=args.fem_lanes)
     if args.no_graphs:
         ctx.set_option("step_graphs", 0)
+    if args.fused:
+        ctx.set_option("fused_scenes", 1)
     ctx.upload(world)
     log("[bench] uploaded")
     for i in range(args.warmup):
@@ -346,7 +353,7 @@ def main():
         res["roofline"] = None
     if not args.no_plain and S > 1:
         log("[bench] plain per-scene comparison")
-        plain = plain_gpu_rate(state, lanes=args.lanes, fem_lanes=args.fem_lanes)
+        plain = plain_gpu_rate(state)
         res["plain_scene_steps_per_s"] = round(plain, 3)
         res["batched_over_plain"] = round(scene_steps / plain, 3)
     res["device_bytes"] = ctx_bytes

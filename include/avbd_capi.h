@@ -309,7 +309,13 @@ int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
  *                    into a HIP graph the second time their launch shape
  *                    (colours, iterations, lane widths and grids) occurs and
  *                    replayed from then on; 0 = always launched one by one.
- *                    Results are identical either way. */
+ *                    Results are identical either way.
+ *   "fused_scenes"   1 = scenes of at most "fused_max_bodies" bodies (default
+ *                    512) without breakable joints or springs run phases 4-7
+ *                    in one workgroup each (one launch for all of them, on
+ *                    its own stream beside the bucket launches) when "lanes"
+ *                    equals "fem_lanes"; 0 (default) = every scene on the
+ *                    per-colour bucket launches.  Results are identical. */
 int avbd_set_option(avbd_ctx* ctx, const char* name, int64_t value);
 
 /* Autotuner table (no reference counterpart): the lane width chosen per

@@ -67,6 +67,18 @@ struct avbd_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // second stream for the FEM bucket of a colour
     hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t side2 = nullptr;  // third stream: the per-scene kernel of the small scenes
+    hipEvent_t fork2 = nullptr, join2 = nullptr;
+    // Small scenes run phases 4-7 in one workgroup each (k_scene_step):
+    // scenes of at most fused_max bodies with no breakable joint or spring,
+    // when the two lane widths agree (the kernel has one).
+    bool use_fused = false;
+    int fused_max = 512;
+    std::vector<int> scene_bodies;      // per scene, at upload
+    std::vector<char> scene_breakable;  // per scene: a joint or spring with a finite fracture
+    std::vector<char> fused_on;         // per scene: ST_FUSED as last written
+    std::vector<int> fusable_ids;
+    int* fused_list = nullptr;          // device copy of fusable_ids
     std::vector<void*> allocs;
     DevWorld w{};
     ManifoldBuf mb[2]{};
@@ -327,6 +339,56 @@ int launch_colour(avbd_ctx* c, const int* off, int colour, int iter, int poststa
     return 0;
 }
 
+// Lane width of the per-scene kernel: the common width of the two bucket
+// classes (4 when both are tuned), 0 when they differ.
+int fused_lanes(const avbd_ctx* c) {
+    if (c->lanes != c->fem_lanes) return 0;
+    return c->lanes == 0 ? 4 : c->lanes;
+}
+
+// Mark on the device which scenes run in the per-scene kernel (ST_FUSED) and
+// keep the list of those scenes; touches device memory only on a change.
+int sync_fused(avbd_ctx* c) {
+    DevWorld& w = c->w;
+    const int S = w.S;
+    const bool on = c->use_fused && fused_lanes(c) > 0;
+    std::vector<char> want(S, 0);
+    for (int s = 0; s < S; ++s) want[s] = on && !c->scene_breakable[s] && c->scene_bodies[s] <= c->fused_max;
+    if (want == c->fused_on) return 0;
+    std::vector<int> st;
+    int rc;
+    if ((rc = download_vec(c, st, w.st, (size_t)S * ST_WORDS))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->fusable_ids.clear();
+    for (int s = 0; s < S; ++s) {
+        st[s * ST_WORDS + ST_FUSED] = want[s];
+        if (want[s]) c->fusable_ids.push_back(s);
+    }
+    if ((rc = upload_vec(c, w.st, st)) || (rc = upload_vec(c, c->fused_list, c->fusable_ids))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->fused_on = want;
+    c->drop_graphs();
+    return 0;
+}
+
+// The per-scene kernel over the marked scenes (mode 0: phases 4-7, 1:
+// post-stabilisation only); its blocks leave at once for scenes not running.
+void launch_scene_step(avbd_ctx* c, hipStream_t st, int mode) {
+    const int n = (int)c->fusable_ids.size();
+    if (n == 0) return;
+    const DevWorld& w = c->w;
+    const int* list = c->fused_list;
+    switch (fused_lanes(c)) {
+        case 1: hipLaunchKernelGGL((k_scene_step<1>), dim3(n), dim3(256), 0, st, w, list, mode); break;
+        case 2: hipLaunchKernelGGL((k_scene_step<2>), dim3(n), dim3(256), 0, st, w, list, mode); break;
+        case 4: hipLaunchKernelGGL((k_scene_step<4>), dim3(n), dim3(256), 0, st, w, list, mode); break;
+        case 8: hipLaunchKernelGGL((k_scene_step<8>), dim3(n), dim3(256), 0, st, w, list, mode); break;
+        case 16: hipLaunchKernelGGL((k_scene_step<16>), dim3(n), dim3(256), 0, st, w, list, mode); break;
+        case 32: hipLaunchKernelGGL((k_scene_step<32>), dim3(n), dim3(256), 0, st, w, list, mode); break;
+        default: break;
+    }
+}
+
 // Sizes of every device array of a world.
 struct WorldDims {
     int S = 0, N = 0, J = 0, SP = 0, T = 0, TR = 0, Mcap = 0;
@@ -363,6 +425,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
 #define A(p, n) if ((rc = c->alloc(p, n))) return rc
     A(d_sc, S);
     A(&w.st, (size_t)S * ST_WORDS);
+    A(&c->fused_list, S);
     A(d_off, S + 1);
     w.sc = *d_sc;
     w.sc_off = *d_off;
@@ -580,6 +643,16 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     c->scenes.assign(in->scenes, in->scenes + S);
     c->sc_off = off;
     c->body_dead.assign(N, 0);
+    c->scene_bodies.assign(S, 0);
+    c->scene_breakable.assign(S, 0);
+    for (int s = 0; s < S; ++s) c->scene_bodies[s] = off[s + 1] - off[s];
+    for (int i = 0; i < J; ++i)
+        for (int r = 0; r < 6; ++r)
+            if (std::isfinite(in->joints[i].fracture[r])) c->scene_breakable[in->bodies[in->joints[i].body_b].scene] = 1;
+    for (int i = 0; i < SP; ++i)
+        if (std::isfinite(in->springs[i].fracture)) c->scene_breakable[in->bodies[in->springs[i].body_b].scene] = 1;
+    c->fused_on.assign(S, 0);
+    c->fusable_ids.clear();
     std::vector<int> st((size_t)S * ST_WORDS, 0);
     for (int s = 0; s < S; ++s) {
         st[s * ST_WORDS + ST_EMERGENCY] = in->scenes[s].emergency_stop;
@@ -767,6 +840,7 @@ int poststab_scenes(avbd_ctx* c, const std::vector<char>& only) {
         for (int col = 0; col < g[G_NCOLOURS]; ++col)
             if ((rc = launch_colour(c, c->bucket_off.data(), col, -1, 1))) return rc;
         hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, c->stream, w);
+        launch_scene_step(c, c->stream, 1);
     }
     // restore the active set
     if ((rc = download_vec(c, st, w.st, (size_t)S * ST_WORDS))) return rc;
@@ -999,6 +1073,13 @@ int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_
     DevWorld& w = c->w;
     hipStream_t st = c->stream;
     const int N = w.N;
+    // the small scenes' whole phases 4-7 on their own stream, beside the buckets
+    const bool fused = !c->fusable_ids.empty();
+    if (fused) {
+        HIP_TRY(hipEventRecord(c->fork2, st));
+        HIP_TRY(hipStreamWaitEvent(c->side2, c->fork2, 0));
+        launch_scene_step(c, c->side2, 0);
+    }
     hipLaunchKernelGGL(k_predict, dim3(grid_for(N)), dim3(256), 0, st, w);
     if (marks) HIP_TRY(hipEventRecord(c->ev[3], st));
     for (int it = 0; it < max_iters; ++it) {
@@ -1018,6 +1099,10 @@ int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_
         for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, -1, 1);
         hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, st, w);
     }
+    if (fused) {
+        HIP_TRY(hipEventRecord(c->join2, c->side2));
+        HIP_TRY(hipStreamWaitEvent(st, c->join2, 0));
+    }
     return 0;
 }
 
@@ -1026,7 +1111,7 @@ int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_
 // grid).  False while a bucket's lane width is still being tuned.
 bool step_graph_key(const avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_post, int dgrid,
                     std::vector<int>* key) {
-    key->assign({c->cur, ncol, max_iters, any_post ? 1 : 0, dgrid});
+    key->assign({c->cur, ncol, max_iters, any_post ? 1 : 0, dgrid, fused_lanes(c), (int)c->fusable_ids.size()});
     for (int k = 0; k < 2 * ncol; ++k) {
         const int n = off[k + 1] - off[k];
         if (n == 0) {
@@ -1050,11 +1135,12 @@ int do_step(avbd_ctx* c) {
         max_iters = std::max(max_iters, s.iterations);
         any_post = any_post || s.post_stabilization != 0;
     }
+    int rc = sync_fused(c);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev[0], st));
     // ---- 1. broad phase
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_NEW_PAIRS, 0);
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_CANDIDATES, 0);
-    int rc = 0;
     if (c->n_items > 0) {
         hipLaunchKernelGGL(k_tile_bounds, dim3(c->n_tiles), dim3(256), 0, st, w);
         hipLaunchKernelGGL(k_broad, dim3(c->n_items), dim3(256), 0, st, w, (const int4*)c->items);
@@ -1253,6 +1339,9 @@ int avbd_create(int32_t device, avbd_ctx** out) {
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork2, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join2, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&c->host_word, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         avbd_destroy(c);
         return fail(AVBD_E_HIP, "stream/event/pinned buffer creation failed");
@@ -1269,9 +1358,13 @@ int avbd_destroy(avbd_ctx* c) {
         if (e) hipEventDestroy(e);
     for (auto& e : c->kev) hipEventDestroy(e);
     if (c->side) hipStreamSynchronize(c->side);
+    if (c->side2) hipStreamSynchronize(c->side2);
     if (c->fork) hipEventDestroy(c->fork);
     if (c->join) hipEventDestroy(c->join);
+    if (c->fork2) hipEventDestroy(c->fork2);
+    if (c->join2) hipEventDestroy(c->join2);
     if (c->side) hipStreamDestroy(c->side);
+    if (c->side2) hipStreamDestroy(c->side2);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->host_word) hipHostFree(c->host_word);
     delete c;
@@ -1289,6 +1382,15 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
     }
     if (n == "kernel_timing") {
         c->ktiming = value != 0;
+        return 0;
+    }
+    if (n == "fused_scenes") {
+        c->use_fused = value != 0;
+        return 0;
+    }
+    if (n == "fused_max_bodies") {
+        if (value < 0 || value > (1 << 20)) return fail(AVBD_E_ARG, "fused_max_bodies out of range");
+        c->fused_max = int(value);
         return 0;
     }
     if (n == "step_graphs") {

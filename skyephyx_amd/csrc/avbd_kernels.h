@@ -24,6 +24,8 @@ __global__ void k_colour_fill(DevWorld w);
 __global__ void k_predict(DevWorld w);
 template <int G, bool FEM>
 __global__ void k_primal(DevWorld w, int bucket, int iter, int poststab);
+template <int G>
+__global__ void k_scene_step(DevWorld w, const int* scenes, int mode);
 __global__ void k_dual(DevWorld w, int iter);
 __global__ void k_fracture_reset(DevWorld w);
 __global__ void k_velocity(DevWorld w);

@@ -28,6 +28,9 @@ AVBD_DEV bool scene_runs(const DevWorld& w, int s) {
     return st[ST_ACTIVE] != 0 && st[ST_EMERGENCY] == 0;
 }
 AVBD_DEV bool is_static(const DevWorld& w, int b) { return (w.flags[b] & AVBD_BODY_STATIC) != 0; }
+// The scene's phases 4-7 run in one workgroup of k_scene_step instead of the
+// per-colour bucket launches.
+AVBD_DEV bool scene_fused(const DevWorld& w, int s) { return w.st[s * ST_WORDS + ST_FUSED] != 0; }
 
 __global__ void k_fill_int(int* p, int n, int v) {
     GRID_STRIDE(i, n) p[i] = v;
@@ -654,7 +657,7 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
 // Sweep bucket of body b (2 * colour + 1 for bodies whose only terms are FEM
 // energies, 2 * colour otherwise), or -1 if it takes no part in the sweeps.
 AVBD_DEV int sweep_bucket(const DevWorld& w, int b) {
-    if (b >= w.N || is_static(w, b) || !scene_runs(w, w.scene[b])) return -1;
+    if (b >= w.N || is_static(w, b) || !scene_runs(w, w.scene[b]) || scene_fused(w, w.scene[b])) return -1;
     int c = w.colour[b];
     if (c < 0 || c >= AVBD_MAX_COLOURS) return -1;
     const float4 I = w.inertia[b];
@@ -721,10 +724,8 @@ __global__ __launch_bounds__(256) void k_colour_fill(DevWorld w) {
 
 // ------------------------------------------------------------------ phase 4
 // Body warm start (solver.cpp:324-384), all bodies of running scenes.
-__global__ void k_predict(DevWorld w) {
-    GRID_STRIDE(b, w.N) {
-        const int s = w.scene[b];
-        if (!scene_runs(w, s)) continue;
+AVBD_DEV void predict_body(const DevWorld& w, int b, int s) {
+    {
         const avbd_scene& sc = w.sc[s];
         const float dt = sc.dt;
         const v3 g{sc.gravity[0], sc.gravity[1], sc.gravity[2]};
@@ -778,6 +779,14 @@ __global__ void k_predict(DevWorld w) {
         v3 da = rotation_difference(nr, rq);
         w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
         w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
+    }
+}
+
+__global__ void k_predict(DevWorld w) {
+    GRID_STRIDE(b, w.N) {
+        const int s = w.scene[b];
+        if (!scene_runs(w, s) || scene_fused(w, s)) continue;
+        predict_body(w, b, s);
     }
 }
 
@@ -1113,69 +1122,160 @@ AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int k, int projMode, v
 // accumulated (9 values through the butterfly instead of 27) and solved with
 // the 3x3 pivoted LDL^T, which performs exactly the operations the 6x6 one
 // performs on that block; the rotation increment is exactly zero.
+// One FEM-only particle: its G lanes (a group within one wave) share the
+// particle's energy list.
+template <int G>
+AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int iter, int poststab) {
+    const int lane = threadIdx.x % G;
+    const int s = w.scene[b];
+    const avbd_scene& sc = w.sc[s];
+    if (w.st[s * ST_WORDS + ST_EMERGENCY]) return;
+    if (poststab ? (!sc.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) : iter >= sc.iterations) return;
+    const float dt2 = sc.dt * sc.dt;
+    float L[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float r[3] = {0.f, 0.f, 0.f};
+    const v3 p = f3(w.pos[b]);
+    if (lane == 0) {
+        const float ml = w.vel[b].w / dt2;
+        const v3 dl = sub(p, f3(w.inert_pos[b]));
+        L[0] = ml; L[2] = ml; L[5] = ml;
+        r[0] = ml * dl.x; r[1] = ml * dl.y; r[2] = ml * dl.z;
+    }
+    // Entries k = e0 + lane, + G, ... in order; the stream record of the
+    // next entry is loaded while the current one is evaluated.
+    const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
+    int k = e0 + lane;
+    ElemIn in0;
+    bool h0 = k < e1;
+    if (h0) elem_load(w, k, in0);
+    while (h0) {
+        const bool h1 = k + G < e1;
+        ElemPos p0;
+        elem_pos_self(w, in0, p, p0);
+        ElemIn in1;
+        if (h1) elem_load(w, k + G, in1);
+        v3 g;
+        float H[6];
+        elem_terms(in0, p0, sc.projection_mode, g, H);
+        r[0] = r[0] + g.x;
+        r[1] = r[1] + g.y;
+        r[2] = r[2] + g.z;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) L[i] = L[i] + H[i];
+        in0 = in1;
+        h0 = h1;
+        k += G;
+    }
+    if (G > 1) {
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) L[i] = L[i] + __shfl_xor(L[i], off, G);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) r[i] = r[i] + __shfl_xor(r[i], off, G);
+        }
+    }
+    if (lane == 0) {
+        float dx[3];
+        ldlt3_solve(L, r, dx);
+        const qt q = fq(w.rot[b]);
+        const qt lr = fq(w.last_rot[b]);
+        v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
+        qt dq = quaternion_from_difference(v3{0.f, 0.f, 0.f}, -1.f);
+        qt nq = qnormalized(qmul(dq, q));
+        w.pos[b] = v4(np, 0.f);
+        w.rot[b] = q4(nq);
+        v3 dl = sub(np, f3(w.last_pos[b]));
+        v3 da = rotation_difference(nq, lr);
+        w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
+        w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
+    }
+}
+
 template <int G>
 AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int poststab) {
-    const int lane = threadIdx.x % G;
     const int gpb = blockDim.x / G;
     for (int base = blockIdx.x * gpb; base < hi - lo; base += gridDim.x * gpb) {
         const int item = base + threadIdx.x / G;
         if (item >= hi - lo) continue;  // lane groups never straddle a wave: the group leaves together
-        const int b = w.col_list[lo + item];
-        const int s = w.scene[b];
+        primal_fem_vertex<G>(w, w.col_list[lo + item], iter, poststab);
+    }
+}
+
+// One body with forces (and possibly energies): its G lanes (a group within
+// one wave) share the body's incidence list.  valid is false for the padding
+// groups of a pass; they take part in nothing.
+template <int G>
+AVBD_DEV void primal_rigid(const DevWorld& w, int b, bool valid, int iter, int poststab) {
+    const int lane = threadIdx.x % G;
+    bool run = valid;
+    int s = 0;
+    if (run) {
+        s = w.scene[b];
+        const avbd_scene& sc0 = w.sc[s];
+        if (w.st[s * ST_WORDS + ST_EMERGENCY]) run = false;
+        // a scene with bodies out of bounds post-stabilises after their removal
+        if (poststab) { if (!sc0.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) run = false; }
+        else if (iter >= sc0.iterations) run = false;
+    }
+    // all lanes of a group agree on `run`; groups never straddle a wave
+    if (run) {
         const avbd_scene& sc = w.sc[s];
-        if (w.st[s * ST_WORDS + ST_EMERGENCY]) continue;
-        if (poststab ? (!sc.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) : iter >= sc.iterations) continue;
-        const float dt2 = sc.dt * sc.dt;
-        float L[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        float r[3] = {0.f, 0.f, 0.f};
+        const float alpha = poststab ? 0.f : sc.alpha;
+        const float dt = sc.dt;
+        const float dt2 = dt * dt;
+        float dSelf[6];
+        load_diff(w, b, dSelf);
+        Acc A;
+        acc_zero(A);
+        const qt lr = fq(w.last_rot[b]);
+        const float4 vm = w.vel[b];
+        const v3 I = f3(w.inertia[b]);
         const v3 p = f3(w.pos[b]);
+        const qt q = fq(w.rot[b]);
         if (lane == 0) {
-            const float ml = w.vel[b].w / dt2;
-            const v3 dl = sub(p, f3(w.inert_pos[b]));
-            L[0] = ml; L[2] = ml; L[5] = ml;
-            r[0] = ml * dl.x; r[1] = ml * dl.y; r[2] = ml * dl.z;
-        }
-        // Entries k = e0 + lane, + G, ... in order; the stream record of the
-        // next entry is loaded while the current one is evaluated.
-        const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
-        int k = e0 + lane;
-        ElemIn in0;
-        bool h0 = k < e1;
-        if (h0) elem_load(w, k, in0);
-        while (h0) {
-            const bool h1 = k + G < e1;
-            ElemPos p0;
-            elem_pos_self(w, in0, p, p0);
-            ElemIn in1;
-            if (h1) elem_load(w, k + G, in1);
-            v3 g;
-            float H[6];
-            elem_terms(in0, p0, sc.projection_mode, g, H);
-            r[0] = r[0] + g.x;
-            r[1] = r[1] + g.y;
-            r[2] = r[2] + g.z;
+            // lhs = M / dt^2 ; rhs = lhs * (x - y, rotdiff(q, q_y))
+            const m3 R = rotmat(lr);
+            float Iw[3][3];
 #pragma unroll
-            for (int i = 0; i < 6; ++i) L[i] = L[i] + H[i];
-            in0 = in1;
-            h0 = h1;
-            k += G;
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    Iw[i][j] = ((R.m[i][0] * I.x) * R.m[j][0] + (R.m[i][1] * I.y) * R.m[j][1]) +
+                               (R.m[i][2] * I.z) * R.m[j][2];
+            const float ml = vm.w / dt2;
+            v3 dl = sub(p, f3(w.inert_pos[b]));
+            v3 da = rotation_difference(q, fq(w.inert_rot[b]));
+            A.L[tri_idx(0, 0)] = ml; A.L[tri_idx(1, 1)] = ml; A.L[tri_idx(2, 2)] = ml;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j <= i; ++j) A.L[tri_idx(3 + i, 3 + j)] = Iw[i][j] / dt2;
+            A.r[0] = ml * dl.x; A.r[1] = ml * dl.y; A.r[2] = ml * dl.z;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                A.r[3 + i] = ((Iw[i][0] / dt2) * da.x + (Iw[i][1] / dt2) * da.y) + (Iw[i][2] / dt2) * da.z;
         }
+        {
+            const int f0 = w.f_off[b], f1 = w.f_off[b + 1];
+            for (int k = f0 + lane; k < f1; k += G) accumulate_force(w, w.f_ent[k], alpha, dSelf, A);
+        }
+        const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
+        for (int k = e0 + lane; k < e1; k += G) accumulate_energy(w, k, sc.projection_mode, A);
         if (G > 1) {
 #pragma unroll
             for (int off = 1; off < G; off <<= 1) {
 #pragma unroll
-                for (int i = 0; i < 6; ++i) L[i] = L[i] + __shfl_xor(L[i], off, G);
+                for (int i = 0; i < 21; ++i) A.L[i] = A.L[i] + __shfl_xor(A.L[i], off, G);
 #pragma unroll
-                for (int i = 0; i < 3; ++i) r[i] = r[i] + __shfl_xor(r[i], off, G);
+                for (int i = 0; i < 6; ++i) A.r[i] = A.r[i] + __shfl_xor(A.r[i], off, G);
             }
         }
+        float dx[6];
+        ldlt6_solve(A.L, A.r, dx);
         if (lane == 0) {
-            float dx[3];
-            ldlt3_solve(L, r, dx);
-            const qt q = fq(w.rot[b]);
-            const qt lr = fq(w.last_rot[b]);
             v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
-            qt dq = quaternion_from_difference(v3{0.f, 0.f, 0.f}, -1.f);
+            qt dq = quaternion_from_difference(v3{dx[3], dx[4], dx[5]}, -1.f);
             qt nq = qnormalized(qmul(dq, q));
             w.pos[b] = v4(np, 0.f);
             w.rot[b] = q4(nq);
@@ -1187,9 +1287,6 @@ AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int p
     }
 }
 
-// One sweep bucket (bodies col_list[col_off[bucket] .. col_off[bucket+1])),
-// read from device memory so that a launch does not depend on this step's
-// bucket sizes (a captured step graph replays with the sizes of its step).
 template <int G, bool FEM>
 __global__ __launch_bounds__(256) void k_primal(DevWorld w, int bucket, int iter, int poststab) {
     const int lo = w.col_off[bucket], hi = w.col_off[bucket + 1];
@@ -1197,89 +1294,11 @@ __global__ __launch_bounds__(256) void k_primal(DevWorld w, int bucket, int iter
         primal_fem_body<G>(w, lo, hi, iter, poststab);
         return;
     }
-    const int lane = threadIdx.x % G;
     const int groups_per_block = blockDim.x / G;
     for (int base = blockIdx.x * groups_per_block; base < hi - lo; base += gridDim.x * groups_per_block) {
         const int item = base + threadIdx.x / G;
         const bool valid = item < hi - lo;
-        int b = valid ? w.col_list[lo + item] : 0;
-        bool run = valid;
-        int s = 0;
-        if (run) {
-            s = w.scene[b];
-            const avbd_scene& sc0 = w.sc[s];
-            if (w.st[s * ST_WORDS + ST_EMERGENCY]) run = false;
-            // a scene with bodies out of bounds post-stabilises after their removal
-            if (poststab) { if (!sc0.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) run = false; }
-            else if (iter >= sc0.iterations) run = false;
-        }
-        // all lanes of a group agree on `run`; groups never straddle a wave
-        if (run) {
-            const avbd_scene& sc = w.sc[s];
-            const float alpha = poststab ? 0.f : sc.alpha;
-            const float dt = sc.dt;
-            const float dt2 = dt * dt;
-            float dSelf[6];
-            load_diff(w, b, dSelf);
-            Acc A;
-            acc_zero(A);
-            const qt lr = fq(w.last_rot[b]);
-            const float4 vm = w.vel[b];
-            const v3 I = f3(w.inertia[b]);
-            const v3 p = f3(w.pos[b]);
-            const qt q = fq(w.rot[b]);
-            if (lane == 0) {
-                // lhs = M / dt^2 ; rhs = lhs * (x - y, rotdiff(q, q_y))
-                const m3 R = rotmat(lr);
-                float Iw[3][3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j)
-                        Iw[i][j] = ((R.m[i][0] * I.x) * R.m[j][0] + (R.m[i][1] * I.y) * R.m[j][1]) +
-                                   (R.m[i][2] * I.z) * R.m[j][2];
-                const float ml = vm.w / dt2;
-                v3 dl = sub(p, f3(w.inert_pos[b]));
-                v3 da = rotation_difference(q, fq(w.inert_rot[b]));
-                A.L[tri_idx(0, 0)] = ml; A.L[tri_idx(1, 1)] = ml; A.L[tri_idx(2, 2)] = ml;
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j <= i; ++j) A.L[tri_idx(3 + i, 3 + j)] = Iw[i][j] / dt2;
-                A.r[0] = ml * dl.x; A.r[1] = ml * dl.y; A.r[2] = ml * dl.z;
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    A.r[3 + i] = ((Iw[i][0] / dt2) * da.x + (Iw[i][1] / dt2) * da.y) + (Iw[i][2] / dt2) * da.z;
-            }
-            if (!FEM) {
-                const int f0 = w.f_off[b], f1 = w.f_off[b + 1];
-                for (int k = f0 + lane; k < f1; k += G) accumulate_force(w, w.f_ent[k], alpha, dSelf, A);
-            }
-            const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
-            for (int k = e0 + lane; k < e1; k += G) accumulate_energy(w, k, sc.projection_mode, A);
-            if (G > 1) {
-#pragma unroll
-                for (int off = 1; off < G; off <<= 1) {
-#pragma unroll
-                    for (int i = 0; i < 21; ++i) A.L[i] = A.L[i] + __shfl_xor(A.L[i], off, G);
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) A.r[i] = A.r[i] + __shfl_xor(A.r[i], off, G);
-                }
-            }
-            float dx[6];
-            ldlt6_solve(A.L, A.r, dx);
-            if (lane == 0) {
-                v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
-                qt dq = quaternion_from_difference(v3{dx[3], dx[4], dx[5]}, -1.f);
-                qt nq = qnormalized(qmul(dq, q));
-                w.pos[b] = v4(np, 0.f);
-                w.rot[b] = q4(nq);
-                v3 dl = sub(np, f3(w.last_pos[b]));
-                v3 da = rotation_difference(nq, lr);
-                w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
-                w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
-            }
-        }
+        primal_rigid<G>(w, valid ? w.col_list[lo + item] : 0, valid, iter, poststab);
     }
 }
 
@@ -1297,101 +1316,116 @@ AVBD_PRIMAL_INST(32)
 // or spring.  A fracture disables the force and queues its bodies so that
 // k_fracture_reset clears their manifolds (order-independent: a cleared
 // manifold stays at lambda = k = 0 for the rest of the sweep).
+AVBD_DEV void dual_contact(const DevWorld& w, int m, int c, int iter) {
+    if (c >= w.m.n[m]) return;
+    const int a = w.m.a[m], b = w.m.b[m];
+    const int s = w.scene[a];
+    if (!scene_runs(w, s)) return;
+    const avbd_scene& sc = w.sc[s];
+    if (iter >= sc.iterations) return;
+    float dA[6], dB[6];
+    load_diff(w, a, dA);
+    load_diff(w, b, dB);
+    ContactRows R;
+    contact_rows(w, m, c, sc.alpha, dA, dB, R);
+    float tl = sqrtf(R.lam[1] * R.lam[1] + R.lam[2] * R.lam[2]);
+    const float4 h1 = w.m.hot[((size_t)m * 8 + c) * CONTACT_HOT_F4 + 1];
+    const float4 h2 = w.m.hot[((size_t)m * 8 + c) * CONTACT_HOT_F4 + 2];
+    float tc = sqrtf(h1.w * h1.w + h2.w * h2.w);
+    int stick = (tl < R.bound) && (tc < 0.01f);
+    float lam[3], k[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        float lo = r == 0 ? -INFINITY : -R.bound;
+        float hi = r == 0 ? 0.f : R.bound;
+        float l = R.lam[r] + R.k[r] * R.C[r];
+        if (l < lo) l = lo;
+        else if (l > hi) l = hi;
+        float kk = R.k[r];
+        if (l > lo && l < hi) kk = fminf(kk + sc.beta * fabsf(R.C[r]), 1000000000.0f);
+        lam[r] = l;
+        k[r] = kk;
+    }
+    float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
+    h[3].w = lam[0];
+    h[4].w = lam[1];
+    h[5] = make_float4(lam[2], k[0], k[1], k[2]);
+    w.m.cold[((size_t)m * 8 + c) * CONTACT_COLD_F4].y = (float)stick;
+}
+
+AVBD_DEV void dual_joint(const DevWorld& w, int j, int iter) {
+    if (w.j_flags[j] & FORCE_DEAD) return;
+    const int s = w.scene[w.j_b[j]];
+    if (!scene_runs(w, s)) return;
+    const avbd_scene& sc = w.sc[s];
+    if (iter >= sc.iterations) return;
+    JointRows R;
+    joint_rows(w, j, true, sc.alpha, R);
+    float* rows = w.j_row + j * 36;
+    for (int r = 0; r < 6; ++r) {
+        float* row = rows + r * 6;
+        const float stiff = row[0], frac = row[1], fmn = row[2], fmx = row[3];
+        float l = (isinf(stiff) ? row[4] : 0.f) + row[5] * R.C[r];
+        if (l < fmn) l = fmn;
+        else if (l > fmx) l = fmx;
+        row[4] = l;
+        if (fabsf(l) >= frac) {
+            for (int q = 0; q < 6; ++q) { rows[q * 6 + 4] = 0.f; rows[q * 6 + 5] = 0.f; }
+            w.j_flags[j] |= AVBD_FORCE_DISABLED;
+            int slot = atomicAdd(&w.g[G_FRACTURES], 2);
+            if (slot + 1 < 2 * (w.J + w.SP) + 2) {
+                w.fracture_bodies[slot] = w.j_a[j];
+                w.fracture_bodies[slot + 1] = w.j_b[j];
+            }
+            break;
+        }
+        if (l > fmn && l < fmx) row[5] = fminf(row[5] + sc.beta * fabsf(R.C[r]), fminf(stiff, 1000000000.0f));
+    }
+}
+
+AVBD_DEV void dual_spring(const DevWorld& w, int sp, int iter) {
+    if (w.s_flags[sp] & FORCE_DEAD) return;
+    const int s = w.scene[w.s_b[sp]];
+    if (!scene_runs(w, s)) return;
+    const avbd_scene& sc = w.sc[s];
+    if (iter >= sc.iterations) return;
+    float* row = w.s_row + sp * 6;
+    const float stiff = row[0], frac = row[1], fmn = row[2], fmx = row[3];
+    float C = spring_C(w, sp);
+    float l = (isinf(stiff) ? row[4] : 0.f) + row[5] * C;
+    if (l < fmn) l = fmn;
+    else if (l > fmx) l = fmx;
+    row[4] = l;
+    if (fabsf(l) >= frac) {
+        row[4] = 0.f;
+        row[5] = 0.f;
+        w.s_flags[sp] |= AVBD_FORCE_DISABLED;
+        int slot = atomicAdd(&w.g[G_FRACTURES], 2);
+        if (slot + 1 < 2 * (w.J + w.SP) + 2) {
+            w.fracture_bodies[slot] = w.s_a[sp];
+            w.fracture_bodies[slot + 1] = w.s_b[sp];
+        }
+        return;
+    }
+    if (l > fmn && l < fmx) row[5] = fminf(row[5] + sc.beta * fabsf(C), fminf(stiff, 1000000000.0f));
+}
+
 __global__ void k_dual(DevWorld w, int iter) {
     const int M = w.g[G_MANIFOLDS];
     const int total = M * 8 + w.J + w.SP;
     GRID_STRIDE(t, total) {
         if (t < M * 8) {
-            const int m = t >> 3, c = t & 7;
-            if (c >= w.m.n[m]) continue;
-            const int a = w.m.a[m], b = w.m.b[m];
-            const int s = w.scene[a];
-            if (!scene_runs(w, s)) continue;
-            const avbd_scene& sc = w.sc[s];
-            if (iter >= sc.iterations) continue;
-            float dA[6], dB[6];
-            load_diff(w, a, dA);
-            load_diff(w, b, dB);
-            ContactRows R;
-            contact_rows(w, m, c, sc.alpha, dA, dB, R);
-            float tl = sqrtf(R.lam[1] * R.lam[1] + R.lam[2] * R.lam[2]);
-            const float4 h1 = w.m.hot[((size_t)m * 8 + c) * CONTACT_HOT_F4 + 1];
-            const float4 h2 = w.m.hot[((size_t)m * 8 + c) * CONTACT_HOT_F4 + 2];
-            float tc = sqrtf(h1.w * h1.w + h2.w * h2.w);
-            int stick = (tl < R.bound) && (tc < 0.01f);
-            float lam[3], k[3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                float lo = r == 0 ? -INFINITY : -R.bound;
-                float hi = r == 0 ? 0.f : R.bound;
-                float l = R.lam[r] + R.k[r] * R.C[r];
-                if (l < lo) l = lo;
-                else if (l > hi) l = hi;
-                float kk = R.k[r];
-                if (l > lo && l < hi) kk = fminf(kk + sc.beta * fabsf(R.C[r]), 1000000000.0f);
-                lam[r] = l;
-                k[r] = kk;
-            }
-            float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
-            h[3].w = lam[0];
-            h[4].w = lam[1];
-            h[5] = make_float4(lam[2], k[0], k[1], k[2]);
-            w.m.cold[((size_t)m * 8 + c) * CONTACT_COLD_F4].y = (float)stick;
+            const int m = t >> 3;
+            if (scene_fused(w, w.scene[w.m.a[m]])) continue;
+            dual_contact(w, m, t & 7, iter);
         } else if (t < M * 8 + w.J) {
             const int j = t - M * 8;
-            if (w.j_flags[j] & FORCE_DEAD) continue;
-            const int s = w.scene[w.j_b[j]];
-            if (!scene_runs(w, s)) continue;
-            const avbd_scene& sc = w.sc[s];
-            if (iter >= sc.iterations) continue;
-            JointRows R;
-            joint_rows(w, j, true, sc.alpha, R);
-            float* rows = w.j_row + j * 36;
-            for (int r = 0; r < 6; ++r) {
-                float* row = rows + r * 6;
-                const float stiff = row[0], frac = row[1], fmn = row[2], fmx = row[3];
-                float l = (isinf(stiff) ? row[4] : 0.f) + row[5] * R.C[r];
-                if (l < fmn) l = fmn;
-                else if (l > fmx) l = fmx;
-                row[4] = l;
-                if (fabsf(l) >= frac) {
-                    for (int q = 0; q < 6; ++q) { rows[q * 6 + 4] = 0.f; rows[q * 6 + 5] = 0.f; }
-                    w.j_flags[j] |= AVBD_FORCE_DISABLED;
-                    int slot = atomicAdd(&w.g[G_FRACTURES], 2);
-                    if (slot + 1 < 2 * (w.J + w.SP) + 2) {
-                        w.fracture_bodies[slot] = w.j_a[j];
-                        w.fracture_bodies[slot + 1] = w.j_b[j];
-                    }
-                    break;
-                }
-                if (l > fmn && l < fmx) row[5] = fminf(row[5] + sc.beta * fabsf(R.C[r]), fminf(stiff, 1000000000.0f));
-            }
+            if (scene_fused(w, w.scene[w.j_b[j]])) continue;
+            dual_joint(w, j, iter);
         } else {
             const int sp = t - M * 8 - w.J;
-            if (w.s_flags[sp] & FORCE_DEAD) continue;
-            const int s = w.scene[w.s_b[sp]];
-            if (!scene_runs(w, s)) continue;
-            const avbd_scene& sc = w.sc[s];
-            if (iter >= sc.iterations) continue;
-            float* row = w.s_row + sp * 6;
-            const float stiff = row[0], frac = row[1], fmn = row[2], fmx = row[3];
-            float C = spring_C(w, sp);
-            float l = (isinf(stiff) ? row[4] : 0.f) + row[5] * C;
-            if (l < fmn) l = fmn;
-            else if (l > fmx) l = fmx;
-            row[4] = l;
-            if (fabsf(l) >= frac) {
-                row[4] = 0.f;
-                row[5] = 0.f;
-                w.s_flags[sp] |= AVBD_FORCE_DISABLED;
-                int slot = atomicAdd(&w.g[G_FRACTURES], 2);
-                if (slot + 1 < 2 * (w.J + w.SP) + 2) {
-                    w.fracture_bodies[slot] = w.s_a[sp];
-                    w.fracture_bodies[slot + 1] = w.s_b[sp];
-                }
-                continue;
-            }
-            if (l > fmn && l < fmx) row[5] = fminf(row[5] + sc.beta * fabsf(C), fminf(stiff, 1000000000.0f));
+            if (scene_fused(w, w.scene[w.s_b[sp]])) continue;
+            dual_spring(w, sp, iter);
         }
     }
 }
@@ -1422,10 +1456,8 @@ __global__ void k_fracture_reset(DevWorld w) {
 // ------------------------------------------------------------------ phase 6
 // Velocities from the displacement of the step, bounds and explosion checks
 // (solver.cpp:493-530, CheckExplosion 182-196).
-__global__ void k_velocity(DevWorld w) {
-    GRID_STRIDE(b, w.N) {
-        const int s = w.scene[b];
-        if (!scene_runs(w, s)) continue;
+AVBD_DEV void velocity_body(const DevWorld& w, int b, int s) {
+    {
         const avbd_scene& sc = w.sc[s];
         const uint32_t fl = w.flags[b];
         float4 v4v = w.vel[b];
@@ -1453,27 +1485,126 @@ __global__ void k_velocity(DevWorld w) {
     }
 }
 
+__global__ void k_velocity(DevWorld w) {
+    GRID_STRIDE(b, w.N) {
+        const int s = w.scene[b];
+        if (!scene_runs(w, s) || scene_fused(w, s)) continue;
+        velocity_body(w, b, s);
+    }
+}
+
 // ------------------------------------------------------------------ phase 7
 // Stick flags as left by the last ComputeConstraints of the step: with
 // post-stabilisation that is the post-stab sweep, using the final lambda.
+AVBD_DEV void stick_contact(const DevWorld& w, int m, int c) {
+    const int a = w.m.a[m], b = w.m.b[m];
+    if (is_static(w, a) && is_static(w, b)) return;
+    const float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
+    const float4 h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4], h5 = h[5];
+    float bound = fabsf(h3.w) * w.m.fric[m];
+    float tl = sqrtf(h4.w * h4.w + h5.x * h5.x);
+    float tc = sqrtf(h1.w * h1.w + h2.w * h2.w);
+    w.m.cold[((size_t)m * 8 + c) * CONTACT_COLD_F4].y = (float)((tl < bound) && (tc < 0.01f));
+}
+
 __global__ void k_stick(DevWorld w) {
     const int M = w.g[G_MANIFOLDS];
     GRID_STRIDE(t, M * 8) {
         const int m = t >> 3, c = t & 7;
         if (c >= w.m.n[m]) continue;
-        const int a = w.m.a[m], b = w.m.b[m];
-        const int s = w.scene[a];
-        if (!scene_runs(w, s)) continue;
+        const int s = w.scene[w.m.a[m]];
+        if (!scene_runs(w, s) || scene_fused(w, s)) continue;
         if (!w.sc[s].post_stabilization || w.st[s * ST_WORDS + ST_OOB]) continue;
-        if (is_static(w, a) && is_static(w, b)) continue;
-        const float4* h = w.m.hot + ((size_t)m * 8 + c) * CONTACT_HOT_F4;
-        const float4 h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4], h5 = h[5];
-        float bound = fabsf(h3.w) * w.m.fric[m];
-        float tl = sqrtf(h4.w * h4.w + h5.x * h5.x);
-        float tc = sqrtf(h1.w * h1.w + h2.w * h2.w);
-        w.m.cold[((size_t)m * 8 + c) * CONTACT_COLD_F4].y = (float)((tl < bound) && (tc < 0.01f));
+        stick_contact(w, m, c);
     }
 }
+
+// ------------------------------------------------ per-scene step (small scenes)
+// Sweep kind of body b in colour col for the per-scene kernel: 0 = block
+// update with forces, 1 = FEM-only particle, -1 = not swept (the bucket
+// rule of sweep_bucket without the scene tests).
+AVBD_DEV int sweep_kind(const DevWorld& w, int b, int col) {
+    if (is_static(w, b) || w.colour[b] != col) return -1;
+    const float4 I = w.inertia[b];
+    const bool fem_only = w.f_off[b + 1] == w.f_off[b] && w.e_off[b + 1] > w.e_off[b] && I.x == 0.f && I.y == 0.f &&
+                          I.z == 0.f;
+    return fem_only ? 1 : 0;
+}
+
+// Phases 4-7 of one small scene in one workgroup: prediction, the
+// iterations (per colour the block updates of the scene's bodies, then the
+// dual of its forces), velocities with the bounds and explosion checks, and
+// post-stabilisation with the stick flags.  The per-body and per-force
+// functions are those of the bucket launches and run in the same order
+// (colour-major; bodies of one colour are independent), with workgroup
+// barriers in place of launch boundaries, so the result is bit-identical.
+// Each force is visited once: a contact point from its body A's incidence
+// entry, a joint or spring from its body B's.  Eligible scenes have no breakable joint or spring (a
+// fracture resets other bodies' contacts through the global queue).
+// mode 1: post-stabilisation only (after the bounds removal).
+template <int G>
+__global__ __launch_bounds__(256) void k_scene_step(DevWorld w, const int* scenes, int mode) {
+    const int s = scenes[blockIdx.x];
+    if (!scene_runs(w, s) || !scene_fused(w, s)) return;
+    const avbd_scene& sc = w.sc[s];
+    const int a = w.sc_off[s], e = w.sc_off[s + 1];
+    const int ncol = w.st[s * ST_WORDS + ST_NCOL];
+    const int gpb = blockDim.x / G;
+    auto sweep = [&](int iter, int poststab) {
+        for (int col = 0; col < ncol; ++col) {
+            for (int base = a; base < e; base += gpb) {
+                const int b = base + (int)threadIdx.x / G;  // one lane group per body
+                const int k = b < e ? sweep_kind(w, b, col) : -1;
+                if (k == 1) primal_fem_vertex<G>(w, b, iter, poststab);
+                else if (k == 0) primal_rigid<G>(w, b, true, iter, poststab);
+            }
+            __syncthreads();
+        }
+    };
+    // the scene's incidence entries are one contiguous range (bodies of a
+    // scene are contiguous): one thread per entry
+    const int q0 = w.f_off[a], q1 = w.f_off[e];
+    auto each_force = [&](auto f) {
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+            const uint32_t ent = w.f_ent[q];
+            const bool contact = ENT_KIND(ent) == ENT_CONTACT;
+            if (contact == (ENT_SIDE_B(ent) != 0)) continue;
+            f(ENT_KIND(ent), ENT_INDEX(ent));
+        }
+    };
+    if (mode == 0) {
+        for (int b = a + (int)threadIdx.x; b < e; b += blockDim.x) predict_body(w, b, s);
+        __syncthreads();
+        for (int it = 0; it < sc.iterations; ++it) {
+            sweep(it, 0);
+            each_force([&](uint32_t kind, uint32_t idx) {
+                if (kind == ENT_CONTACT) dual_contact(w, (int)(idx >> 3), (int)(idx & 7u), it);
+                else if (kind == ENT_JOINT) dual_joint(w, (int)idx, it);
+                else dual_spring(w, (int)idx, it);
+            });
+            __syncthreads();
+        }
+        for (int b = a + (int)threadIdx.x; b < e; b += blockDim.x) velocity_body(w, b, s);
+        __syncthreads();
+    }
+    // post-stabilisation unless the bounds or explosion check fired (flags
+    // set by other threads with atomics: read at device scope)
+    const int* st = w.st + s * ST_WORDS;
+    if (!sc.post_stabilization || __atomic_load_n(&st[ST_OOB], __ATOMIC_RELAXED) != 0 ||
+        __atomic_load_n(&st[ST_EMERGENCY], __ATOMIC_RELAXED) != 0)
+        return;
+    sweep(-1, 1);
+    each_force([&](uint32_t kind, uint32_t idx) {
+        if (kind == ENT_CONTACT) stick_contact(w, (int)(idx >> 3), (int)(idx & 7u));
+    });
+}
+
+template __global__ void k_scene_step<1>(DevWorld, const int*, int);
+template __global__ void k_scene_step<2>(DevWorld, const int*, int);
+template __global__ void k_scene_step<4>(DevWorld, const int*, int);
+template __global__ void k_scene_step<8>(DevWorld, const int*, int);
+template __global__ void k_scene_step<16>(DevWorld, const int*, int);
+template __global__ void k_scene_step<32>(DevWorld, const int*, int);
 
 // ------------------------------------------------ out-of-bounds removal
 // Solver::RemoveBody for the bodies that left the world bounds

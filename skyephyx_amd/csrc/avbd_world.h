@@ -31,6 +31,7 @@
 #define ST_ACTIVE 3
 #define ST_DIRTY 4   // topology changed -> recolour
 #define ST_NCOL 5    // colours of the scene (last colouring)
+#define ST_FUSED 6   // phases 4-7 run in the per-scene kernel (k_scene_step)
 #define ST_WORDS 8
 
 // global error/status words
