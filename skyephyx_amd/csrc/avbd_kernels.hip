@@ -1065,17 +1065,6 @@ AVBD_DEV void elem_pos(const DevWorld& w, const ElemIn& x, ElemPos& p) {
     p.p3 = EENT_TRI(x.e) ? p.p2 : f3(w.pos[x.v.w]);
 }
 
-// The same, with the visiting vertex's own position (in registers, and not
-// written by anyone else during its colour's sweep) in its slot: three
-// gathers per tet instead of four.
-AVBD_DEV void elem_pos_self(const DevWorld& w, const ElemIn& x, v3 self, ElemPos& p) {
-    const int local = EENT_LOCAL(x.e);
-    p.p0 = local == 0 ? self : f3(w.pos[x.v.x]);
-    p.p1 = local == 1 ? self : f3(w.pos[x.v.y]);
-    p.p2 = local == 2 ? self : f3(w.pos[x.v.z]);
-    p.p3 = EENT_TRI(x.e) ? p.p2 : (local == 3 ? self : f3(w.pos[x.v.w]));
-}
-
 AVBD_DEV void elem_terms(const ElemIn& x, const ElemPos& p, int projMode, v3& g, float H[6]) {
     const int local = EENT_LOCAL(x.e);
     if (!EENT_TRI(x.e)) {
@@ -1125,8 +1114,7 @@ AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int k, int projMode, v
 // One FEM-only particle: its G lanes (a group within one wave) share the
 // particle's energy list.
 template <int G>
-AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int iter, int poststab) {
-    const int lane = threadIdx.x % G;
+AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, int poststab) {
     const int s = w.scene[b];
     const avbd_scene& sc = w.sc[s];
     if (w.st[s * ST_WORDS + ST_EMERGENCY]) return;
@@ -1151,7 +1139,7 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int iter, int poststab
     while (h0) {
         const bool h1 = k + G < e1;
         ElemPos p0;
-        elem_pos_self(w, in0, p, p0);
+        elem_pos(w, in0, p0);
         ElemIn in1;
         if (h1) elem_load(w, k + G, in1);
         v3 g;
@@ -1194,11 +1182,12 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int iter, int poststab
 
 template <int G>
 AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int poststab) {
+    const int lane = threadIdx.x % G;
     const int gpb = blockDim.x / G;
     for (int base = blockIdx.x * gpb; base < hi - lo; base += gridDim.x * gpb) {
         const int item = base + threadIdx.x / G;
         if (item >= hi - lo) continue;  // lane groups never straddle a wave: the group leaves together
-        primal_fem_vertex<G>(w, w.col_list[lo + item], iter, poststab);
+        primal_fem_vertex<G>(w, w.col_list[lo + item], lane, iter, poststab);
     }
 }
 
@@ -1555,7 +1544,7 @@ __global__ __launch_bounds__(256) void k_scene_step(DevWorld w, const int* scene
             for (int base = a; base < e; base += gpb) {
                 const int b = base + (int)threadIdx.x / G;  // one lane group per body
                 const int k = b < e ? sweep_kind(w, b, col) : -1;
-                if (k == 1) primal_fem_vertex<G>(w, b, iter, poststab);
+                if (k == 1) primal_fem_vertex<G>(w, b, (int)threadIdx.x % G, iter, poststab);
                 else if (k == 0) primal_rigid<G>(w, b, true, iter, poststab);
             }
             __syncthreads();
