@@ -1065,6 +1065,15 @@ AVBD_DEV void elem_pos(const DevWorld& w, const ElemIn& x, ElemPos& p) {
     p.p3 = EENT_TRI(x.e) ? p.p2 : f3(w.pos[x.v.w]);
 }
 
+// All four vertex positions of an element, unconditionally (a triangle's
+// slot 3 holds id 0 and is not used).
+AVBD_DEV void elem_gather(const DevWorld& w, int4 v, ElemPos& p) {
+    p.p0 = f3(w.pos[v.x]);
+    p.p1 = f3(w.pos[v.y]);
+    p.p2 = f3(w.pos[v.z]);
+    p.p3 = f3(w.pos[v.w]);
+}
+
 AVBD_DEV void elem_terms(const ElemIn& x, const ElemPos& p, int projMode, v3& g, float H[6]) {
     const int local = EENT_LOCAL(x.e);
     if (!EENT_TRI(x.e)) {
@@ -1129,30 +1138,48 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
         L[0] = ml; L[2] = ml; L[5] = ml;
         r[0] = ml * dl.x; r[1] = ml * dl.y; r[2] = ml * dl.z;
     }
-    // Entries k = e0 + lane, + G, ... in order; the stream record of the
-    // next entry is loaded while the current one is evaluated.
+    // Entries k = e0 + lane, + G, ... in order, software-pipelined: while
+    // element k is evaluated, the next element's stream record and vertex
+    // positions and the one after's vertex ids are in flight.  Loads are
+    // unconditional (indices clamped to the list; a triangle's slot 3 id is
+    // 0) so that none sits under a branch, which would make every wait
+    // drain all loads in flight.
     const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
     int k = e0 + lane;
-    ElemIn in0;
-    bool h0 = k < e1;
-    if (h0) elem_load(w, k, in0);
-    while (h0) {
-        const bool h1 = k + G < e1;
+    if (k < e1) {
+        const int last = e1 - 1;
+        ElemIn in0;
+        elem_load(w, k, in0);
+        int4 ids1 = w.ev_v[min(k + G, last)];
         ElemPos p0;
-        elem_pos(w, in0, p0);
-        ElemIn in1;
-        if (h1) elem_load(w, k + G, in1);
-        v3 g;
-        float H[6];
-        elem_terms(in0, p0, sc.projection_mode, g, H);
-        r[0] = r[0] + g.x;
-        r[1] = r[1] + g.y;
-        r[2] = r[2] + g.z;
+        elem_gather(w, in0.v, p0);
+        bool h0 = true;
+        while (h0) {
+            const bool h1 = k + G < e1;
+            const int k1 = min(k + G, last), k2 = min(k + 2 * G, last);
+            ElemIn in1;
+            in1.e = w.e_ent[k1];
+            in1.a = w.ev_a[k1];
+            in1.b = w.ev_b[k1];
+            in1.c = w.ev_c[k1];
+            in1.v = ids1;
+            const int4 ids2 = w.ev_v[k2];
+            ElemPos p1;
+            elem_gather(w, ids1, p1);
+            v3 g;
+            float H[6];
+            elem_terms(in0, p0, sc.projection_mode, g, H);
+            r[0] = r[0] + g.x;
+            r[1] = r[1] + g.y;
+            r[2] = r[2] + g.z;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) L[i] = L[i] + H[i];
-        in0 = in1;
-        h0 = h1;
-        k += G;
+            for (int i = 0; i < 6; ++i) L[i] = L[i] + H[i];
+            in0 = in1;
+            p0 = p1;
+            ids1 = ids2;
+            h0 = h1;
+            k += G;
+        }
     }
     if (G > 1) {
 #pragma unroll
