@@ -75,6 +75,7 @@ struct avbd_ctx {
     bool use_fused = false;
     int fused_max = 512;
     std::vector<int> scene_bodies;      // per scene, at upload
+    int max_scene_bodies = 0;
     std::vector<char> scene_breakable;  // per scene: a joint or spring with a finite fracture
     std::vector<char> fused_on;         // per scene: ST_FUSED as last written
     std::vector<int> fusable_ids;
@@ -646,6 +647,7 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     c->scene_bodies.assign(S, 0);
     c->scene_breakable.assign(S, 0);
     for (int s = 0; s < S; ++s) c->scene_bodies[s] = off[s + 1] - off[s];
+    c->max_scene_bodies = S > 0 ? *std::max_element(c->scene_bodies.begin(), c->scene_bodies.end()) : 0;
     for (int i = 0; i < J; ++i)
         for (int r = 0; r < 6; ++r)
             if (std::isfinite(in->joints[i].fracture[r])) c->scene_breakable[in->bodies[in->joints[i].body_b].scene] = 1;
@@ -804,7 +806,10 @@ int colour_and_buckets(avbd_ctx* c, int* g) {
     DevWorld& w = c->w;
     hipStream_t st = c->stream;
     const int N = w.N;
-    hipLaunchKernelGGL(k_colour, dim3(w.S), dim3(1024), 0, st, w);
+    // scenes up to COLOUR_LDS_MAX bodies with LDS words, larger ones without
+    const size_t lds = sizeof(unsigned int) * (size_t)std::min(c->max_scene_bodies, COLOUR_LDS_MAX);
+    hipLaunchKernelGGL(k_colour<true>, dim3(w.S), dim3(1024), lds, st, w);
+    if (c->max_scene_bodies > COLOUR_LDS_MAX) hipLaunchKernelGGL(k_colour<false>, dim3(w.S), dim3(1024), 0, st, w);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, w.col_cnt, AVBD_BUCKETS + 1, 0);
     hipLaunchKernelGGL(k_colour_hist, dim3(grid_for(N)), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_colour_scan, dim3(1), dim3(256), 0, st, w);
@@ -1345,6 +1350,12 @@ int avbd_create(int32_t device, avbd_ctx** out) {
         hipHostMalloc((void**)&c->host_word, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         avbd_destroy(c);
         return fail(AVBD_E_HIP, "stream/event/pinned buffer creation failed");
+    }
+    // the LDS colouring takes up to 128 KiB of dynamic LDS per workgroup
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_colour<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(unsigned int) * COLOUR_LDS_MAX)) != hipSuccess) {
+        avbd_destroy(c);
+        return fail(AVBD_E_HIP, "cannot reserve LDS for the colouring kernel");
     }
     *out = c;
     return 0;

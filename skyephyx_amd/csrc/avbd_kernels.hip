@@ -567,77 +567,147 @@ AVBD_DEV void for_each_neighbour(const DevWorld& w, int v, F f) {
     }
 }
 
+// Neighbours of v reached through its incidence entries part, part + parts,
+// ... (the share of one lane of a group of `parts` lanes).
+template <class F>
+AVBD_DEV void for_each_neighbour_part(const DevWorld& w, int v, int part, int parts, F f) {
+    for (int k = w.f_off[v] + part; k < w.f_off[v + 1]; k += parts) {
+        const int o = ent_other(w, w.f_ent[k]);
+        if (o >= 0) f(o);
+    }
+    for (int k = w.e_off[v] + part; k < w.e_off[v + 1]; k += parts) {
+        const uint32_t e = w.e_ent[k];
+        const int local = EENT_LOCAL(e);
+        const int4 t = w.ev_v[k];
+        if (local != 0) f(t.x);
+        if (local != 1) f(t.y);
+        if (local != 2) f(t.z);
+        if (!EENT_TRI(e) && local != 3) f(t.w);
+    }
+}
+
 // Greedy first-fit colouring of one scene per workgroup, bit-identical to the
 // sequential rule "dynamic bodies in ascending index take the smallest colour
 // not used by lower-index dynamic neighbours".  A body is coloured as soon as
 // all its lower-index neighbours are (a counter per body), so the result does
-// not depend on the order threads run in.
+// not depend on the order threads run in.  Rounds follow the longest chain of
+// ascending neighbours (~180 for a 32^3 tet grid), and each round reads every
+// neighbour's colour, so those reads decide the cost: scenes of up to
+// COLOUR_LDS_MAX bodies keep one word per body in LDS, (colour << 16) |
+// pending lower neighbours (colour 0xFFFF = not yet, 0xFFFE = static); larger
+// scenes keep colours and counters in global memory.  Each instantiation
+// takes only the scenes of its kind.
+template <bool LDS>
 __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
+    extern __shared__ unsigned int cw[];
+    constexpr int CG = 8;  // lanes sharing one body's incidence list
     const int s = blockIdx.x;
     if (!scene_runs(w, s)) return;
     int* st = w.st + s * ST_WORDS;
     if (!st[ST_DIRTY]) return;
     const int off = w.sc_off[s], end = w.sc_off[s + 1];
     const int n = end - off;
+    if (LDS != (n <= COLOUR_LDS_MAX)) return;
     int* qa = w.queue + 2 * off;
     int* qb = qa + n;
     __shared__ int qn[2];
     __shared__ int maxc;
-    if (threadIdx.x == 0) { qn[0] = 0; qn[1] = 0; maxc = -1; }
+    __shared__ int overflow;
+    if (threadIdx.x == 0) { qn[0] = 0; qn[1] = 0; maxc = -1; overflow = 0; }
+    const unsigned UNCOL = 0xFFFFu, STAT = 0xFFFEu;
+    if (LDS) {
+        // static flags first: neighbour tests then read LDS only
+        for (int v = off + threadIdx.x; v < end; v += blockDim.x) cw[v - off] = (is_static(w, v) ? STAT : UNCOL) << 16;
+    } else {
+        for (int v = off + threadIdx.x; v < end; v += blockDim.x) w.colour[v] = -1;
+    }
     __syncthreads();
-    for (int v = off + threadIdx.x; v < end; v += blockDim.x) {
-        w.colour[v] = -1;
-        if (is_static(w, v)) continue;
+    auto nb_static = [&](int o) -> bool { return LDS ? (cw[o - off] >> 16) == STAT : is_static(w, o); };
+    auto nb_colour = [&](int o) -> int {
+        if (LDS) {
+            const unsigned c = cw[o - off] >> 16;
+            return c >= STAT ? -1 : (int)c;
+        }
+        return __atomic_load_n(&w.colour[o], __ATOMIC_RELAXED);
+    };
+    const int part = threadIdx.x % CG;
+    const int groups = blockDim.x / CG;
+    // lower-neighbour counts; bodies with none start the first round
+    for (int base = off; base < end; base += groups) {
+        const int v = base + threadIdx.x / CG;
+        const bool dyn = v < end && !nb_static(v);
         int c = 0;
-        for_each_neighbour(w, v, [&](int o) {
-            if (o < v && !is_static(w, o)) ++c;
-        });
-        w.ccount[v] = c;
-        if (c == 0) qa[atomicAdd(&qn[0], 1)] = v;
+        if (dyn)
+            for_each_neighbour_part(w, v, part, CG, [&](int o) {
+                if (o < v && !nb_static(o)) ++c;
+            });
+#pragma unroll
+        for (int m = 1; m < CG; m <<= 1) c += __shfl_xor(c, m, CG);
+        if (dyn && part == 0) {
+            if (LDS) {
+                if (c > 0xFFFF) overflow = 1;
+                cw[v - off] = (UNCOL << 16) | (unsigned)min(c, 0xFFFF);
+            } else {
+                w.ccount[v] = c;
+            }
+            if (c == 0) qa[atomicAdd(&qn[0], 1)] = v;
+        }
     }
     __syncthreads();
     int cur = 0;
-    while (true) {
+    while (!overflow) {
         const int len = qn[cur];
         if (len == 0) break;
         int* q = cur ? qb : qa;
         int* nq = cur ? qa : qb;
-        for (int k = threadIdx.x; k < len; k += blockDim.x) {
-            const int v = q[k];
+        for (int kb = 0; kb < len; kb += groups) {
+            const int k = kb + threadIdx.x / CG;
+            const bool valid = k < len;
+            const int v = valid ? q[k] : 0;
             unsigned long long used = 0ull;
-            bool high = false;
-            for_each_neighbour(w, v, [&](int o) {
-                if (o < v && !is_static(w, o)) {
-                    int c = __atomic_load_n(&w.colour[o], __ATOMIC_RELAXED);
-                    if (c >= 0 && c < 64) used |= (1ull << c);
-                    else if (c >= 64) high = true;
-                }
-            });
+            if (valid)
+                for_each_neighbour_part(w, v, part, CG, [&](int o) {
+                    if (o < v && !nb_static(o)) {
+                        const int c = nb_colour(o);
+                        if (c >= 0 && c < 64) used |= (1ull << c);
+                    }
+                });
+#pragma unroll
+            for (int m = 1; m < CG; m <<= 1) used |= __shfl_xor(used, m, CG);
             int c = __ffsll((long long)~used) - 1;
             if (used == ~0ull) {
                 // more than 64 colours: linear search above 64
                 c = 64;
                 while (true) {
-                    bool taken = false;
-                    for_each_neighbour(w, v, [&](int o) {
-                        if (o < v && !is_static(w, o) && __atomic_load_n(&w.colour[o], __ATOMIC_RELAXED) == c) taken = true;
-                    });
+                    int taken = 0;
+                    if (valid)
+                        for_each_neighbour_part(w, v, part, CG, [&](int o) {
+                            if (o < v && !nb_static(o) && nb_colour(o) == c) taken = 1;
+                        });
+#pragma unroll
+                    for (int m = 1; m < CG; m <<= 1) taken |= __shfl_xor(taken, m, CG);
                     if (!taken) break;
                     ++c;
                 }
             }
-            (void)high;
-            __atomic_store_n(&w.colour[v], c, __ATOMIC_RELAXED);
-            atomicMax(&maxc, c);
+            if (valid && part == 0) {
+                if (LDS) cw[v - off] = (unsigned)min(c, 0xFFFD) << 16;  // its counter is 0
+                else __atomic_store_n(&w.colour[v], c, __ATOMIC_RELAXED);
+                atomicMax(&maxc, c);
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) qn[cur ^ 1] = 0;
         __syncthreads();
-        for (int k = threadIdx.x; k < len; k += blockDim.x) {
+        for (int kb = 0; kb < len; kb += groups) {
+            const int k = kb + threadIdx.x / CG;
+            if (k >= len) continue;
             const int u = q[k];
-            for_each_neighbour(w, u, [&](int o) {
-                if (o > u && !is_static(w, o)) {
-                    if (atomicSub(&w.ccount[o], 1) == 1) nq[atomicAdd(&qn[cur ^ 1], 1)] = o;
+            for_each_neighbour_part(w, u, part, CG, [&](int o) {
+                if (o > u && !nb_static(o)) {
+                    const bool ready = LDS ? (atomicSub(&cw[o - off], 1u) & 0xFFFFu) == 1u
+                                           : atomicSub(&w.ccount[o], 1) == 1;
+                    if (ready) nq[atomicAdd(&qn[cur ^ 1], 1)] = o;
                 }
             });
         }
@@ -646,12 +716,21 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
         cur ^= 1;
         __syncthreads();
     }
+    if (LDS) {
+        for (int v = off + threadIdx.x; v < end; v += blockDim.x) {
+            const unsigned c = cw[v - off] >> 16;
+            w.colour[v] = c >= STAT ? -1 : (int)c;
+        }
+    }
     if (threadIdx.x == 0) {
         st[ST_NCOL] = maxc + 1;
         st[ST_DIRTY] = 0;
-        if (maxc + 1 > AVBD_MAX_COLOURS) w.g[G_ERR_COLOURS] = 1;
+        if (maxc + 1 > AVBD_MAX_COLOURS || overflow) w.g[G_ERR_COLOURS] = 1;
     }
 }
+
+template __global__ void k_colour<true>(DevWorld);
+template __global__ void k_colour<false>(DevWorld);
 
 // Colour of body b if it takes part in this step's sweeps, else -1.
 // Sweep bucket of body b (2 * colour + 1 for bodies whose only terms are FEM

@@ -34,6 +34,10 @@
 #define ST_FUSED 6   // phases 4-7 run in the per-scene kernel (k_scene_step)
 #define ST_WORDS 8
 
+// scenes of up to this many bodies are coloured with per-body words in LDS
+// (4 bytes each); larger ones in global memory
+#define COLOUR_LDS_MAX 32768
+
 // global error/status words
 #define G_ERR_MANIFOLD_CAP 0
 #define G_ERR_PAIR_CAP 1
