@@ -586,6 +586,70 @@ AVBD_DEV void for_each_neighbour_part(const DevWorld& w, int v, int part, int pa
     }
 }
 
+// The neighbours one lane of a group reaches (for_each_neighbour_part), held
+// in registers so that a body's two scans in a colouring round load them
+// once, with all loads issued before any is used.  Lanes whose share exceeds
+// NB_CACHE entries of a kind reach the rest through the plain loops.
+#define NB_CACHE 4
+struct NbCache {
+    int nf, ne;
+    int fo[NB_CACHE];
+    uint32_t ee[NB_CACHE];
+    int4 ev[NB_CACHE];
+};
+
+AVBD_DEV void nb_load(const DevWorld& w, int v, int part, int parts, NbCache& c) {
+    const int f0 = w.f_off[v] + part, f1 = w.f_off[v + 1];
+    const int e0 = w.e_off[v] + part, e1 = w.e_off[v + 1];
+    c.nf = f1 > f0 ? min((f1 - f0 + parts - 1) / parts, NB_CACHE) : 0;
+    c.ne = e1 > e0 ? min((e1 - e0 + parts - 1) / parts, NB_CACHE) : 0;
+    uint32_t fe[NB_CACHE];
+#pragma unroll
+    for (int j = 0; j < NB_CACHE; ++j) {
+        if (j < c.nf) fe[j] = w.f_ent[f0 + j * parts];
+        if (j < c.ne) {
+            c.ee[j] = w.e_ent[e0 + j * parts];
+            c.ev[j] = w.ev_v[e0 + j * parts];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB_CACHE; ++j)
+        if (j < c.nf) c.fo[j] = ent_other(w, fe[j]);
+}
+
+template <class F>
+AVBD_DEV void nb_apply(const DevWorld& w, int v, int part, int parts, const NbCache& c, F f) {
+#pragma unroll
+    for (int j = 0; j < NB_CACHE; ++j)
+        if (j < c.nf && c.fo[j] >= 0) f(c.fo[j]);
+#pragma unroll
+    for (int j = 0; j < NB_CACHE; ++j) {
+        if (j < c.ne) {
+            const uint32_t e = c.ee[j];
+            const int local = EENT_LOCAL(e);
+            const int4 t = c.ev[j];
+            if (local != 0) f(t.x);
+            if (local != 1) f(t.y);
+            if (local != 2) f(t.z);
+            if (!EENT_TRI(e) && local != 3) f(t.w);
+        }
+    }
+    // entries beyond the cache
+    for (int k = w.f_off[v] + part + NB_CACHE * parts; k < w.f_off[v + 1]; k += parts) {
+        const int o = ent_other(w, w.f_ent[k]);
+        if (o >= 0) f(o);
+    }
+    for (int k = w.e_off[v] + part + NB_CACHE * parts; k < w.e_off[v + 1]; k += parts) {
+        const uint32_t e = w.e_ent[k];
+        const int local = EENT_LOCAL(e);
+        const int4 t = w.ev_v[k];
+        if (local != 0) f(t.x);
+        if (local != 1) f(t.y);
+        if (local != 2) f(t.z);
+        if (!EENT_TRI(e) && local != 3) f(t.w);
+    }
+}
+
 // Greedy first-fit colouring of one scene per workgroup, bit-identical to the
 // sequential rule "dynamic bodies in ascending index take the smallest colour
 // not used by lower-index dynamic neighbours".  A body is coloured as soon as
@@ -654,6 +718,12 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
         }
     }
     __syncthreads();
+    // Rounds: every body of the current queue takes its colour, then at once
+    // releases its higher neighbours (their counters); those that reach zero
+    // form the next queue.  No barrier is needed between the two steps of one
+    // body: a body of the queue never reads the colour of another body of the
+    // same queue (they are not neighbours, or one would still be waiting for
+    // the other), and a released body is only coloured in a later round.
     int cur = 0;
     while (!overflow) {
         const int len = qn[cur];
@@ -664,9 +734,12 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
             const int k = kb + threadIdx.x / CG;
             const bool valid = k < len;
             const int v = valid ? q[k] : 0;
+            NbCache nc;
+            nc.nf = nc.ne = 0;
+            if (valid) nb_load(w, v, part, CG, nc);
             unsigned long long used = 0ull;
             if (valid)
-                for_each_neighbour_part(w, v, part, CG, [&](int o) {
+                nb_apply(w, v, part, CG, nc, [&](int o) {
                     if (o < v && !nb_static(o)) {
                         const int c = nb_colour(o);
                         if (c >= 0 && c < 64) used |= (1ull << c);
@@ -692,24 +765,21 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
             }
             if (valid && part == 0) {
                 if (LDS) cw[v - off] = (unsigned)min(c, 0xFFFD) << 16;  // its counter is 0
-                else __atomic_store_n(&w.colour[v], c, __ATOMIC_RELAXED);
+                else __atomic_store_n(&w.colour[v], c, __ATOMIC_RELEASE);
                 atomicMax(&maxc, c);
             }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) qn[cur ^ 1] = 0;
-        __syncthreads();
-        for (int kb = 0; kb < len; kb += groups) {
-            const int k = kb + threadIdx.x / CG;
-            if (k >= len) continue;
-            const int u = q[k];
-            for_each_neighbour_part(w, u, part, CG, [&](int o) {
-                if (o > u && !nb_static(o)) {
-                    const bool ready = LDS ? (atomicSub(&cw[o - off], 1u) & 0xFFFFu) == 1u
-                                           : atomicSub(&w.ccount[o], 1) == 1;
-                    if (ready) nq[atomicAdd(&qn[cur ^ 1], 1)] = o;
-                }
-            });
+            // the group's colour write precedes every lane's releases
+            if (LDS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __builtin_amdgcn_wave_barrier();
+            if (valid)
+                nb_apply(w, v, part, CG, nc, [&](int o) {
+                    if (o > v && !nb_static(o)) {
+                        const bool ready = LDS ? (atomicSub(&cw[o - off], 1u) & 0xFFFFu) == 1u
+                                               : atomicSub(&w.ccount[o], 1) == 1;
+                        if (ready) nq[atomicAdd(&qn[cur ^ 1], 1)] = o;
+                    }
+                });
         }
         __syncthreads();
         if (threadIdx.x == 0) qn[cur] = 0;
