@@ -234,7 +234,6 @@ __global__ void k_pair_append(DevWorld w, const unsigned long long* keys) {
             w.m.serial[m] = base + p;
             w.m.n[m] = 0;
             w.m.fric[m] = 0.f;
-            w.st[w.scene[a] * ST_WORDS + ST_DIRTY] = 1;
         }
     }
 }
@@ -256,8 +255,14 @@ __global__ void k_pair_finish(DevWorld w) {
 // ------------------------------------------------------------------ phase 2
 // Manifold::Initialize (manifold.cpp:32-194) + the force warm start of
 // solver.cpp:269-292, one thread per manifold.
+// The scene's colouring is redone only when its constraint graph changes: a
+// manifold created this step (index >= the count before the broad phase)
+// that keeps contacts, or an older one that loses them.  A new pair whose
+// boxes do not touch is created and dropped in the same step and leaves the
+// graph as it was.
 __global__ void k_manifold_init(DevWorld w) {
     const int M = w.g[G_MANIFOLDS];
+    const int M_old = w.g[G_MANIFOLDS_OLD];
     GRID_STRIDE(m, w.Mcap) {
         if (m >= M) {
             w.alive[m] = 0;
@@ -312,10 +317,11 @@ __global__ void k_manifold_init(DevWorld w) {
         w.m.fric[m] = fric;
         if (n == 0) {
             w.alive[m] = 0;
-            w.st[s * ST_WORDS + ST_DIRTY] = 1;
+            if (m < M_old) w.st[s * ST_WORDS + ST_DIRTY] = 1;
             continue;
         }
         w.alive[m] = 1;
+        if (m >= M_old) w.st[s * ST_WORDS + ST_DIRTY] = 1;
         const m3 RA = rotmat(qA), RB = rotmat(qB);
         const bool decay = sc.post_stabilization == 0;
         for (int i = 0; i < n; ++i) {
