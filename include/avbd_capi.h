@@ -315,7 +315,11 @@ int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
  *                    in one workgroup each (one launch for all of them, on
  *                    its own stream beside the bucket launches) when "lanes"
  *                    equals "fem_lanes"; 0 (default) = every scene on the
- *                    per-colour bucket launches.  Results are identical. */
+ *                    per-colour bucket launches.  Results are identical.
+ *   "colour_updates" 1 (default) = a scene whose constraint graph gained or
+ *                    lost edges since its last colouring re-evaluates only
+ *                    the bodies the change can reach; 0 = every recolouring
+ *                    from scratch.  The colouring is the same either way. */
 int avbd_set_option(avbd_ctx* ctx, const char* name, int64_t value);
 
 /* Autotuner table (no reference counterpart): the lane width chosen per

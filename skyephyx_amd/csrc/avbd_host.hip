@@ -74,6 +74,9 @@ struct avbd_ctx {
     // when the two lane widths agree (the kernel has one).
     bool use_fused = false;
     int fused_max = 512;
+    // a scene whose constraint graph changed a little updates its last
+    // colouring from the changed bodies instead of colouring from scratch
+    bool colour_updates = true;
     std::vector<int> scene_bodies;      // per scene, at upload
     int max_scene_bodies = 0;
     std::vector<char> scene_breakable;  // per scene: a joint or spring with a finite fracture
@@ -453,7 +456,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     A(&w.ev_a, (size_t)4 * T + 3 * TR + 1); A(&w.ev_b, (size_t)4 * T + 3 * TR + 1);
     A(&w.ev_c, (size_t)4 * T + 3 * TR + 1); A(&w.ev_v, (size_t)4 * T + 3 * TR + 1);
     A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
-    A(&w.ccount, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1);
+    A(&w.ccount, N); A(&w.cseed, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1);
     w.pair_cap = (int)d.pair_cap;
     A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
     int* toff;
@@ -660,7 +663,7 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
         st[s * ST_WORDS + ST_EMERGENCY] = in->scenes[s].emergency_stop;
         st[s * ST_WORDS + ST_REMOVED] = in->scenes[s].n_removed;
         st[s * ST_WORDS + ST_ACTIVE] = 1;
-        st[s * ST_WORDS + ST_DIRTY] = 1;
+        st[s * ST_WORDS + ST_DIRTY] = DIRTY_FULL;
     }
     if ((rc = upload_vec(c, d_sc, c->scenes))) return rc;
     if ((rc = upload_vec(c, w.st, st))) return rc;
@@ -808,8 +811,10 @@ int colour_and_buckets(avbd_ctx* c, int* g) {
     const int N = w.N;
     // scenes up to COLOUR_LDS_MAX bodies with LDS words, larger ones without
     const size_t lds = sizeof(unsigned int) * (size_t)std::min(c->max_scene_bodies, COLOUR_LDS_MAX);
-    hipLaunchKernelGGL(k_colour<true>, dim3(w.S), dim3(1024), lds, st, w);
-    if (c->max_scene_bodies > COLOUR_LDS_MAX) hipLaunchKernelGGL(k_colour<false>, dim3(w.S), dim3(1024), 0, st, w);
+    const int full = c->colour_updates ? 0 : 1;
+    hipLaunchKernelGGL(k_colour<true>, dim3(w.S), dim3(1024), lds, st, w, full);
+    if (c->max_scene_bodies > COLOUR_LDS_MAX)
+        hipLaunchKernelGGL(k_colour<false>, dim3(w.S), dim3(1024), 0, st, w, full);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, w.col_cnt, AVBD_BUCKETS + 1, 0);
     hipLaunchKernelGGL(k_colour_hist, dim3(grid_for(N)), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_colour_scan, dim3(1), dim3(256), 0, st, w);
@@ -932,7 +937,7 @@ int remove_out_of_bounds(avbd_ctx* c) {
             }
         }
         stw[s * ST_WORDS + ST_REMOVED] += (int)gone.size();
-        stw[s * ST_WORDS + ST_DIRTY] = 1;
+        stw[s * ST_WORDS + ST_DIRTY] = DIRTY_FULL;
         segs.push_back({a, e, (int)order.size()});
         int k = a;
         for (int o : order) {
@@ -1402,6 +1407,10 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
     if (n == "fused_max_bodies") {
         if (value < 0 || value > (1 << 20)) return fail(AVBD_E_ARG, "fused_max_bodies out of range");
         c->fused_max = int(value);
+        return 0;
+    }
+    if (n == "colour_updates") {
+        c->colour_updates = value != 0;
         return 0;
     }
     if (n == "step_graphs") {

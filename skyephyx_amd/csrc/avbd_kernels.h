@@ -18,7 +18,7 @@ __global__ void k_inc_count(DevWorld w);
 __global__ void k_inc_fill(DevWorld w);
 __global__ void k_inc_sort(DevWorld w);
 template <bool LDS>
-__global__ void k_colour(DevWorld w);
+__global__ void k_colour(DevWorld w, int full);
 __global__ void k_colour_hist(DevWorld w);
 __global__ void k_colour_scan(DevWorld w);
 __global__ void k_colour_fill(DevWorld w);

@@ -255,8 +255,18 @@ __global__ void k_pair_finish(DevWorld w) {
 // ------------------------------------------------------------------ phase 2
 // Manifold::Initialize (manifold.cpp:32-194) + the force warm start of
 // solver.cpp:269-292, one thread per manifold.
-// The scene's colouring is redone only when its constraint graph changes: a
-// manifold created this step (index >= the count before the broad phase)
+// An edge (a, b) of the scene's constraint graph appeared or disappeared:
+// the higher-index body's set of lower neighbours changed, so its colour may
+// change (and through it those of bodies above it); the colouring is updated
+// from there.  An edge to the world anchor (-1) is no edge.
+AVBD_DEV void colour_seed(const DevWorld& w, int s, int a, int b) {
+    if (a < 0 || b < 0) return;
+    w.cseed[max(a, b)] = 1;
+    atomicOr(&w.st[s * ST_WORDS + ST_DIRTY], DIRTY_SEEDS);
+}
+
+// The scene's colouring is updated only when its constraint graph changes:
+// a manifold created this step (index >= the count before the broad phase)
 // that keeps contacts, or an older one that loses them.  A new pair whose
 // boxes do not touch is created and dropped in the same step and leaves the
 // graph as it was.
@@ -317,11 +327,11 @@ __global__ void k_manifold_init(DevWorld w) {
         w.m.fric[m] = fric;
         if (n == 0) {
             w.alive[m] = 0;
-            if (m < M_old) w.st[s * ST_WORDS + ST_DIRTY] = 1;
+            if (m < M_old) colour_seed(w, s, a, b);
             continue;
         }
         w.alive[m] = 1;
-        if (m >= M_old) w.st[s * ST_WORDS + ST_DIRTY] = 1;
+        if (m >= M_old) colour_seed(w, s, a, b);
         const m3 RA = rotmat(qA), RB = rotmat(qB);
         const bool decay = sc.post_stabilization == 0;
         for (int i = 0; i < n; ++i) {
@@ -383,7 +393,7 @@ __global__ void k_force_init(DevWorld w) {
             if (!scene_runs(w, s)) continue;
             if (w.j_flags[j] & AVBD_FORCE_DISABLED) {
                 w.j_flags[j] |= FORCE_DEAD;
-                w.st[s * ST_WORDS + ST_DIRTY] = 1;
+                colour_seed(w, s, w.j_a[j], b);
                 continue;
             }
             const int a = w.j_a[j];
@@ -413,7 +423,7 @@ __global__ void k_force_init(DevWorld w) {
             if (!scene_runs(w, s)) continue;
             if (w.s_flags[sp] & AVBD_FORCE_DISABLED) {
                 w.s_flags[sp] |= FORCE_DEAD;
-                w.st[s * ST_WORDS + ST_DIRTY] = 1;
+                colour_seed(w, s, w.s_a[sp], b);
                 continue;
             }
             const avbd_scene& sc = w.sc[s];
@@ -666,15 +676,18 @@ AVBD_DEV void nb_apply(const DevWorld& w, int v, int part, int parts, const NbCa
 // COLOUR_LDS_MAX bodies keep one word per body in LDS, (colour << 16) |
 // pending lower neighbours (colour 0xFFFF = not yet, 0xFFFE = static); larger
 // scenes keep colours and counters in global memory.  Each instantiation
-// takes only the scenes of its kind.
+// takes only the scenes of its kind.  A scene whose graph only gained or lost
+// some edges since its last colouring (DIRTY_SEEDS, the higher endpoints in
+// cseed) updates that colouring instead (full = 0 allows it).
 template <bool LDS>
-__global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
+__global__ __launch_bounds__(1024) void k_colour(DevWorld w, int full) {
     extern __shared__ unsigned int cw[];
     constexpr int CG = 8;  // lanes sharing one body's incidence list
     const int s = blockIdx.x;
     if (!scene_runs(w, s)) return;
     int* st = w.st + s * ST_WORDS;
-    if (!st[ST_DIRTY]) return;
+    const int dirty = st[ST_DIRTY];
+    if (!dirty) return;
     const int off = w.sc_off[s], end = w.sc_off[s + 1];
     const int n = end - off;
     if (LDS != (n <= COLOUR_LDS_MAX)) return;
@@ -685,13 +698,6 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
     __shared__ int overflow;
     if (threadIdx.x == 0) { qn[0] = 0; qn[1] = 0; maxc = -1; overflow = 0; }
     const unsigned UNCOL = 0xFFFFu, STAT = 0xFFFEu;
-    if (LDS) {
-        // static flags first: neighbour tests then read LDS only
-        for (int v = off + threadIdx.x; v < end; v += blockDim.x) cw[v - off] = (is_static(w, v) ? STAT : UNCOL) << 16;
-    } else {
-        for (int v = off + threadIdx.x; v < end; v += blockDim.x) w.colour[v] = -1;
-    }
-    __syncthreads();
     auto nb_static = [&](int o) -> bool { return LDS ? (cw[o - off] >> 16) == STAT : is_static(w, o); };
     auto nb_colour = [&](int o) -> int {
         if (LDS) {
@@ -702,101 +708,186 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
     };
     const int part = threadIdx.x % CG;
     const int groups = blockDim.x / CG;
-    // lower-neighbour counts; bodies with none start the first round
-    for (int base = off; base < end; base += groups) {
-        const int v = base + threadIdx.x / CG;
-        const bool dyn = v < end && !nb_static(v);
-        int c = 0;
-        if (dyn)
-            for_each_neighbour_part(w, v, part, CG, [&](int o) {
-                if (o < v && !nb_static(o)) ++c;
+    // smallest colour not taken by v's lower-index dynamic neighbours (the
+    // group's CG lanes each see a share of them)
+    auto first_free = [&](int v, bool valid, const NbCache& nc) -> int {
+        unsigned long long used = 0ull;
+        if (valid)
+            nb_apply(w, v, part, CG, nc, [&](int o) {
+                if (o < v && !nb_static(o)) {
+                    const int c = nb_colour(o);
+                    if (c >= 0 && c < 64) used |= (1ull << c);
+                }
             });
 #pragma unroll
-        for (int m = 1; m < CG; m <<= 1) c += __shfl_xor(c, m, CG);
-        if (dyn && part == 0) {
-            if (LDS) {
-                if (c > 0xFFFF) overflow = 1;
-                cw[v - off] = (UNCOL << 16) | (unsigned)min(c, 0xFFFF);
-            } else {
-                w.ccount[v] = c;
+        for (int m = 1; m < CG; m <<= 1) used |= __shfl_xor(used, m, CG);
+        int c = __ffsll((long long)~used) - 1;
+        if (used == ~0ull) {
+            // more than 64 colours: linear search above 64
+            c = 64;
+            while (true) {
+                int taken = 0;
+                if (valid)
+                    for_each_neighbour_part(w, v, part, CG, [&](int o) {
+                        if (o < v && !nb_static(o) && nb_colour(o) == c) taken = 1;
+                    });
+#pragma unroll
+                for (int m = 1; m < CG; m <<= 1) taken |= __shfl_xor(taken, m, CG);
+                if (!taken) break;
+                ++c;
             }
-            if (c == 0) qa[atomicAdd(&qn[0], 1)] = v;
         }
-    }
-    __syncthreads();
-    // Rounds: every body of the current queue takes its colour, then at once
-    // releases its higher neighbours (their counters); those that reach zero
-    // form the next queue.  No barrier is needed between the two steps of one
-    // body: a body of the queue never reads the colour of another body of the
-    // same queue (they are not neighbours, or one would still be waiting for
-    // the other), and a released body is only coloured in a later round.
-    int cur = 0;
-    while (!overflow) {
-        const int len = qn[cur];
-        if (len == 0) break;
-        int* q = cur ? qb : qa;
-        int* nq = cur ? qa : qb;
-        for (int kb = 0; kb < len; kb += groups) {
-            const int k = kb + threadIdx.x / CG;
-            const bool valid = k < len;
-            const int v = valid ? q[k] : 0;
-            NbCache nc;
-            nc.nf = nc.ne = 0;
-            if (valid) nb_load(w, v, part, CG, nc);
-            unsigned long long used = 0ull;
-            if (valid)
-                nb_apply(w, v, part, CG, nc, [&](int o) {
-                    if (o < v && !nb_static(o)) {
-                        const int c = nb_colour(o);
-                        if (c >= 0 && c < 64) used |= (1ull << c);
-                    }
+        return c;
+    };
+    if ((dirty & DIRTY_FULL) || full) {
+        if (LDS) {
+            // static flags first: neighbour tests then read LDS only
+            for (int v = off + threadIdx.x; v < end; v += blockDim.x)
+                cw[v - off] = (is_static(w, v) ? STAT : UNCOL) << 16;
+        } else {
+            for (int v = off + threadIdx.x; v < end; v += blockDim.x) w.colour[v] = -1;
+        }
+        __syncthreads();
+        // lower-neighbour counts; bodies with none start the first round
+        for (int base = off; base < end; base += groups) {
+            const int v = base + threadIdx.x / CG;
+            const bool dyn = v < end && !nb_static(v);
+            int c = 0;
+            if (dyn)
+                for_each_neighbour_part(w, v, part, CG, [&](int o) {
+                    if (o < v && !nb_static(o)) ++c;
                 });
 #pragma unroll
-            for (int m = 1; m < CG; m <<= 1) used |= __shfl_xor(used, m, CG);
-            int c = __ffsll((long long)~used) - 1;
-            if (used == ~0ull) {
-                // more than 64 colours: linear search above 64
-                c = 64;
-                while (true) {
-                    int taken = 0;
-                    if (valid)
-                        for_each_neighbour_part(w, v, part, CG, [&](int o) {
-                            if (o < v && !nb_static(o) && nb_colour(o) == c) taken = 1;
-                        });
-#pragma unroll
-                    for (int m = 1; m < CG; m <<= 1) taken |= __shfl_xor(taken, m, CG);
-                    if (!taken) break;
-                    ++c;
+            for (int m = 1; m < CG; m <<= 1) c += __shfl_xor(c, m, CG);
+            if (dyn && part == 0) {
+                if (LDS) {
+                    if (c > 0xFFFF) overflow = 1;
+                    cw[v - off] = (UNCOL << 16) | (unsigned)min(c, 0xFFFF);
+                } else {
+                    w.ccount[v] = c;
+                }
+                if (c == 0) qa[atomicAdd(&qn[0], 1)] = v;
+            }
+        }
+        __syncthreads();
+        // Rounds: every body of the current queue takes its colour, then at
+        // once releases its higher neighbours (their counters); those that
+        // reach zero form the next queue.  No barrier is needed between the
+        // two steps of one body: a body of the queue never reads the colour
+        // of another body of the same queue (they are not neighbours, or one
+        // would still be waiting for the other), and a released body is only
+        // coloured in a later round.
+        int cur = 0;
+        while (!overflow) {
+            const int len = qn[cur];
+            if (len == 0) break;
+            int* q = cur ? qb : qa;
+            int* nq = cur ? qa : qb;
+            for (int kb = 0; kb < len; kb += groups) {
+                const int k = kb + threadIdx.x / CG;
+                const bool valid = k < len;
+                const int v = valid ? q[k] : 0;
+                NbCache nc;
+                nc.nf = nc.ne = 0;
+                if (valid) nb_load(w, v, part, CG, nc);
+                const int c = first_free(v, valid, nc);
+                if (valid && part == 0) {
+                    if (LDS) cw[v - off] = (unsigned)min(c, 0xFFFD) << 16;  // its counter is 0
+                    else __atomic_store_n(&w.colour[v], c, __ATOMIC_RELEASE);
+                    atomicMax(&maxc, c);
+                }
+                // the group's colour write precedes every lane's releases
+                if (LDS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __builtin_amdgcn_wave_barrier();
+                if (valid)
+                    nb_apply(w, v, part, CG, nc, [&](int o) {
+                        if (o > v && !nb_static(o)) {
+                            const bool ready = LDS ? (atomicSub(&cw[o - off], 1u) & 0xFFFFu) == 1u
+                                                   : atomicSub(&w.ccount[o], 1) == 1;
+                            if (ready) nq[atomicAdd(&qn[cur ^ 1], 1)] = o;
+                        }
+                    });
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) qn[cur] = 0;
+            cur ^= 1;
+            __syncthreads();
+        }
+        if (LDS) {
+            for (int v = off + threadIdx.x; v < end; v += blockDim.x) {
+                const unsigned c = cw[v - off] >> 16;
+                w.colour[v] = c >= STAT ? -1 : (int)c;
+            }
+        }
+        for (int v = off + threadIdx.x; v < end; v += blockDim.x) w.cseed[v] = 0;
+    } else {
+        // Update of the last colouring: only bodies whose lower neighbours
+        // changed can change colour, and a body whose colour changes can
+        // change those of its higher neighbours.  Work lists of such bodies
+        // are re-evaluated round by round until no colour changes.  The
+        // greedy colouring is the unique fixed point of "colour(v) = first
+        // colour free among v's lower neighbours" (by induction over the
+        // index), and when a round changes nothing every body last evaluated
+        // saw its lower neighbours' final colours (a later change would have
+        // queued it again), so the result equals a colouring from scratch.
+        // A body is queued at most once per round (ccount = round tag).
+        if (LDS) {
+            for (int v = off + threadIdx.x; v < end; v += blockDim.x) {
+                const int c = w.colour[v];
+                cw[v - off] = (is_static(w, v) ? STAT : (unsigned)min(c, 0xFFFD)) << 16;
+            }
+        }
+        for (int v = off + threadIdx.x; v < end; v += blockDim.x) w.ccount[v] = -1;
+        __syncthreads();
+        for (int v = off + threadIdx.x; v < end; v += blockDim.x) {
+            if (w.cseed[v]) {
+                w.cseed[v] = 0;
+                if (!nb_static(v)) {
+                    w.ccount[v] = 0;
+                    qa[atomicAdd(&qn[0], 1)] = v;
                 }
             }
-            if (valid && part == 0) {
-                if (LDS) cw[v - off] = (unsigned)min(c, 0xFFFD) << 16;  // its counter is 0
-                else __atomic_store_n(&w.colour[v], c, __ATOMIC_RELEASE);
-                atomicMax(&maxc, c);
+        }
+        __syncthreads();
+        int cur = 0;
+        for (int round = 0;; ++round) {
+            const int len = qn[cur];
+            if (len == 0) break;
+            int* q = cur ? qb : qa;
+            int* nq = cur ? qa : qb;
+            for (int kb = 0; kb < len; kb += groups) {
+                const int k = kb + threadIdx.x / CG;
+                const bool valid = k < len;
+                const int v = valid ? q[k] : 0;
+                NbCache nc;
+                nc.nf = nc.ne = 0;
+                if (valid) nb_load(w, v, part, CG, nc);
+                const int c = first_free(v, valid, nc);
+                const bool changed = valid && c != nb_colour(v);
+                // every lane of the group has read the old colour
+                __builtin_amdgcn_wave_barrier();
+                if (changed && part == 0) {
+                    if (LDS) cw[v - off] = (unsigned)min(c, 0xFFFD) << 16;
+                    else __atomic_store_n(&w.colour[v], c, __ATOMIC_RELAXED);
+                }
+                if (changed)
+                    nb_apply(w, v, part, CG, nc, [&](int o) {
+                        if (o > v && !nb_static(o) && atomicExch(&w.ccount[o], round + 1) != round + 1)
+                            nq[atomicAdd(&qn[cur ^ 1], 1)] = o;
+                    });
             }
-            // the group's colour write precedes every lane's releases
-            if (LDS) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __builtin_amdgcn_wave_barrier();
-            if (valid)
-                nb_apply(w, v, part, CG, nc, [&](int o) {
-                    if (o > v && !nb_static(o)) {
-                        const bool ready = LDS ? (atomicSub(&cw[o - off], 1u) & 0xFFFFu) == 1u
-                                               : atomicSub(&w.ccount[o], 1) == 1;
-                        if (ready) nq[atomicAdd(&qn[cur ^ 1], 1)] = o;
-                    }
-                });
+            __syncthreads();
+            if (threadIdx.x == 0) qn[cur] = 0;
+            cur ^= 1;
+            __syncthreads();
         }
-        __syncthreads();
-        if (threadIdx.x == 0) qn[cur] = 0;
-        cur ^= 1;
-        __syncthreads();
-    }
-    if (LDS) {
         for (int v = off + threadIdx.x; v < end; v += blockDim.x) {
-            const unsigned c = cw[v - off] >> 16;
-            w.colour[v] = c >= STAT ? -1 : (int)c;
+            const int c = nb_colour(v);
+            if (!nb_static(v)) atomicMax(&maxc, c);
+            if (LDS) w.colour[v] = nb_static(v) ? -1 : c;
         }
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
         st[ST_NCOL] = maxc + 1;
@@ -805,8 +896,8 @@ __global__ __launch_bounds__(1024) void k_colour(DevWorld w) {
     }
 }
 
-template __global__ void k_colour<true>(DevWorld);
-template __global__ void k_colour<false>(DevWorld);
+template __global__ void k_colour<true>(DevWorld, int);
+template __global__ void k_colour<false>(DevWorld, int);
 
 // Colour of body b if it takes part in this step's sweeps, else -1.
 // Sweep bucket of body b (2 * colour + 1 for bodies whose only terms are FEM

@@ -29,10 +29,15 @@
 #define ST_REMOVED 1
 #define ST_OOB 2
 #define ST_ACTIVE 3
-#define ST_DIRTY 4   // topology changed -> recolour
+#define ST_DIRTY 4   // constraint graph changed -> recolour (DIRTY_* bits)
 #define ST_NCOL 5    // colours of the scene (last colouring)
 #define ST_FUSED 6   // phases 4-7 run in the per-scene kernel (k_scene_step)
 #define ST_WORDS 8
+
+// ST_DIRTY bits: colour the scene from scratch / update the last colouring
+// from the bodies marked in cseed
+#define DIRTY_FULL 1
+#define DIRTY_SEEDS 2
 
 // scenes of up to this many bodies are coloured with per-body words in LDS
 // (4 bytes each); larger ones in global memory
@@ -156,6 +161,7 @@ struct DevWorld {
     int* col_list;      // [N]
     // scratch for the colouring and the broad phase
     int* ccount;        // [N]
+    int* cseed;         // [N] bodies whose set of lower-index neighbours changed
     int* queue;         // [2N]
     int* cnt_i;         // [N]
     int* cur_i;         // [N+1]

@@ -79,3 +79,40 @@ def summary(g, o):
     if len(o.springs):
         s["spring_lambda_rel"] = compare_rows(g.springs["lambda"], o.springs["lambda"])
     return s
+
+
+def colouring_ok(world, colours):
+    """Independent check of the greedy spec: proper, and every dynamic body
+    has the smallest colour unused by its lower-index dynamic neighbours."""
+    dyn = (world.bodies["flags"] & 1) == 0
+    nb = [set() for _ in range(len(world.bodies))]
+
+    def link(ids):
+        ids = [int(i) for i in ids if i >= 0 and dyn[i]]
+        for x in ids:
+            for y in ids:
+                if x != y:
+                    nb[x].add(y)
+
+    for m in world.manifolds:
+        link([m["body_a"], m["body_b"]])
+    for j in world.joints:
+        link([j["body_a"], j["body_b"]])
+    for s in world.springs:
+        link([s["body_a"], s["body_b"]])
+    for t in world.tets:
+        link(t["v"])
+    for t in world.tris:
+        link(t["v"])
+    for v in range(len(world.bodies)):
+        if not dyn[v]:
+            continue
+        lower = {colours[u] for u in nb[v] if u < v}
+        mex = 0
+        while mex in lower:
+            mex += 1
+        if colours[v] != mex:
+            return False
+        if any(colours[u] == colours[v] for u in nb[v]):
+            return False
+    return True

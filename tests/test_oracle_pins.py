@@ -12,6 +12,7 @@ import pytest
 from scipy.spatial.transform import Rotation
 
 import oracle_py as O
+from parity import colouring_ok
 from skyephyx_amd import scenes
 from skyephyx_amd.world import MODEL_CUBE, WorldBuilder
 
@@ -370,43 +371,6 @@ def test_sat_against_brute_force():
 
 
 # ----------------------------------------------------------------- colouring
-def colouring_ok(world, colours):
-    """Independent check of the greedy spec: proper, and every dynamic body
-    has the smallest colour unused by its lower-index dynamic neighbours."""
-    dyn = (world.bodies["flags"] & 1) == 0
-    nb = [set() for _ in range(len(world.bodies))]
-
-    def link(ids):
-        ids = [int(i) for i in ids if i >= 0 and dyn[i]]
-        for x in ids:
-            for y in ids:
-                if x != y:
-                    nb[x].add(y)
-
-    for m in world.manifolds:
-        link([m["body_a"], m["body_b"]])
-    for j in world.joints:
-        link([j["body_a"], j["body_b"]])
-    for s in world.springs:
-        link([s["body_a"], s["body_b"]])
-    for t in world.tets:
-        link(t["v"])
-    for t in world.tris:
-        link(t["v"])
-    for v in range(len(world.bodies)):
-        if not dyn[v]:
-            continue
-        lower = {colours[u] for u in nb[v] if u < v}
-        mex = 0
-        while mex in lower:
-            mex += 1
-        if colours[v] != mex:
-            return False
-        if any(colours[u] == colours[v] for u in nb[v]):
-            return False
-    return True
-
-
 @pytest.mark.parametrize("make", [lambda: scenes.pyramid_2d(), lambda: scenes.fem_block(n=5),
                                   lambda: scenes.box_lattice(shape=(3, 3, 2), spacing=3.0, n_chain=6, n_springs=4,
                                                              bounds=50.0)])
