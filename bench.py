@@ -292,6 +292,7 @@ def main():
     ctx_bytes = ctx.memory()
     ctx.close()
     log(f"[bench] timed {args.steps} steps in {elapsed:.3f}s; emergency scenes: {int(em.sum())}")
+    log("[bench] step ms: " + " ".join(f"{t:.1f}" for t in step_ms))
 
     S = len(state.scenes)
     scene_steps = S * args.steps / elapsed

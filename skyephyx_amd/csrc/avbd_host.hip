@@ -873,59 +873,73 @@ int poststab_scenes(avbd_ctx* c, const std::vector<char>& only) {
 int remove_out_of_bounds(avbd_ctx* c) {
     DevWorld& w = c->w;
     hipStream_t st = c->stream;
-    const int S = w.S, N = w.N;
+    const int S = w.S;
     c->drop_graphs();
     std::vector<int> stw;
-    std::vector<float4> pos;
-    std::vector<uint32_t> flags;
     int rc;
-    if ((rc = download_vec(c, stw, w.st, (size_t)S * ST_WORDS)) || (rc = download_vec(c, pos, w.pos, N)) ||
-        (rc = download_vec(c, flags, w.flags, N)))
-        return rc;
+    if ((rc = download_vec(c, stw, w.st, (size_t)S * ST_WORDS))) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     std::vector<char> aff(S, 0);
-    bool any = false;
+    std::vector<int> aff_ids;
     for (int s = 0; s < S; ++s) {
         int* sw = &stw[s * ST_WORDS];
         if (sw[ST_OOB] > 0 && !sw[ST_EMERGENCY] && sw[ST_ACTIVE]) {
             aff[s] = 1;
-            any = true;
+            aff_ids.push_back(s);
         }
         sw[ST_OOB] = 0;
     }
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_ANY_OOB, 0);
-    if (!any) {
+    if (aff_ids.empty()) {
         if ((rc = upload_vec(c, w.st, stw))) return rc;
         HIP_TRY(hipStreamSynchronize(st));
         return 0;
     }
+    // Only the affected scenes' body ranges are read back and renumbered;
+    // every other index stays where it is.
+    struct Seg {
+        int s, a, e, n_live;
+        std::vector<float4> pos;
+        std::vector<uint32_t> flags;
+        std::vector<int> perm;  // new local -> old global
+    };
+    std::vector<Seg> segs(aff_ids.size());
+    for (size_t q = 0; q < aff_ids.size(); ++q) {
+        Seg& g = segs[q];
+        g.s = aff_ids[q];
+        g.a = c->sc_off[g.s];
+        g.e = c->sc_off[g.s + 1];
+        if ((rc = download_vec(c, g.pos, w.pos + g.a, (size_t)(g.e - g.a))) ||
+            (rc = download_vec(c, g.flags, w.flags + g.a, (size_t)(g.e - g.a))))
+            return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
     // New order inside each affected scene: its live bodies (earlier dead
     // slots are at the end of its index range), the removals applied one by
     // one as swap with last, then the removed bodies and the earlier dead
-    // slots as the scene's trailing dead slots.  Other scenes keep every
-    // index.  perm: new -> old; map: old -> new, -1 for the removed.
-    std::vector<int> perm(N), map(N);
-    for (int i = 0; i < N; ++i) perm[i] = map[i] = i;
-    std::vector<int> dead_new;  // new dead slots
-    struct Seg { int a, e, n_live; };
-    std::vector<Seg> segs;
-    for (int s = 0; s < S; ++s) {
-        if (!aff[s]) continue;
-        const int a = c->sc_off[s], e = c->sc_off[s + 1];
+    // slots as the scene's trailing dead slots.  aoff[s]: where scene s's
+    // part of lmap starts (-1: not affected); lmap: old local -> new global
+    // index, -1 for a removed body.  mv_dst[k] <- mv_src[k] moves the bodies.
+    std::vector<int> aoff(S, -1), lmap, mv_src, mv_dst, dead_new;
+    for (Seg& g : segs) {
+        const int a = g.a, e = g.e;
+        aoff[g.s] = (int)lmap.size();
+        const size_t L0 = lmap.size();
+        lmap.resize(L0 + (e - a), -1);
         std::vector<int> order, where(e - a, -1), gone;
         for (int i = a; i < e; ++i)
             if (!c->body_dead[i]) {
                 where[i - a] = (int)order.size();
                 order.push_back(i);
             }
-        const avbd_scene& sc = c->scenes[s];
+        const avbd_scene& sc = c->scenes[g.s];
         for (int i = a; i < e; ++i) {
-            if (c->body_dead[i] || (flags[i] & AVBD_BODY_STATIC)) continue;
-            const float4 p = pos[i];
+            if (c->body_dead[i] || (g.flags[i - a] & AVBD_BODY_STATIC)) continue;
+            const float4 p = g.pos[i - a];
             if (p.x < sc.bounds_min[0] || p.x > sc.bounds_max[0] || p.y < sc.bounds_min[1] || p.y > sc.bounds_max[1] ||
                 p.z < sc.bounds_min[2] || p.z > sc.bounds_max[2]) {
                 const int k = where[i - a], last = (int)order.size() - 1;
-                c->removed_scene.push_back(s);
+                c->removed_scene.push_back(g.s);
                 c->removed_index.push_back(k);
                 if (k != last) {
                     order[k] = order[last];
@@ -936,26 +950,29 @@ int remove_out_of_bounds(avbd_ctx* c) {
                 gone.push_back(i);
             }
         }
-        stw[s * ST_WORDS + ST_REMOVED] += (int)gone.size();
-        stw[s * ST_WORDS + ST_DIRTY] = DIRTY_FULL;
-        segs.push_back({a, e, (int)order.size()});
+        stw[g.s * ST_WORDS + ST_REMOVED] += (int)gone.size();
+        stw[g.s * ST_WORDS + ST_DIRTY] = DIRTY_FULL;
+        g.n_live = (int)order.size();
         int k = a;
-        for (int o : order) {
-            perm[k] = o;
-            map[o] = k++;
-        }
+        auto place = [&](int o, bool live) {
+            g.perm.push_back(o);
+            mv_dst.push_back(k);
+            mv_src.push_back(o);
+            lmap[L0 + (o - a)] = live ? k : -1;
+            ++k;
+        };
+        for (int o : order) place(o, true);
         for (int o : gone) {
-            perm[k] = o;
-            map[o] = -1;
-            dead_new.push_back(k++);
+            dead_new.push_back(k);
+            place(o, false);
         }
         for (int i = a; i < e; ++i)
-            if (c->body_dead[i]) {
-                perm[k] = i;
-                map[i] = -1;
-                ++k;
-            }
+            if (c->body_dead[i]) place(i, false);
     }
+    // old global index -> new global index (-1 removed) inside a segment
+    auto new_index = [&](const Seg& g, int o) -> int {
+        return o >= g.a && o < g.e ? lmap[aoff[g.s] + (o - g.a)] : -1;
+    };
     // energy lists of the affected scenes: each is the contiguous range
     // [e_off[a], e_off[e]) of the incidence arrays (an element's vertices
     // share its scene).  Rebuilt on the host in the new vertex order without
@@ -974,9 +991,8 @@ int remove_out_of_bounds(avbd_ctx* c) {
     }
     HIP_TRY(hipStreamSynchronize(st));
     for (size_t q = 0; q < segs.size(); ++q) {
-        const Seg& g = segs[q];
         SegData& d = sd[q];
-        const int E0 = d.off[0], E1 = d.off[g.e - g.a];
+        const int E0 = d.off[0], E1 = d.off[segs[q].e - segs[q].a];
         if ((rc = download_vec(c, d.ent, w.e_ent + E0, (size_t)(E1 - E0))) ||
             (rc = download_vec(c, d.ea, w.ev_a + E0, (size_t)(E1 - E0))) ||
             (rc = download_vec(c, d.eb, w.ev_b + E0, (size_t)(E1 - E0))) ||
@@ -996,12 +1012,14 @@ int remove_out_of_bounds(avbd_ctx* c) {
         for (int k = 0; k < g.e - g.a; ++k) {
             noff[k] = E0 + (int)nent.size();
             if (k >= g.n_live) continue;
-            const int o = perm[g.a + k] - g.a;  // old local index
+            const int o = g.perm[k] - g.a;  // old local index
             for (int x = d.off[o] - E0; x < d.off[o + 1] - E0; ++x) {
                 const uint32_t ent = d.ent[x];
                 const int4 v = d.ev[x];
                 const bool tri = (ent >> 31) != 0;
-                if (map[v.x] < 0 || map[v.y] < 0 || map[v.z] < 0 || (!tri && map[v.w] < 0)) {
+                const int m0 = new_index(g, v.x), m1 = new_index(g, v.y), m2 = new_index(g, v.z);
+                const int m3 = tri ? 0 : new_index(g, v.w);
+                if (m0 < 0 || m1 < 0 || m2 < 0 || m3 < 0) {
                     const uint32_t idx = ent & 0x1FFFFFFFu;
                     if (tri) c->tri_dead[idx] = 1;
                     else c->tet_dead[idx] = 1;
@@ -1011,7 +1029,7 @@ int remove_out_of_bounds(avbd_ctx* c) {
                 na.push_back(d.ea[x]);
                 nb.push_back(d.eb[x]);
                 nc.push_back(d.ec[x]);
-                nv.push_back(make_int4(map[v.x], map[v.y], map[v.z], tri ? 0 : map[v.w]));
+                nv.push_back(make_int4(m0, m1, m2, m3));
             }
         }
         if (E0 + (int)nent.size() > E1) return fail(AVBD_E_STATE, "energy list grew during removal");
@@ -1022,44 +1040,46 @@ int remove_out_of_bounds(avbd_ctx* c) {
             return rc;
         HIP_TRY(hipStreamSynchronize(st));
     }
-    // device renumbering of the bodies, forces and element vertex ids
+    // device renumbering of the moved bodies, then of the forces' and
+    // elements' body ids and the manifolds' endpoints
+    const int K = (int)mv_src.size();
     struct Scratch {  // freed on every return path
-        void* p[3] = {nullptr, nullptr, nullptr};
-        ~Scratch() { for (void* q : p) if (q) hipFree(q); }
+        void* p = nullptr;
+        ~Scratch() { if (p) hipFree(p); }
     } scratch;
-    HIP_TRY(hipMalloc(&scratch.p[0], sizeof(int) * std::max(N, 1)));
-    HIP_TRY(hipMalloc(&scratch.p[1], sizeof(int) * std::max(N, 1)));
-    HIP_TRY(hipMalloc(&scratch.p[2], sizeof(float4) * std::max(N, 1)));
-    int* d_perm = static_cast<int*>(scratch.p[0]);
-    int* d_map = static_cast<int*>(scratch.p[1]);
-    float4* d_tmp = static_cast<float4*>(scratch.p[2]);
-    if ((rc = upload_vec(c, d_perm, perm)) || (rc = upload_vec(c, d_map, map))) return rc;
+    const size_t ints = 3 * (size_t)K + (size_t)S + (size_t)dead_new.size() + 4;
+    HIP_TRY(hipMalloc(&scratch.p, sizeof(float4) * (size_t)K + sizeof(int) * ints));
+    float4* d_tmp = static_cast<float4*>(scratch.p);
+    int* d_src = reinterpret_cast<int*>(d_tmp + K);
+    int* d_dst = d_src + K;
+    int* d_lmap = d_dst + K;
+    int* d_aoff = d_lmap + K;
+    int* d_dead = d_aoff + S;
+    if ((rc = upload_vec(c, d_src, mv_src)) || (rc = upload_vec(c, d_dst, mv_dst)) || (rc = upload_vec(c, d_lmap, lmap)) ||
+        (rc = upload_vec(c, d_aoff, aoff)) || (rc = upload_vec(c, d_dead, dead_new)))
+        return rc;
     {
         float4* f4[] = {w.pos, w.rot, w.scale, w.vel, w.angvel, w.pvel, w.pangvel, w.inertia,
                         w.last_pos, w.last_rot, w.inert_pos, w.inert_rot, w.diff0, w.diff1};
-        for (float4* a : f4) {
-            hipLaunchKernelGGL(k_gather_f4, dim3(grid_for(N)), dim3(256), 0, st, d_tmp, a, d_perm, N);
-            HIP_TRY(hipMemcpyAsync(a, d_tmp, sizeof(float4) * N, hipMemcpyDeviceToDevice, st));
+        for (float4* arr : f4) {
+            hipLaunchKernelGGL(k_gather_f4, dim3(grid_for(K)), dim3(256), 0, st, d_tmp, arr, d_src, K);
+            hipLaunchKernelGGL(k_scatter_f4, dim3(grid_for(K)), dim3(256), 0, st, arr, d_tmp, d_dst, K);
         }
         uint32_t* u32[] = {w.flags, reinterpret_cast<uint32_t*>(w.scene)};
-        for (uint32_t* a : u32) {
-            hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(N)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(d_tmp), a,
-                               d_perm, N);
-            HIP_TRY(hipMemcpyAsync(a, d_tmp, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
+        uint32_t* t32 = reinterpret_cast<uint32_t*>(d_tmp);
+        for (uint32_t* arr : u32) {
+            hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(K)), dim3(256), 0, st, t32, arr, d_src, K);
+            hipLaunchKernelGGL(k_scatter_u32, dim3(grid_for(K)), dim3(256), 0, st, arr, t32, d_dst, K);
         }
     }
-    if (!dead_new.empty()) {
-        // the new dead slots: static, no broad-phase sphere, at rest (d_perm
-        // is free again once the gathers above have run: same stream)
-        if ((rc = upload_vec(c, d_perm, dead_new))) return rc;
-        hipLaunchKernelGGL(k_kill_bodies, dim3(grid_for((int)dead_new.size())), dim3(256), 0, st, w, (const int*)d_perm,
+    if (!dead_new.empty())
+        // the new dead slots: static, no broad-phase sphere, at rest
+        hipLaunchKernelGGL(k_kill_bodies, dim3(grid_for((int)dead_new.size())), dim3(256), 0, st, w, (const int*)d_dead,
                            (int)dead_new.size());
-    }
-    if (w.J + w.SP > 0)
-        hipLaunchKernelGGL(k_remap_forces, dim3(grid_for(w.J + w.SP)), dim3(256), 0, st, w, (const int*)d_map);
-    if (w.T + w.TR > 0)
-        hipLaunchKernelGGL(k_remap_elements, dim3(grid_for(w.T + w.TR)), dim3(256), 0, st, w, (const int*)d_map);
-    hipLaunchKernelGGL(k_remap_manifolds, dim3(grid_for(w.Mcap)), dim3(256), 0, st, w, (const int*)d_map);
+    const BodyRemap rm{d_aoff, d_lmap};
+    if (w.J + w.SP > 0) hipLaunchKernelGGL(k_remap_forces, dim3(grid_for(w.J + w.SP)), dim3(256), 0, st, w, rm);
+    if (w.T + w.TR > 0) hipLaunchKernelGGL(k_remap_elements, dim3(grid_for(w.T + w.TR)), dim3(256), 0, st, w, rm);
+    hipLaunchKernelGGL(k_remap_manifolds, dim3(grid_for(w.Mcap)), dim3(256), 0, st, w, rm);
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.alive + w.Mcap, 0);
     if ((rc = scan_ints(c, w.alive, w.alive_off, w.Mcap + 1))) return rc;
     ManifoldBuf& dst = c->mb[c->cur ^ 1];

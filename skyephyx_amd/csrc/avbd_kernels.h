@@ -33,7 +33,9 @@ __global__ void k_velocity(DevWorld w);
 __global__ void k_stick(DevWorld w);
 __global__ void k_gather_f4(float4* dst, const float4* src, const int* perm, int n);
 __global__ void k_gather_u32(uint32_t* dst, const uint32_t* src, const int* perm, int n);
-__global__ void k_remap_forces(DevWorld w, const int* map);
-__global__ void k_remap_manifolds(DevWorld w, const int* map);
+__global__ void k_scatter_f4(float4* dst, const float4* src, const int* to, int n);
+__global__ void k_scatter_u32(uint32_t* dst, const uint32_t* src, const int* to, int n);
+__global__ void k_remap_forces(DevWorld w, BodyRemap rm);
+__global__ void k_remap_manifolds(DevWorld w, BodyRemap rm);
 __global__ void k_kill_bodies(DevWorld w, const int* list, int n);
-__global__ void k_remap_elements(DevWorld w, const int* map);
+__global__ void k_remap_elements(DevWorld w, BodyRemap rm);

@@ -1870,19 +1870,36 @@ template __global__ void k_scene_step<32>(DevWorld, const int*, int);
 
 // ------------------------------------------------ out-of-bounds removal
 // Solver::RemoveBody for the bodies that left the world bounds
-// (solver.cpp:520-534, 104-116): the host computes the surviving order
-// (perm: new index -> old, map: old -> new or -1) and these kernels move the
-// device state to it.
+// (solver.cpp:520-534, 104-116): the host computes the surviving order inside
+// the affected scenes, the moves (new index <- old index, gathered into a
+// buffer, then scattered) and the renumbering (BodyRemap), and these kernels
+// apply them to the device state.
 __global__ void k_gather_f4(float4* dst, const float4* src, const int* perm, int n) {
     GRID_STRIDE(i, n) dst[i] = src[perm[i]];
 }
 __global__ void k_gather_u32(uint32_t* dst, const uint32_t* src, const int* perm, int n) {
     GRID_STRIDE(i, n) dst[i] = src[perm[i]];
 }
+__global__ void k_scatter_f4(float4* dst, const float4* src, const int* to, int n) {
+    GRID_STRIDE(i, n) dst[to[i]] = src[i];
+}
+__global__ void k_scatter_u32(uint32_t* dst, const uint32_t* src, const int* to, int n) {
+    GRID_STRIDE(i, n) dst[to[i]] = src[i];
+}
+
+// New index of body i (any index; -1 for the world anchor stays -1).  The
+// scene of i is read after the move, which keeps every body in its scene's
+// range, so scene[i] is the scene of the old index as well.
+AVBD_DEV int remap_body(const DevWorld& w, BodyRemap rm, int i) {
+    if (i < 0) return -1;
+    const int s = w.scene[i];
+    const int o = rm.aoff[s];
+    return o < 0 ? i : rm.lmap[o + (i - w.sc_off[s])];
+}
 
 // Joints and springs: endpoints renumbered; a force that loses a body is
 // removed with it (~Mesh deletes the body's forces, geometry.cpp:11-24).
-__global__ void k_remap_forces(DevWorld w, const int* map) {
+__global__ void k_remap_forces(DevWorld w, BodyRemap rm) {
     GRID_STRIDE(t, w.J + w.SP) {
         int *pa, *pb;
         uint32_t* fl;
@@ -1890,7 +1907,7 @@ __global__ void k_remap_forces(DevWorld w, const int* map) {
         else { pa = &w.s_a[t - w.J]; pb = &w.s_b[t - w.J]; fl = &w.s_flags[t - w.J]; }
         if (*fl & FORCE_DEAD) continue;
         const int a = *pa, b = *pb;
-        const int na = a >= 0 ? map[a] : -1, nb = map[b];
+        const int na = remap_body(w, rm, a), nb = remap_body(w, rm, b);
         if ((a >= 0 && na < 0) || nb < 0) {
             *fl |= FORCE_DEAD;
             continue;
@@ -1919,25 +1936,25 @@ __global__ void k_kill_bodies(DevWorld w, const int* list, int n) {
 
 // Element vertex ids renumbered (-1 for a removed body, kept by later
 // removals; such an element is removed with its body and no longer read).
-__global__ void k_remap_elements(DevWorld w, const int* map) {
+__global__ void k_remap_elements(DevWorld w, BodyRemap rm) {
     GRID_STRIDE(t, w.T + w.TR) {
         int4* v = t < w.T ? &w.t_v[t] : &w.r_v[t - w.T];
         const int4 o = *v;
-        auto r = [&](int x) { return x >= 0 ? map[x] : -1; };
+        auto r = [&](int x) { return remap_body(w, rm, x); };
         *v = make_int4(r(o.x), r(o.y), r(o.z), t < w.T ? r(o.w) : 0);
     }
 }
 
 // Manifolds: endpoints renumbered (the bodies keep their roles A and B);
 // those that lose a body are dropped by the compaction that follows.
-__global__ void k_remap_manifolds(DevWorld w, const int* map) {
+__global__ void k_remap_manifolds(DevWorld w, BodyRemap rm) {
     const int M = w.g[G_MANIFOLDS];
     GRID_STRIDE(m, w.Mcap) {
         if (m >= M) {
             w.alive[m] = 0;
             continue;
         }
-        const int na = map[w.m.a[m]], nb = map[w.m.b[m]];
+        const int na = remap_body(w, rm, w.m.a[m]), nb = remap_body(w, rm, w.m.b[m]);
         const bool keep = na >= 0 && nb >= 0;
         w.alive[m] = keep ? 1 : 0;
         if (keep) {

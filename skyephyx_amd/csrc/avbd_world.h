@@ -182,3 +182,12 @@ struct DevWorld {
     int* g;             // global status words [G_WORDS]
     long long* next_serial;
 };
+
+// Body renumbering of a bounds removal, which permutes bodies only inside the
+// affected scenes: aoff[s] = start of scene s's part of lmap (-1: scene not
+// affected, its indices stay), lmap[aoff[s] + i - sc_off[s]] = new index of
+// the scene's old body i (-1: removed).
+struct BodyRemap {
+    const int* aoff;  // [S]
+    const int* lmap;  // [bodies of the affected scenes]
+};

@@ -59,3 +59,17 @@ def test_removal_in_one_scene_leaves_the_others_alone():
     np.testing.assert_array_equal(mid.bodies["pos"], alone.bodies["pos"])
     np.testing.assert_array_equal(mid.bodies["rot"], alone.bodies["rot"])
     assert int(both.scenes["n_removed"][0]) == int(both.scenes["n_removed"][2]) == 9
+    # both removing scenes end as the scene does alone: same order, poses,
+    # surviving tets (renumbered) and forces
+    with Context(lanes=4, fem_lanes=4) as ctx:
+        ctx.upload(scenes.out_of_bounds())
+        ctx.step(3)
+        oob = ctx.download()
+    for k in (0, 2):
+        sc = both.extract_scene(k)
+        for f in ("pos", "rot", "vel", "ang_vel"):
+            np.testing.assert_array_equal(sc.bodies[f], oob.bodies[f])
+        np.testing.assert_array_equal(sc.tets["v"], oob.tets["v"])
+        np.testing.assert_array_equal(sc.springs["body_b"], oob.springs["body_b"])
+        np.testing.assert_array_equal(sc.manifolds["body_a"], oob.manifolds["body_a"])
+        np.testing.assert_array_equal(sc.manifolds["body_b"], oob.manifolds["body_b"])
