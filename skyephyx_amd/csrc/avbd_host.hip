@@ -74,6 +74,9 @@ struct avbd_ctx {
     // when the two lane widths agree (the kernel has one).
     bool use_fused = false;
     int fused_max = 512;
+    // a world with at most SMALL_PAIR_SORT possible candidate pairs sorts its
+    // new pairs in one workgroup, without reading their count back
+    bool small_pairs = false;
     // a scene whose constraint graph changed a little updates its last
     // colouring from the changed bodies instead of colouring from scratch
     bool colour_updates = true;
@@ -650,6 +653,12 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     c->scene_bodies.assign(S, 0);
     c->scene_breakable.assign(S, 0);
     for (int s = 0; s < S; ++s) c->scene_bodies[s] = off[s + 1] - off[s];
+    {
+        // every candidate pair lies inside one scene
+        long long most = 0;
+        for (int s = 0; s < S; ++s) most += (long long)c->scene_bodies[s] * (c->scene_bodies[s] - 1) / 2;
+        c->small_pairs = most <= SMALL_PAIR_SORT;
+    }
     c->max_scene_bodies = S > 0 ? *std::max_element(c->scene_bodies.begin(), c->scene_bodies.end()) : 0;
     for (int i = 0; i < J; ++i)
         for (int r = 0; r < 6; ++r)
@@ -1175,17 +1184,23 @@ int do_step(avbd_ctx* c) {
         hipLaunchKernelGGL(k_tile_bounds, dim3(c->n_tiles), dim3(256), 0, st, w);
         hipLaunchKernelGGL(k_broad, dim3(c->n_items), dim3(256), 0, st, w, (const int4*)c->items);
         hipLaunchKernelGGL(k_pair_filter, dim3(grid_for(N)), dim3(256), 0, st, w);
-        // the new-pair count sizes the sort
-        HIP_TRY(hipMemcpyAsync(c->host_word, w.g + G_NEW_PAIRS, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        const int K = std::min(*c->host_word, w.pair_cap);
-        if (K > 0) {
-            hipcub::DoubleBuffer<unsigned long long> kb(reinterpret_cast<unsigned long long*>(w.pairs),
-                                                         reinterpret_cast<unsigned long long*>(w.pairs_tmp));
-            size_t bytes = c->cub_bytes;
-            HIP_TRY(hipcub::DeviceRadixSort::SortKeys(c->cub_tmp, bytes, kb, K, 0, pair_key_bits(N), st));
-            hipLaunchKernelGGL(k_pair_append, dim3(grid_for(K)), dim3(256), 0, st, w,
-                               (const unsigned long long*)kb.Current());
+        if (c->small_pairs) {
+            hipLaunchKernelGGL(k_pair_sort_small, dim3(1), dim3(1024), 0, st, w);
+            hipLaunchKernelGGL(k_pair_append, dim3(grid_for(SMALL_PAIR_SORT)), dim3(256), 0, st, w,
+                               (const unsigned long long*)w.pairs);
+        } else {
+            // the new-pair count sizes the sort
+            HIP_TRY(hipMemcpyAsync(c->host_word, w.g + G_NEW_PAIRS, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            const int K = std::min(*c->host_word, w.pair_cap);
+            if (K > 0) {
+                hipcub::DoubleBuffer<unsigned long long> kb(reinterpret_cast<unsigned long long*>(w.pairs),
+                                                             reinterpret_cast<unsigned long long*>(w.pairs_tmp));
+                size_t bytes = c->cub_bytes;
+                HIP_TRY(hipcub::DeviceRadixSort::SortKeys(c->cub_tmp, bytes, kb, K, 0, pair_key_bits(N), st));
+                hipLaunchKernelGGL(k_pair_append, dim3(grid_for(K)), dim3(256), 0, st, w,
+                                   (const unsigned long long*)kb.Current());
+            }
         }
     }
     hipLaunchKernelGGL(k_pair_finish, dim3(1), dim3(1), 0, st, w);

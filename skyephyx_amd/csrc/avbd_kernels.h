@@ -8,6 +8,7 @@ __global__ void k_set_word(int* p, int v);
 __global__ void k_tile_bounds(DevWorld w);
 __global__ void k_broad(DevWorld w, const int4* items);
 __global__ void k_pair_filter(DevWorld w);
+__global__ void k_pair_sort_small(DevWorld w);
 __global__ void k_pair_append(DevWorld w, const unsigned long long* keys);
 __global__ void k_pair_finish(DevWorld w);
 __global__ void k_manifold_init(DevWorld w);

@@ -218,6 +218,28 @@ __global__ void k_pair_filter(DevWorld w) {
     }
 }
 
+// The new-pair keys in ascending order, in place, in one workgroup: each
+// key's position is the number of smaller keys (the keys are distinct
+// pairs).  For worlds whose possible candidate pairs fit SMALL_PAIR_SORT, so
+// the count need not be read back to size a device-wide sort.
+__global__ __launch_bounds__(1024) void k_pair_sort_small(DevWorld w) {
+    __shared__ unsigned long long sk[SMALL_PAIR_SORT];
+    const int K = min(w.g[G_NEW_PAIRS], w.pair_cap);
+    if (K > SMALL_PAIR_SORT) {  // cannot happen for the worlds this is used on
+        if (threadIdx.x == 0) w.g[G_ERR_PAIR_CAP] = 1;
+        return;
+    }
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(w.pairs);
+    for (int i = threadIdx.x; i < K; i += blockDim.x) sk[i] = keys[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += blockDim.x) {
+        const unsigned long long k = sk[i];
+        int r = 0;
+        for (int j = 0; j < K; ++j) r += sk[j] < k ? 1 : 0;
+        keys[r] = k;
+    }
+}
+
 // Append the new pairs as manifolds in lexicographic (i, j) order -- the
 // creation order of the reference's nested loop -- from the sorted keys.
 __global__ void k_pair_append(DevWorld w, const unsigned long long* keys) {

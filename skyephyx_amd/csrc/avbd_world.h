@@ -43,6 +43,10 @@
 // (4 bytes each); larger ones in global memory
 #define COLOUR_LDS_MAX 32768
 
+// worlds with at most this many possible candidate pairs sort their new pairs
+// in one workgroup (k_pair_sort_small)
+#define SMALL_PAIR_SORT 2048
+
 // global error/status words
 #define G_ERR_MANIFOLD_CAP 0
 #define G_ERR_PAIR_CAP 1
