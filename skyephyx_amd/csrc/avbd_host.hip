@@ -280,9 +280,13 @@ int build_incidence(avbd_ctx* c) {
 // Blocks of a bucket launch: enough for n bodies at G lanes, rounded up to a
 // power of two (the kernel strides over the bucket, whose size it reads from
 // device memory), so that a step graph keyed on it survives small changes.
-int bucket_grid(int n, int G) {
+int bucket_grid(const avbd_ctx* c, int n, int G) {
     const int g = grid_for((long long)n * G, 256, 16384);
-    int p = 1;
+    // In a world of several scenes at least 64 blocks (the surplus ones exit
+    // at once): the launch shape of small buckets then does not change with
+    // their exact size, so a served tick's step graph repeats although its
+    // scene set changes.  A single small scene keeps the smallest grid.
+    int p = c->w.S > 1 ? 64 : 1;
     while (p < g) p <<= 1;
     return std::min(p, 16384);
 }
@@ -317,7 +321,7 @@ void launch_bucket(avbd_ctx* c, int bucket, int n, int iter, int poststab) {
     int G = fixed > 0 ? fixed : c->tuner.choose(FEM ? 1 : 0, n, &measure);
     const int mark = c->mark_begin(FEM ? avbd_ctx::K_FEM : avbd_ctx::K_FORCES);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].a, c->stream);
-    launch_primal_g<FEM>(c, G, bucket_grid(n, G), bucket, iter, poststab);
+    launch_primal_g<FEM>(c, G, bucket_grid(c, n, G), bucket, iter, poststab);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].b, c->stream);
     c->mark_end(mark);
 }
@@ -1159,7 +1163,7 @@ bool step_graph_key(const avbd_ctx* c, const int* off, int ncol, int max_iters, 
         }
         const int G = bucket_lanes(c, (k & 1) != 0, n);
         if (G == 0) return false;
-        key->push_back(G * 65536 + bucket_grid(n, G));
+        key->push_back(G * 65536 + bucket_grid(c, n, G));
     }
     return true;
 }
