@@ -244,6 +244,9 @@ def main():
     ap.add_argument("--serve-load", type=float, nargs="+", default=[0.8],
                     help="served load points as fractions of the served (closed-loop) saturation")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per served load point")
+    ap.add_argument("--snapshot-dir", default=None,
+                    help="after the timed steps, save the device world as a snapshot file in this folder and "
+                         "load it into a fresh context (reports bytes and both rates; the file is removed)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only one GPU is supported in this build (asked for --gpus {args.gpus}); "
@@ -290,7 +293,27 @@ def main():
     em, _ = ctx.status(len(state.scenes))
     tune = ctx.tune_table()
     ctx_bytes = ctx.memory()
+    snapshot = None
+    if args.snapshot_dir:
+        path = os.path.join(args.snapshot_dir, f"{args.workload}.avbdsnap")
+        t0 = time.perf_counter()
+        ctx.save_snapshot(path)
+        t_save = time.perf_counter() - t0
     ctx.close()
+    if args.snapshot_dir:
+        size = os.path.getsize(path)
+        c2 = Context(lanes=args.lanes, fem_lanes=args.fem_lanes)
+        t0 = time.perf_counter()
+        c2.load_snapshot(path)
+        c2.synchronize()
+        t_load = time.perf_counter() - t0
+        c2.close()
+        os.remove(path)
+        snapshot = {"bytes": size, "save_s": round(t_save, 3), "save_GB_per_s": round(size / t_save / 1e9, 3),
+                    "load_s": round(t_load, 3), "load_GB_per_s": round(size / t_load / 1e9, 3),
+                    "note": "save = device download + file write; load = file read + upload (host records to "
+                            "device arrays)"}
+        log(f"[bench] snapshot {size / 1e9:.2f} GB: save {t_save:.2f} s, load {t_load:.2f} s")
     log(f"[bench] timed {args.steps} steps in {elapsed:.3f}s; emergency scenes: {int(em.sum())}")
     log("[bench] step ms: " + " ".join(f"{t:.1f}" for t in step_ms))
 
@@ -405,6 +428,8 @@ def main():
                           f"({n_steps} steps); weighted median step time under contention",
             }
             res["gpu_over_cpu_all_cores"] = round(scene_steps / rate_all, 2)
+    if snapshot:
+        res["snapshot"] = snapshot
     print(json.dumps(res), flush=True)
 
 

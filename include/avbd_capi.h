@@ -35,6 +35,7 @@
  *   avbd_step_scenes          -> (no counterpart: batched/served stepping of a
  *                                subset of resident scenes)
  *   avbd_server_*             -> (no counterpart: the served request loop)
+ *   avbd_snapshot_*           -> (no counterpart: world snapshot files)
  *   avbd_footprint, avbd_get_memory, avbd_set_option, avbd_tune_*,
  *   avbd_kernel_stats         -> (no counterpart: device memory, tuning and
  *                                measurement)
@@ -58,6 +59,7 @@ extern "C" {
 #define AVBD_E_STATE (-4)     /* call not valid in the current state */
 #define AVBD_E_NOGPU (-5)     /* no usable gfx950 device */
 #define AVBD_E_UNSUPPORTED (-6)
+#define AVBD_E_IO (-7)        /* file missing, unreadable, corrupt or of another format */
 
 /* body flags */
 #define AVBD_BODY_STATIC 1u
@@ -264,6 +266,29 @@ int avbd_get_counts(avbd_ctx* ctx, avbd_world* out);
 
 /* Copy the world back into caller arrays sized by avbd_get_counts. */
 int avbd_download(avbd_ctx* ctx, avbd_world* out);
+
+/* World snapshots (no reference counterpart: the on-disk form of the records
+ * above, for fixtures, reproducible configurations and server preload).
+ * One binary file: a 128-byte header (magic "AVBDSNAP", format version, ABI
+ * version, the size and count of each record kind, a 64-bit checksum of the
+ * record bytes) and the scenes, bodies, joints, springs, manifolds (contacts
+ * inline), tets and tris as arrays of the records of this header, each
+ * starting at a 64-byte aligned offset.  A snapshot is read only by a build
+ * with the same record layout; AVBD_E_IO for a missing, truncated, corrupt
+ * or foreign file. */
+#define AVBD_SNAPSHOT_VERSION 1
+int avbd_snapshot_write(const char* path, const avbd_world* w);
+/* Record counts of a snapshot (the n_* fields; pointers untouched). */
+int avbd_snapshot_counts(const char* path, avbd_world* counts);
+/* Reads a snapshot into caller arrays sized from avbd_snapshot_counts. */
+int avbd_snapshot_read(const char* path, avbd_world* out);
+/* The context's world (as avbd_download gives it) to a snapshot file. */
+int avbd_snapshot_save(avbd_ctx* ctx, const char* path);
+/* A snapshot file into the context (as avbd_upload with that world).  The
+ * loaded context steps as the saved one would have; manifolds it creates
+ * later may carry other creation serials (numbering restarts after the
+ * largest serial in the file), in the same order. */
+int avbd_snapshot_load(avbd_ctx* ctx, const char* path, int64_t manifold_capacity);
 
 /* Per-scene status words: emergency_stop and n_removed for `n` scenes. */
 int avbd_get_status(avbd_ctx* ctx, int32_t* emergency_stop, int32_t* n_removed, int32_t n);

@@ -72,6 +72,11 @@ def lib():
         L.avbd_server_download_scene.argtypes = [vp, C.c_int32, C.POINTER(AvbdWorld)]
         L.avbd_server_context.argtypes = [vp]
         L.avbd_server_context.restype = vp
+        L.avbd_snapshot_write.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
+        L.avbd_snapshot_counts.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
+        L.avbd_snapshot_read.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
+        L.avbd_snapshot_save.argtypes = [vp, C.c_char_p]
+        L.avbd_snapshot_load.argtypes = [vp, C.c_char_p, C.c_int64]
         _lib = L
     return _lib
 
@@ -91,6 +96,21 @@ def footprint(world: World, manifold_capacity: int = 0) -> int:
     out = C.c_int64()
     _check(lib().avbd_footprint(C.byref(cw), int(manifold_capacity), C.byref(out)))
     return int(out.value)
+
+
+def save_snapshot(world: World, path) -> None:
+    """Write `world` as a snapshot file (avbd_snapshot_write; host only)."""
+    keep, cw = world_to_c(world)
+    _check(lib().avbd_snapshot_write(os.fsencode(path), C.byref(cw)))
+
+
+def load_snapshot(path) -> World:
+    """Read a snapshot file into a World (avbd_snapshot_read; host only)."""
+    cnt = AvbdWorld()
+    _check(lib().avbd_snapshot_counts(os.fsencode(path), C.byref(cnt)))
+    w, cw, keep = alloc_world(cnt)
+    _check(lib().avbd_snapshot_read(os.fsencode(path), C.byref(cw)))
+    return w
 
 
 def _check(rc):
@@ -143,6 +163,14 @@ class Context:
     def upload(self, world: World, manifold_capacity: int = 0):
         keep, cw = world_to_c(world)
         _check(self.L.avbd_upload(self.h, C.byref(cw), int(manifold_capacity)))
+
+    def save_snapshot(self, path):
+        """The device world to a snapshot file (avbd_snapshot_save)."""
+        _check(self.L.avbd_snapshot_save(self.h, os.fsencode(path)))
+
+    def load_snapshot(self, path, manifold_capacity: int = 0):
+        """A snapshot file into this context (avbd_snapshot_load)."""
+        _check(self.L.avbd_snapshot_load(self.h, os.fsencode(path), int(manifold_capacity)))
 
     def step(self, n: int = 1):
         _check(self.L.avbd_step(self.h, int(n)))
