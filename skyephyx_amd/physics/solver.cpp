@@ -280,6 +280,13 @@ avbd_ctx* Solver::DeviceContext() {
     return ctx_;
 }
 
+void Solver::SaveSnapshot(const std::string& path) {
+    WorldRecords r;
+    ExportWorld(r);
+    avbd_world w = r.view();
+    if (avbd_snapshot_write(path.c_str(), &w) != 0) throw std::runtime_error(avbd_last_error());
+}
+
 void Solver::ExportWorld(WorldRecords& out) {
     RebuildPtrCaches();
     std::vector<unsigned char> sc;

@@ -124,6 +124,9 @@ public:
     // The host world as ABI records (bodies in solverIndex order, forces and
     // energies with their creation serials, contacts in creation order).
     void ExportWorld(WorldRecords& out);
+    // ExportWorld written as a world snapshot file (avbd_snapshot_write);
+    // throws std::runtime_error with the library's message on failure.
+    void SaveSnapshot(const std::string& path);
 
     // Used by the Force / Energy constructors and the Mesh destructor.
     int64_t TakeForceSerial();

@@ -142,6 +142,12 @@ int main(int argc, char** argv) {
             if (!build(s, argv[2])) return 2;
             return write_records(s, argv[3]);
         }
+        if (mode == "snapshot" && argc == 4) {
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            s.SaveSnapshot(argv[3]);
+            return 0;
+        }
         if (mode == "step" && argc == 7) {
             Solver s;
             if (!build(s, argv[2])) return 2;

@@ -19,7 +19,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from skyephyx_amd import scenes
+from skyephyx_amd import capi, scenes
 from skyephyx_amd.world import (BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, SCENE_DTYPE, SPRING_DTYPE, TET_DTYPE,
                                 TRI_DTYPE, World)
 
@@ -78,6 +78,17 @@ def _same(a, b, fields, where, rtol=0.0):
             np.testing.assert_allclose(a[f], b[f], rtol=rtol, atol=0, err_msg=f"{where}.{f}")
         else:
             np.testing.assert_array_equal(a[f], b[f], err_msg=f"{where}.{f}")
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_facade_snapshot_holds_its_records(driver, tmp_path, name):
+    """Solver::SaveSnapshot writes exactly the records ExportWorld gives."""
+    rec, snap = str(tmp_path / "w.bin"), str(tmp_path / "w.avbdsnap")
+    subprocess.run([driver, "records", name, rec], check=True)
+    subprocess.run([driver, "snapshot", name, snap], check=True)
+    c, s = read_records(rec), capi.load_snapshot(snap)
+    for arr in ("scenes", "bodies", "joints", "springs", "manifolds", "tets", "tris"):
+        assert getattr(c, arr).tobytes() == getattr(s, arr).tobytes(), arr
 
 
 @pytest.mark.parametrize("name", SCENES)
