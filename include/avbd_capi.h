@@ -293,13 +293,23 @@ int avbd_snapshot_load(avbd_ctx* ctx, const char* path, int64_t manifold_capacit
 /* Per-scene status words: emergency_stop and n_removed for `n` scenes. */
 int avbd_get_status(avbd_ctx* ctx, int32_t* emergency_stop, int32_t* n_removed, int32_t n);
 
+/* Solver::RemoveBody (solver.cpp:104-116, ~Mesh geometry.cpp:11-24) on the
+ * device-resident world, without a re-upload: bodies[k] are live body
+ * indices (the order avbd_download gives) naming bodies before this call;
+ * they are removed in the listed order, each swapped with the last live
+ * body of its scene, with their forces and energies (tets/tris that lose a
+ * vertex).  Other scenes keep every index.  avbd_last_removed then reports
+ * these removals (scene by scene in ascending scene order, each scene's in
+ * the listed order). */
+int avbd_remove_bodies(avbd_ctx* ctx, const int32_t* bodies, int32_t n);
+
 /* Bodies the last avbd_step / avbd_step_scenes call removed for leaving the
  * world bounds (Solver::RemoveBody from Step, solver.cpp:520-534,
- * 104-116), in removal order: the scene and the body's index within its
- * scene at the moment of its removal (its solverIndex then).  Applying
- * RemoveBody (swap with last) in this order to a host copy of the scene
- * reproduces the device's body order.  Writes at most cap entries; *n_out
- * receives the full count. */
+ * 104-116), or the last avbd_remove_bodies call removed, in removal order:
+ * the scene and the body's index within its scene at the moment of its
+ * removal (its solverIndex then).  Applying RemoveBody (swap with last) in
+ * this order to a host copy of the scene reproduces the device's body
+ * order.  Writes at most cap entries; *n_out receives the full count. */
 int avbd_last_removed(avbd_ctx* ctx, int32_t* scene, int32_t* index, int32_t cap, int32_t* n_out);
 
 /* Body state only, n = the live body count (avbd_get_counts), in body order:

@@ -23,6 +23,19 @@ struct Handle {
     World<double> wd;
 };
 
+// Solver::RemoveBody (solver.cpp:104-116) for the bodies at the listed
+// indices (indices before the call), in the listed order.
+template <class R>
+int remove_listed(World<R>& w, const int32_t* idx, int n) {
+    std::vector<Body<R>*> victims;
+    for (int k = 0; k < n; ++k) {
+        if (idx[k] < 0 || idx[k] >= int(w.bodies.size())) return -1;
+        victims.push_back(w.bodies[idx[k]].get());
+    }
+    for (Body<R>* b : victims) w.remove_body(b);
+    return 0;
+}
+
 template <class R>
 V3<R> v3(const float* p) { return V3<R>(R(p[0]), R(p[1]), R(p[2])); }
 template <class R>
@@ -405,6 +418,10 @@ int orc_step(void* hp, int n) {
         if (h->fp64) h->wd.step(); else h->wf.step();
     }
     return 0;
+}
+int orc_remove_bodies(void* hp, const int32_t* idx, int n) {
+    Handle* h = static_cast<Handle*>(hp);
+    return h->fp64 ? remove_listed(h->wd, idx, n) : remove_listed(h->wf, idx, n);
 }
 int orc_counts(void* hp, avbd_world* out) {
     Handle* h = static_cast<Handle*>(hp);

@@ -41,6 +41,7 @@ def lib():
         L.orc_load.argtypes = [C.c_void_p, C.POINTER(AvbdWorld), C.c_int]
         L.orc_set_order.argtypes = [C.c_void_p, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_remove_bodies.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int]
         L.orc_counts.argtypes = [C.c_void_p, C.POINTER(AvbdWorld)]
         L.orc_save.argtypes = [C.c_void_p, C.POINTER(AvbdWorld)]
         L.orc_colours.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int]
@@ -85,6 +86,12 @@ class Oracle:
 
     def set_order(self, order):
         self.L.orc_set_order(self.h, order)
+
+    def remove_bodies(self, idx):
+        """Solver::RemoveBody of the bodies at these indices (before the call), in order."""
+        a = np.ascontiguousarray(idx, np.int32)
+        if self.L.orc_remove_bodies(self.h, a.ctypes.data_as(C.POINTER(C.c_int32)), len(a)) != 0:
+            raise ValueError("body index out of range")
 
     def step(self, n=1):
         self.L.orc_step(self.h, n)

@@ -72,6 +72,7 @@ def lib():
         L.avbd_server_download_scene.argtypes = [vp, C.c_int32, C.POINTER(AvbdWorld)]
         L.avbd_server_context.argtypes = [vp]
         L.avbd_server_context.restype = vp
+        L.avbd_remove_bodies.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.avbd_snapshot_write.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
         L.avbd_snapshot_counts.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
         L.avbd_snapshot_read.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
@@ -195,6 +196,11 @@ class Context:
         _check(self.L.avbd_get_status(self.h, em.ctypes.data_as(C.POINTER(C.c_int32)),
                                       rem.ctypes.data_as(C.POINTER(C.c_int32)), n_scenes))
         return em, rem
+
+    def remove_bodies(self, bodies):
+        """Solver::RemoveBody of live bodies (indices before the call), in order."""
+        ids = np.ascontiguousarray(bodies, np.int32)
+        _check(self.L.avbd_remove_bodies(self.h, ids.ctypes.data_as(C.POINTER(C.c_int32)), len(ids)))
 
     def last_removed(self):
         """(scene, index at removal) of the bodies the last step call removed, in order."""

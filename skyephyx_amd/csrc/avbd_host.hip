@@ -874,40 +874,27 @@ int poststab_scenes(avbd_ctx* c, const std::vector<char>& only) {
     return 0;
 }
 
-// Out-of-bounds removal (solver.cpp:520-534) for the scenes k_velocity
-// flagged, then their post-stabilisation, which the step skipped for them.
-// The removal follows Solver::RemoveBody: the bodies that left the bounds, in
-// index order, each replaced by the scene's last body (swap with last,
-// solver.cpp:104-116); their forces and energies go with them (~Mesh,
-// geometry.cpp:11-24).  The device state is renumbered in place, so the
+// Solver::RemoveBody for lists of bodies: explicit_lists[s] holds, for each
+// scene marked in aff, its bodies to remove (device indices) in removal
+// order; without explicit lists, the bodies of the scenes marked in aff that
+// are outside their scene's bounds, in index order.  stw: the scenes' status
+// words (as downloaded; uploaded again here with the removal counts).  Each
+// removal swaps the body with the scene's last live body
+// (solver.cpp:104-116); its forces and energies go with it (~Mesh,
+// geometry.cpp:11-24).  The device state is renumbered in place, so a
 // step's own state (inertial targets, poses at step start, contact rows)
-// carries into the post-stabilisation.  A scene whose explosion check fired
-// removes nothing (the reference returns before, solver.cpp:528-529).
-int remove_out_of_bounds(avbd_ctx* c) {
+// carries into a post-stabilisation that follows.
+int remove_bodies(avbd_ctx* c, const std::vector<char>& aff, std::vector<int>& stw,
+                  const std::vector<std::vector<int>>* explicit_lists) {
     DevWorld& w = c->w;
     hipStream_t st = c->stream;
     const int S = w.S;
-    c->drop_graphs();
-    std::vector<int> stw;
     int rc;
-    if ((rc = download_vec(c, stw, w.st, (size_t)S * ST_WORDS))) return rc;
-    HIP_TRY(hipStreamSynchronize(st));
-    std::vector<char> aff(S, 0);
+    c->drop_graphs();
     std::vector<int> aff_ids;
-    for (int s = 0; s < S; ++s) {
-        int* sw = &stw[s * ST_WORDS];
-        if (sw[ST_OOB] > 0 && !sw[ST_EMERGENCY] && sw[ST_ACTIVE]) {
-            aff[s] = 1;
-            aff_ids.push_back(s);
-        }
-        sw[ST_OOB] = 0;
-    }
-    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_ANY_OOB, 0);
-    if (aff_ids.empty()) {
-        if ((rc = upload_vec(c, w.st, stw))) return rc;
-        HIP_TRY(hipStreamSynchronize(st));
-        return 0;
-    }
+    for (int s = 0; s < S; ++s)
+        if (aff[s]) aff_ids.push_back(s);
+    if (aff_ids.empty()) return 0;
     // Only the affected scenes' body ranges are read back and renumbered;
     // every other index stays where it is.
     struct Seg {
@@ -946,24 +933,30 @@ int remove_out_of_bounds(avbd_ctx* c) {
                 order.push_back(i);
             }
         const avbd_scene& sc = c->scenes[g.s];
-        for (int i = a; i < e; ++i) {
-            if (c->body_dead[i] || (g.flags[i - a] & AVBD_BODY_STATIC)) continue;
-            const float4 p = g.pos[i - a];
-            if (p.x < sc.bounds_min[0] || p.x > sc.bounds_max[0] || p.y < sc.bounds_min[1] || p.y > sc.bounds_max[1] ||
-                p.z < sc.bounds_min[2] || p.z > sc.bounds_max[2]) {
-                const int k = where[i - a], last = (int)order.size() - 1;
-                c->removed_scene.push_back(g.s);
-                c->removed_index.push_back(k);
-                if (k != last) {
-                    order[k] = order[last];
-                    where[order[k] - a] = k;
-                }
-                order.pop_back();
-                where[i - a] = -1;
-                gone.push_back(i);
+        auto remove = [&](int i) {
+            const int k = where[i - a], last = (int)order.size() - 1;
+            c->removed_scene.push_back(g.s);
+            c->removed_index.push_back(k);
+            if (k != last) {
+                order[k] = order[last];
+                where[order[k] - a] = k;
+            }
+            order.pop_back();
+            where[i - a] = -1;
+            gone.push_back(i);
+        };
+        if (explicit_lists) {
+            for (int i : (*explicit_lists)[g.s]) remove(i);
+        } else {
+            for (int i = a; i < e; ++i) {
+                if (c->body_dead[i] || (g.flags[i - a] & AVBD_BODY_STATIC)) continue;
+                const float4 p = g.pos[i - a];
+                if (p.x < sc.bounds_min[0] || p.x > sc.bounds_max[0] || p.y < sc.bounds_min[1] ||
+                    p.y > sc.bounds_max[1] || p.z < sc.bounds_min[2] || p.z > sc.bounds_max[2])
+                    remove(i);
             }
         }
-        stw[g.s * ST_WORDS + ST_REMOVED] += (int)gone.size();
+        if (!explicit_lists) stw[g.s * ST_WORDS + ST_REMOVED] += (int)gone.size();  // bounds removals only
         stw[g.s * ST_WORDS + ST_DIRTY] = DIRTY_FULL;
         g.n_live = (int)order.size();
         int k = a;
@@ -1105,6 +1098,38 @@ int remove_out_of_bounds(avbd_ctx* c) {
     if ((rc = upload_vec(c, w.st, stw))) return rc;
     if ((rc = build_incidence(c))) return rc;
     HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Out-of-bounds removal (solver.cpp:520-534) for the scenes k_velocity
+// flagged, then their post-stabilisation, which the step skipped for them.
+// A scene whose explosion check fired removes nothing (the reference returns
+// before, solver.cpp:528-529).
+int remove_out_of_bounds(avbd_ctx* c) {
+    DevWorld& w = c->w;
+    hipStream_t st = c->stream;
+    const int S = w.S;
+    std::vector<int> stw;
+    int rc;
+    if ((rc = download_vec(c, stw, w.st, (size_t)S * ST_WORDS))) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<char> aff(S, 0);
+    bool any = false;
+    for (int s = 0; s < S; ++s) {
+        int* sw = &stw[s * ST_WORDS];
+        if (sw[ST_OOB] > 0 && !sw[ST_EMERGENCY] && sw[ST_ACTIVE]) {
+            aff[s] = 1;
+            any = true;
+        }
+        sw[ST_OOB] = 0;
+    }
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_ANY_OOB, 0);
+    if (!any) {
+        if ((rc = upload_vec(c, w.st, stw))) return rc;
+        HIP_TRY(hipStreamSynchronize(st));
+        return 0;
+    }
+    if ((rc = remove_bodies(c, aff, stw, nullptr))) return rc;
     return poststab_scenes(c, aff);
 }
 
@@ -1748,6 +1773,40 @@ int avbd_get_status(avbd_ctx* c, int32_t* emergency, int32_t* removed, int32_t n
         if (removed) removed[s] = st[s * ST_WORDS + ST_REMOVED];
     }
     return 0;
+}
+
+int avbd_remove_bodies(avbd_ctx* c, const int32_t* bodies, int32_t n) {
+    if (!c || (!bodies && n > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    if (n < 0) return fail(AVBD_E_ARG, "negative body count");
+    HIP_TRY(hipSetDevice(c->device));
+    DevWorld& w = c->w;
+    const int S = w.S;
+    // live index -> device index
+    std::vector<int> dev;
+    dev.reserve(w.N);
+    for (int i = 0; i < w.N; ++i)
+        if (!c->body_dead[i]) dev.push_back(i);
+    std::vector<char> aff(S, 0), taken(w.N, 0);
+    std::vector<std::vector<int>> lists(S);
+    for (int k = 0; k < n; ++k) {
+        const int b = bodies[k];
+        if (b < 0 || b >= (int)dev.size()) return fail(AVBD_E_ARG, "body index out of range");
+        const int d = dev[b];
+        if (taken[d]) return fail(AVBD_E_ARG, "body listed twice");
+        taken[d] = 1;
+        int s = (int)(std::upper_bound(c->sc_off.begin(), c->sc_off.end(), d) - c->sc_off.begin()) - 1;
+        aff[s] = 1;
+        lists[s].push_back(d);
+    }
+    c->removed_scene.clear();
+    c->removed_index.clear();
+    if (n == 0) return 0;
+    std::vector<int> stw;
+    int rc;
+    if ((rc = download_vec(c, stw, w.st, (size_t)S * ST_WORDS))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return remove_bodies(c, aff, stw, &lists);
 }
 
 int avbd_last_removed(avbd_ctx* c, int32_t* scene, int32_t* index, int32_t cap, int32_t* n_out) {

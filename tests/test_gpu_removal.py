@@ -73,3 +73,88 @@ def test_removal_in_one_scene_leaves_the_others_alone():
         np.testing.assert_array_equal(sc.springs["body_b"], oob.springs["body_b"])
         np.testing.assert_array_equal(sc.manifolds["body_a"], oob.manifolds["body_a"])
         np.testing.assert_array_equal(sc.manifolds["body_b"], oob.manifolds["body_b"])
+
+
+def _emulate(n, ids):
+    """Body order after RemoveBody of ids (indices before the call), and the
+    index each had when it was removed."""
+    order = list(range(n))
+    at = []
+    for i in ids:
+        k = order.index(i)
+        at.append(k)
+        order[k] = order[-1]
+        order.pop()
+    return order, at
+
+
+def _warm(world, steps):
+    o = O.Oracle(world, fp64=False, order=O.NATURAL)
+    o.step(steps)
+    return o.save()
+
+
+EXPLICIT = {
+    # boxes with contacts, a joint chain and springs
+    "lattice_mix": (lambda: _warm(scenes.box_lattice(shape=(4, 3, 3), spacing=3.0, n_chain=12, n_springs=8,
+                                                     bounds=50.0), 20), lambda n: [5, 0, n - 1, 40]),
+    # FEM vertices: their tets go with them
+    "fem_block": (lambda: scenes.fem_block(n=5, cell=0.25), lambda n: [62, 0, 31]),
+    "pyramid": (lambda: _warm(scenes.pyramid_2d(), 30), lambda n: [n - 1, 10, 1]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EXPLICIT))
+def test_explicit_removal_matches_oracle(case):
+    """avbd_remove_bodies on the device-resident world equals the oracle's
+    RemoveBody sequence, and the next step keeps the parity bars."""
+    make, pick = EXPLICIT[case]
+    world = make()
+    ids = pick(len(world.bodies))
+    o = O.Oracle(world, fp64=False, order=O.COLOUR_MAJOR)
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(world)
+        ctx.remove_bodies(ids)
+        o.remove_bodies(ids)
+        g, ref = ctx.download(), o.save()
+        order, at = _emulate(len(world.bodies), ids)
+        assert ctx.last_removed() == [(0, k) for k in at]
+        assert _counts(g) == _counts(ref), (_counts(g), _counts(ref))
+        for f in ("pos", "rot", "vel", "ang_vel", "flags", "mass"):
+            np.testing.assert_array_equal(g.bodies[f], ref.bodies[f], err_msg=f)
+        np.testing.assert_array_equal(g.bodies["pos"], world.bodies["pos"][order])
+        np.testing.assert_array_equal(np.sort(g.tets["v"], axis=1), np.sort(ref.tets["v"], axis=1))
+        res = summary(g, ref)
+        assert res["pairs_equal"] and res["n_equal"] and res["ids_equal"], res
+        ctx.step(1)
+        o.step(1)
+        g, ref = ctx.download(), o.save()
+        assert _counts(g) == _counts(ref)
+        res = summary(g, ref)
+        assert res["pairs_equal"] and res["n_equal"] and res["ids_equal"], res
+        assert res["pos_rel"] <= POS_TOL and res["rot"] <= POS_TOL, res
+
+
+def test_explicit_removal_across_scenes():
+    """Removals in two scenes of one world, interleaved in the list: each
+    scene ends as its own oracle does; the scene between is untouched."""
+    a = _warm(scenes.pyramid_2d(rows=5), 20)
+    mid = scenes.joint_chain(4)
+    b = scenes.fem_block(n=4, cell=0.25)
+    world = World.concat([a, mid, b])
+    na, nm = len(a.bodies), len(mid.bodies)
+    ids = [na + nm + 5, 3, na + nm + 0, 7]  # b:5, a:3, b:0, a:7
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(world)
+        ctx.remove_bodies(ids)
+        g = ctx.download()
+        assert [s for s, _ in ctx.last_removed()] == [0, 0, 2, 2]
+    for scene, sub, local in ((0, a, [3, 7]), (2, b, [5, 0])):
+        o = O.Oracle(sub, fp64=False, order=O.COLOUR_MAJOR)
+        o.remove_bodies(local)
+        ref = o.save()
+        got = g.extract_scene(scene)
+        np.testing.assert_array_equal(got.bodies["pos"], ref.bodies["pos"])
+        assert len(got.manifolds) == len(ref.manifolds) and len(got.tets) == len(ref.tets)
+        np.testing.assert_array_equal(np.sort(got.tets["v"], axis=1), np.sort(ref.tets["v"], axis=1))
+    np.testing.assert_array_equal(g.extract_scene(1).bodies["pos"], mid.bodies["pos"])

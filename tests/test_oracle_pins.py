@@ -483,3 +483,36 @@ def test_explosion_stops_the_scene():
     before = s.bodies["pos"].copy()
     o.step(3)
     np.testing.assert_array_equal(o.save().bodies["pos"], before)
+
+
+# ----------------------------------------------------------- RemoveBody
+def test_oracle_remove_bodies_is_swap_with_last():
+    """RemoveBody (solver.cpp:104-116): each listed body (indices before the
+    call) is swapped with the current last body and dropped, with every
+    force and energy that touches it (~Mesh, geometry.cpp:11-24)."""
+    w = scenes.box_lattice(shape=(3, 2, 2), spacing=3.0, n_chain=4, n_springs=3, bounds=50.0)
+    o = O.Oracle(w)
+    o.step(8)
+    before = o.save()
+    ids = [3, 0, len(before.bodies) - 2]
+    o.remove_bodies(ids)
+    after = o.save()
+    # emulate the order: a list of original indices
+    order = list(range(len(before.bodies)))
+    for i in ids:
+        k = order.index(i)
+        order[k] = order[-1]
+        order.pop()
+    assert len(after.bodies) == len(order)
+    np.testing.assert_array_equal(after.bodies["pos"], before.bodies["pos"][order])
+    gone = set(ids)
+    keep = lambda a, b: a not in gone and b not in gone  # noqa: E731
+    assert len(after.joints) == sum(keep(int(j["body_a"]), int(j["body_b"])) for j in before.joints)
+    assert len(after.springs) == sum(keep(int(s["body_a"]), int(s["body_b"])) for s in before.springs)
+    assert len(after.manifolds) == sum(keep(int(m["body_a"]), int(m["body_b"])) for m in before.manifolds)
+    # surviving forces point at the same bodies (renumbered)
+    new_of = {old: new for new, old in enumerate(order)}
+    pairs_before = sorted((new_of[int(m["body_a"])], new_of[int(m["body_b"])]) for m in before.manifolds
+                          if keep(int(m["body_a"]), int(m["body_b"])))
+    pairs_after = sorted((int(m["body_a"]), int(m["body_b"])) for m in after.manifolds)
+    assert pairs_before == pairs_after
