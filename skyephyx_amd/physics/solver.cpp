@@ -85,7 +85,7 @@ void Solver::RebuildPtrCaches() {
 
 // ---------------------------------------------------------------- mutation
 void Solver::MarkStructureChanged() {
-    if (mirroring_) return;
+    if (mirroring_ || removingOnDevice_) return;
     // the host forces must be current before the world changes under them
     if (forcesStale_) SyncForces();
     structureDirty_ = true;
@@ -202,14 +202,33 @@ void Solver::RemoveEnergy(Energy* energy) {
 }
 
 void Solver::RemoveBody(Mesh* body) {
-    MarkStructureChanged();
     const int idx = body->solverIndex;
     const int last = static_cast<int>(solverBodies.size()) - 1;
+    // With the device world current, the device removes the body itself
+    // (avbd_remove_bodies) and the next Step needs no re-upload.
+    const bool on_device = ctx_ && uploaded_ && !structureDirty_ && !mirroring_;
+    if (on_device) {
+        if (forcesStale_) SyncForces();
+        const int32_t i = idx;
+        check(avbd_remove_bodies(ctx_, &i, 1), "avbd_remove_bodies");
+        avbd_body* rec = reinterpret_cast<avbd_body*>(lastBodies_.data());
+        const size_t n = lastBodies_.size() / sizeof(avbd_body);
+        if (n == solverBodies.size()) {
+            if (idx != last) rec[idx] = rec[last];
+            lastBodies_.resize((n - 1) * sizeof(avbd_body));
+        } else {
+            structureDirty_ = true;  // out of step: upload at the next Step
+        }
+        removingOnDevice_ = true;
+    } else {
+        MarkStructureChanged();
+    }
     if (idx != last) {
         std::swap(solverBodies[idx], solverBodies[last]);
         solverBodies[idx]->solverIndex = idx;
     }
     solverBodies.pop_back();  // ~Mesh removes the body's forces and energies
+    removingOnDevice_ = false;
     RebuildPtrCaches();
 }
 
@@ -401,6 +420,7 @@ void Solver::Upload() {
     avbd_world w = rec.view();
     EnsureContext();
     check(avbd_upload(ctx_, &w, 0), "avbd_upload");
+    ++uploads;
     uploaded_ = true;
     structureDirty_ = false;
     lastScene_.assign(reinterpret_cast<const unsigned char*>(&rec.scene),

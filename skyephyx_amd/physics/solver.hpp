@@ -115,6 +115,7 @@ public:
     Vector3f boundsMax = Vector3f(50.f, 50.f, 50.f);
     int device = 0;          // HIP device of the step
     bool syncForces = true;  // mirror forces and contacts into solverForces after every step
+    int64_t uploads = 0;     // full world uploads so far (a RemoveBody between steps needs none)
     // Bring solverForces (joints, springs, manifolds and their row state) up
     // to date with the device now.
     void SyncForces();
@@ -142,6 +143,7 @@ private:
     bool structureDirty_ = true;
     bool forcesStale_ = false;
     bool mirroring_ = false;  // applying device changes to the host objects
+    bool removingOnDevice_ = false;  // a RemoveBody the device applies itself
     int64_t nextForceSerial_ = 0;
     int32_t nextEnergySerial_ = 0;
     std::vector<unsigned char> lastScene_;   // scene record of the last upload

@@ -142,6 +142,22 @@ int main(int argc, char** argv) {
             if (!build(s, argv[2])) return 2;
             return write_records(s, argv[3]);
         }
+        if (mode == "remove" && argc >= 6) {
+            // remove <scene> <steps before> <steps after> <out> [body index ...]:
+            // the listed bodies (indices before the removal) are removed with
+            // RemoveBody between the steps
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            avbd_ctx* ctx = s.DeviceContext();
+            if (avbd_set_option(ctx, "lanes", 1) || avbd_set_option(ctx, "fem_lanes", 1)) return 3;
+            for (int k = 0, n = std::atoi(argv[3]); k < n; ++k) s.Step();
+            std::vector<Mesh*> victims;
+            for (int a = 6; a < argc; ++a) victims.push_back(s.bodyPtrs.at(std::atoi(argv[a])));
+            for (Mesh* m : victims) s.RemoveBody(m);
+            for (int k = 0, n = std::atoi(argv[4]); k < n; ++k) s.Step();
+            std::printf("uploads %lld\n", (long long)s.uploads);
+            return write_records(s, argv[5]);
+        }
         if (mode == "snapshot" && argc == 4) {
             Solver s;
             if (!build(s, argv[2])) return 2;

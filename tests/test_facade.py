@@ -172,3 +172,35 @@ def test_step_mirrors_bounds_removal(driver, tmp_path):
     assert "fast" not in names and "top" in names
     # swap with last: "top" (the last body) now sits where the first removed body was
     assert names.index("top") < len(names) - 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,ids", [("pyramid", [3, 0, 40]), ("fem_block", [62, 0, 31]),
+                                      ("spring_bridge", [2, 5])])
+def test_remove_body_between_steps_stays_on_the_device(driver, tmp_path, name, ids):
+    """Solver::RemoveBody between steps: the device removes the bodies
+    itself (avbd_remove_bodies), the next Step uploads nothing, and the
+    result equals the C ABI driven directly with the same removals."""
+    from skyephyx_amd.capi import Context
+
+    before, after = 6, 4
+    out = str(tmp_path / "r.bin")
+    r = subprocess.run([driver, "remove", name, str(before), str(after), out] + [str(i) for i in ids],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "uploads 1" in r.stdout, r.stdout
+    fac = read_records(out)
+    subprocess.run([driver, "records", name, str(tmp_path / "w.bin")], check=True)
+    built = read_records(str(tmp_path / "w.bin"))
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(built)
+        ctx.step(before)
+        ctx.remove_bodies(ids)
+        ctx.step(after)
+        ref = ctx.download()
+    assert len(fac.bodies) == len(ref.bodies) == len(built.bodies) - len(ids)
+    _same(fac.bodies, ref.bodies, ("pos", "rot", "vel", "ang_vel", "flags", "mass"), "bodies")
+    assert len(fac.manifolds) == len(ref.manifolds)
+    _same(fac.joints, ref.joints, ("body_a", "body_b", "serial", "flags", "lambda"), "joints")
+    _same(fac.springs, ref.springs, ("body_a", "body_b", "serial", "flags", "lambda"), "springs")
+    assert len(fac.tets) == len(ref.tets) and len(fac.tris) == len(ref.tris)
