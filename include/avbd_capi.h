@@ -27,6 +27,9 @@
  *                                CheckExplosion (solver.cpp:182-196)
  *   avbd_last_removed         -> the RemoveBody calls of Step's bounds check
  *                                (solver.cpp:520-534)
+ *   avbd_remove_bodies,
+ *   avbd_add_bodies           -> Solver::RemoveBody / AddBody between steps
+ *                                (solver.cpp:50-124), on the resident world
  *   avbd_download_bodies,
  *   avbd_upload_bodies        -> per-frame reads/writes of Mesh state
  *                                (geometry.hpp:148-256)
@@ -302,6 +305,13 @@ int avbd_get_status(avbd_ctx* ctx, int32_t* emergency_stop, int32_t* n_removed, 
  * these removals (scene by scene in ascending scene order, each scene's in
  * the listed order). */
 int avbd_remove_bodies(avbd_ctx* ctx, const int32_t* bodies, int32_t n);
+
+/* Solver::AddBody / AddParticle (solver.cpp:50-124) on the device-resident
+ * world, without a re-upload: each record (its .scene names the scene) is
+ * appended to its scene's body order, into a slot a removal freed.
+ * AVBD_E_CAPACITY when a scene has no such slot left (upload the world
+ * again).  Forces and energies are added with a full upload. */
+int avbd_add_bodies(avbd_ctx* ctx, const avbd_body* bodies, int32_t n);
 
 /* Bodies the last avbd_step / avbd_step_scenes call removed for leaving the
  * world bounds (Solver::RemoveBody from Step, solver.cpp:520-534,

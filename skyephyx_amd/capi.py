@@ -73,6 +73,7 @@ def lib():
         L.avbd_server_context.argtypes = [vp]
         L.avbd_server_context.restype = vp
         L.avbd_remove_bodies.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
+        L.avbd_add_bodies.argtypes = [vp, vp, C.c_int32]
         L.avbd_snapshot_write.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
         L.avbd_snapshot_counts.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
         L.avbd_snapshot_read.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
@@ -201,6 +202,12 @@ class Context:
         """Solver::RemoveBody of live bodies (indices before the call), in order."""
         ids = np.ascontiguousarray(bodies, np.int32)
         _check(self.L.avbd_remove_bodies(self.h, ids.ctypes.data_as(C.POINTER(C.c_int32)), len(ids)))
+
+    def add_bodies(self, bodies: np.ndarray):
+        """Solver::AddBody of body records (field scene names the scene), into
+        slots removals freed; AvbdError AVBD_E_CAPACITY when a scene has none."""
+        b = np.ascontiguousarray(bodies, BODY_DTYPE)
+        _check(self.L.avbd_add_bodies(self.h, b.ctypes.data, len(b)))
 
     def last_removed(self):
         """(scene, index at removal) of the bodies the last step call removed, in order."""

@@ -1809,6 +1809,57 @@ int avbd_remove_bodies(avbd_ctx* c, const int32_t* bodies, int32_t n) {
     return remove_bodies(c, aff, stw, &lists);
 }
 
+int avbd_add_bodies(avbd_ctx* c, const avbd_body* bodies, int32_t n) {
+    if (!c || (!bodies && n > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    if (n < 0) return fail(AVBD_E_ARG, "negative body count");
+    if (n == 0) return 0;
+    HIP_TRY(hipSetDevice(c->device));
+    DevWorld& w = c->w;
+    const int S = w.S;
+    // the slots: per scene its first dead slots (they trail the live bodies)
+    std::vector<int> next(S, -1), slots(n);
+    for (int k = 0; k < n; ++k) {
+        const int s = bodies[k].scene;
+        if (s < 0 || s >= S) return fail(AVBD_E_ARG, "body scene out of range");
+        const int a = c->sc_off[s], e = c->sc_off[s + 1];
+        if (next[s] < 0) {
+            next[s] = a;
+            while (next[s] < e && !c->body_dead[next[s]]) ++next[s];
+        }
+        if (next[s] >= e)
+            return fail(AVBD_E_CAPACITY, "scene has no free body slot (bodies can be added into slots left by "
+                                         "removals; otherwise upload the world again)");
+        slots[k] = next[s]++;
+    }
+    std::vector<int> stw;
+    int rc;
+    if ((rc = download_vec(c, stw, w.st, (size_t)S * ST_WORDS))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->drop_graphs();
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < n; ++k) {
+        const avbd_body& b = bodies[k];
+        const int i = slots[k];
+        // the record as avbd_upload converts it; the step's caches cleared
+        const float4 f4s[] = {f4(b.pos, 0.f), make_float4(b.rot[0], b.rot[1], b.rot[2], b.rot[3]), f4(b.scale, b.radius),
+                              f4(b.vel, b.mass), f4(b.ang_vel, b.friction), f4(b.prev_vel, 0.f),
+                              f4(b.prev_ang_vel, 0.f), f4(b.inertia, 0.f), z, z, z, z, z, z};
+        float4* dst[] = {w.pos, w.rot, w.scale, w.vel, w.angvel, w.pvel, w.pangvel, w.inertia,
+                         w.last_pos, w.last_rot, w.inert_pos, w.inert_rot, w.diff0, w.diff1};
+        for (int a = 0; a < 14; ++a)
+            HIP_TRY(hipMemcpyAsync(dst[a] + i, &f4s[a], sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(w.flags + i, &b.flags, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));  // the sources are on this frame
+        c->body_dead[i] = 0;
+        stw[b.scene * ST_WORDS + ST_DIRTY] = DIRTY_FULL;
+    }
+    if ((rc = upload_vec(c, w.st, stw))) return rc;
+    if ((rc = build_incidence(c))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int avbd_last_removed(avbd_ctx* c, int32_t* scene, int32_t* index, int32_t cap, int32_t* n_out) {
     if (!c || !n_out) return fail(AVBD_E_ARG, "null argument");
     const int n = (int)c->removed_index.size();

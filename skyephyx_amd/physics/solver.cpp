@@ -106,7 +106,14 @@ int32_t Solver::TakeEnergySerial() {
 Mesh* Solver::AddBody(ModelType modelType, float density, float friction, const Vector3f& position,
                       const Vector3f& scale, const Vector3f& velocity, const Quaternionf rotation,
                       const Vector3f& angularVelocity, bool isStatic, const Vector3f& color) {
-    MarkStructureChanged();
+    // With the device world current, the next Step adds the body on the
+    // device (avbd_add_bodies, into a slot a removal freed) if it can.
+    const bool on_device = ctx_ && uploaded_ && !structureDirty_ && !mirroring_;
+    if (on_device) {
+        if (forcesStale_) SyncForces();
+    } else {
+        MarkStructureChanged();
+    }
     auto mesh = std::make_unique<Mesh>(this, modelType, color);
     Mesh* raw = mesh.get();
     raw->density = density;
@@ -147,6 +154,7 @@ Mesh* Solver::AddBody(ModelType modelType, float density, float friction, const 
     }
     raw->solverIndex = static_cast<int>(solverBodies.size());
     solverBodies.push_back(std::move(mesh));
+    if (on_device) pendingAdds_.push_back(raw);
     RebuildPtrCaches();
     return raw;
 }
@@ -206,7 +214,7 @@ void Solver::RemoveBody(Mesh* body) {
     const int last = static_cast<int>(solverBodies.size()) - 1;
     // With the device world current, the device removes the body itself
     // (avbd_remove_bodies) and the next Step needs no re-upload.
-    const bool on_device = ctx_ && uploaded_ && !structureDirty_ && !mirroring_;
+    const bool on_device = ctx_ && uploaded_ && !structureDirty_ && !mirroring_ && pendingAdds_.empty();
     if (on_device) {
         if (forcesStale_) SyncForces();
         const int32_t i = idx;
@@ -421,6 +429,7 @@ void Solver::Upload() {
     EnsureContext();
     check(avbd_upload(ctx_, &w, 0), "avbd_upload");
     ++uploads;
+    pendingAdds_.clear();
     uploaded_ = true;
     structureDirty_ = false;
     lastScene_.assign(reinterpret_cast<const unsigned char*>(&rec.scene),
@@ -439,6 +448,23 @@ void Solver::Step() {
     if (sc != lastScene_) structureDirty_ = true;
     std::vector<avbd_body> bodies;
     BodyRecords(bodies);
+    if (!structureDirty_ && !pendingAdds_.empty()) {
+        // bodies added since the last step are the last ones
+        const size_t n_old = lastBodies_.size() / sizeof(avbd_body);
+        if (n_old + pendingAdds_.size() == bodies.size()) {
+            const int rc = avbd_add_bodies(ctx_, bodies.data() + n_old, (int32_t)pendingAdds_.size());
+            if (rc == AVBD_E_CAPACITY) {
+                structureDirty_ = true;  // no free slot: upload the world
+            } else {
+                check(rc, "avbd_add_bodies");
+                lastBodies_.insert(lastBodies_.end(), reinterpret_cast<const unsigned char*>(bodies.data() + n_old),
+                                   reinterpret_cast<const unsigned char*>(bodies.data() + bodies.size()));
+            }
+        } else {
+            structureDirty_ = true;
+        }
+    }
+    pendingAdds_.clear();
     bool state_changed = false;
     if (!structureDirty_) {
         const avbd_body* prev = reinterpret_cast<const avbd_body*>(lastBodies_.data());

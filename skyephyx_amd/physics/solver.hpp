@@ -115,7 +115,8 @@ public:
     Vector3f boundsMax = Vector3f(50.f, 50.f, 50.f);
     int device = 0;          // HIP device of the step
     bool syncForces = true;  // mirror forces and contacts into solverForces after every step
-    int64_t uploads = 0;     // full world uploads so far (a RemoveBody between steps needs none)
+    int64_t uploads = 0;     // full world uploads so far (RemoveBody, and AddBody into slots removals
+                             // freed, between steps need none)
     // Bring solverForces (joints, springs, manifolds and their row state) up
     // to date with the device now.
     void SyncForces();
@@ -144,6 +145,7 @@ private:
     bool forcesStale_ = false;
     bool mirroring_ = false;  // applying device changes to the host objects
     bool removingOnDevice_ = false;  // a RemoveBody the device applies itself
+    std::vector<Mesh*> pendingAdds_;  // bodies added since the last Step, for avbd_add_bodies
     int64_t nextForceSerial_ = 0;
     int32_t nextEnergySerial_ = 0;
     std::vector<unsigned char> lastScene_;   // scene record of the last upload

@@ -158,6 +158,27 @@ int main(int argc, char** argv) {
             std::printf("uploads %lld\n", (long long)s.uploads);
             return write_records(s, argv[5]);
         }
+        if (mode == "spawn" && argc >= 8) {
+            // spawn <scene> <steps before> <steps after> <out> <added out> <n add> [body index ...]:
+            // the listed bodies are removed, then n boxes are added (into the
+            // freed slots), between the steps; <added out> gets the records
+            // right after the additions
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            avbd_ctx* ctx = s.DeviceContext();
+            if (avbd_set_option(ctx, "lanes", 1) || avbd_set_option(ctx, "fem_lanes", 1)) return 3;
+            for (int k = 0, n = std::atoi(argv[3]); k < n; ++k) s.Step();
+            std::vector<Mesh*> victims;
+            for (int a = 8; a < argc; ++a) victims.push_back(s.bodyPtrs.at(std::atoi(argv[a])));
+            for (Mesh* m : victims) s.RemoveBody(m);
+            for (int k = 0, n = std::atoi(argv[7]); k < n; ++k)
+                s.AddBody(ModelType_Cube, 1.f, 0.5f, Vector3f(0.3f * k, 6.f + 1.5f * k, 0.2f), Vector3f(1.f, 1.f, 1.f),
+                          Vector3f::Zero(), Quaternionf::Identity(), Vector3f::Zero(), false);
+            if (write_records(s, argv[6])) return 5;
+            for (int k = 0, n = std::atoi(argv[4]); k < n; ++k) s.Step();
+            std::printf("uploads %lld\n", (long long)s.uploads);
+            return write_records(s, argv[5]);
+        }
         if (mode == "snapshot" && argc == 4) {
             Solver s;
             if (!build(s, argv[2])) return 2;

@@ -204,3 +204,31 @@ def test_remove_body_between_steps_stays_on_the_device(driver, tmp_path, name, i
     _same(fac.joints, ref.joints, ("body_a", "body_b", "serial", "flags", "lambda"), "joints")
     _same(fac.springs, ref.springs, ("body_a", "body_b", "serial", "flags", "lambda"), "springs")
     assert len(fac.tets) == len(ref.tets) and len(fac.tris) == len(ref.tris)
+
+
+@pytest.mark.gpu
+def test_spawn_into_freed_slots_stays_on_the_device(driver, tmp_path):
+    """RemoveBody, then AddBody of new boxes (SpawnRandomBox) between steps:
+    the boxes take the freed slots on the device, the next Step uploads
+    nothing, and the result equals the C ABI driven directly."""
+    from skyephyx_amd.capi import Context
+
+    before, after, ids, n_add = 6, 4, [3, 20], 2
+    out, added = str(tmp_path / "r.bin"), str(tmp_path / "a.bin")
+    r = subprocess.run([driver, "spawn", "pyramid", str(before), str(after), out, added, str(n_add)]
+                       + [str(i) for i in ids], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "uploads 1" in r.stdout, r.stdout
+    fac, mid = read_records(out), read_records(added)
+    subprocess.run([driver, "records", "pyramid", str(tmp_path / "w.bin")], check=True)
+    built = read_records(str(tmp_path / "w.bin"))
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(built)
+        ctx.step(before)
+        ctx.remove_bodies(ids)
+        ctx.add_bodies(mid.bodies[-n_add:])
+        ctx.step(after)
+        ref = ctx.download()
+    assert len(fac.bodies) == len(ref.bodies) == len(built.bodies) - len(ids) + n_add
+    _same(fac.bodies, ref.bodies, ("pos", "rot", "vel", "ang_vel", "flags", "mass"), "bodies")
+    assert len(fac.manifolds) == len(ref.manifolds)

@@ -158,3 +158,39 @@ def test_explicit_removal_across_scenes():
         assert len(got.manifolds) == len(ref.manifolds) and len(got.tets) == len(ref.tets)
         np.testing.assert_array_equal(np.sort(got.tets["v"], axis=1), np.sort(ref.tets["v"], axis=1))
     np.testing.assert_array_equal(g.extract_scene(1).bodies["pos"], mid.bodies["pos"])
+
+
+def test_add_bodies_into_freed_slots():
+    """avbd_add_bodies appends bodies to a scene's body order in the slots a
+    removal freed: the world then equals the oracle's after RemoveBody and
+    AddBody (a body appended to the records), and steps within the parity
+    bars.  A scene with no freed slot left refuses with AVBD_E_CAPACITY."""
+    world = _warm(scenes.pyramid_2d(), 30)
+    ids = [12, 30]
+    o = O.Oracle(world, fp64=False, order=O.COLOUR_MAJOR)
+    o.remove_bodies(ids)
+    removed = o.save()
+    new = world.bodies[[5, 6]].copy()  # two unit boxes, dropped above the pile
+    new["pos"] = [(0.3, 14.0, 0.0), (-0.4, 16.0, 0.0)]
+    for f in ("vel", "ang_vel", "prev_vel", "prev_ang_vel"):
+        new[f] = 0.0
+    expect = World(removed.scenes, np.concatenate([removed.bodies, new]), removed.joints, removed.springs,
+                   removed.manifolds, removed.tets, removed.tris)
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(world)
+        ctx.remove_bodies(ids)
+        ctx.add_bodies(new)
+        g = ctx.download()
+        assert len(g.bodies) == len(expect.bodies)
+        assert g.bodies.tobytes() == expect.bodies.tobytes()
+        with pytest.raises(Exception) as e:
+            ctx.add_bodies(new[:1])  # both freed slots are taken
+        assert getattr(e.value, "code", None) == -3
+        ref = O.Oracle(expect, fp64=False, order=O.COLOUR_MAJOR)
+        for _ in range(3):
+            ctx.step(1)
+            ref.step(1)
+            g, r = ctx.download(), ref.save()
+            res = summary(g, r)
+            assert res["pairs_equal"] and res["n_equal"] and res["ids_equal"], res
+            assert res["pos_rel"] <= POS_TOL and res["rot"] <= POS_TOL, res
