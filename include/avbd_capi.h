@@ -39,6 +39,9 @@
  *                                subset of resident scenes)
  *   avbd_server_*             -> (no counterpart: the served request loop)
  *   avbd_snapshot_*           -> (no counterpart: world snapshot files)
+ *   avbd_get_debug_lines/_points,
+ *   avbd_set_body_colours     -> Solver::lineData / debugPointData
+ *                                (solver.hpp:36-37), Mesh::color
  *   avbd_footprint, avbd_get_memory, avbd_set_option, avbd_tune_*,
  *   avbd_kernel_stats         -> (no counterpart: device memory, tuning and
  *                                measurement)
@@ -296,6 +299,34 @@ int avbd_snapshot_load(avbd_ctx* ctx, const char* path, int64_t manifold_capacit
 /* Per-scene status words: emergency_stop and n_removed for `n` scenes. */
 int avbd_get_status(avbd_ctx* ctx, int32_t* emergency_stop, int32_t* n_removed, int32_t n);
 
+/* Debug/render output of a step (option "debug_output"): the line and point
+ * data Solver::Step collects for the renderer (Solver::lineData /
+ * debugPointData, solver.hpp:36-37, filled at solver.cpp:266-313 by
+ * Manifold/Joint/Spring/NeoHookeanFEM/STVKFEM::AddLineData and
+ * Manifold::AddDebugPointData), as GPULineData / GPUDebugPointData records
+ * (geometry.hpp:48-60).  Made on the device from the step-start poses after
+ * each step (before its bounds removal); for scenes that ran.  Lines come
+ * grouped as joints, springs, contacts, tet edges, triangle edges (the
+ * reference holds the same set in the order of its force and energy
+ * vectors).  An element edge takes the mean colour of its two bodies
+ * (Mesh::color, set with avbd_set_body_colours after an upload; white
+ * otherwise) and alpha 1. */
+typedef struct avbd_line {
+    float start[3];
+    float end[3];
+    float color[4];
+} avbd_line;
+typedef struct avbd_debug_point {
+    float position[3];
+    float pad;
+    float color[4];
+} avbd_debug_point;
+/* rgb: 3 floats per live body (avbd_download's order) */
+int avbd_set_body_colours(avbd_ctx* ctx, const float* rgb, int32_t n);
+/* The last step's lines / points: copies at most cap, *n = the full count. */
+int avbd_get_debug_lines(avbd_ctx* ctx, avbd_line* out, int64_t cap, int64_t* n);
+int avbd_get_debug_points(avbd_ctx* ctx, avbd_debug_point* out, int64_t cap, int64_t* n);
+
 /* Solver::RemoveBody (solver.cpp:104-116, ~Mesh geometry.cpp:11-24) on the
  * device-resident world, without a re-upload: bodies[k] are live body
  * indices (the order avbd_download gives) naming bodies before this call;
@@ -361,6 +392,8 @@ int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
  *                    its own stream beside the bucket launches) when "lanes"
  *                    equals "fem_lanes"; 0 (default) = every scene on the
  *                    per-colour bucket launches.  Results are identical.
+ *   "debug_output"   1 = after every step, make its line and point data
+ *                    (avbd_get_debug_lines / _points); 0 (default) = none.
  *   "colour_updates" 1 (default) = a scene whose constraint graph gained or
  *                    lost edges since its last colouring re-evaluates only
  *                    the bodies the change can reach; 0 = every recolouring

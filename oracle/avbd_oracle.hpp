@@ -912,6 +912,56 @@ struct World {
     int nColours = 0;
     Timings timings;
 
+    // The renderer's line and point data of the last step (Solver::lineData /
+    // debugPointData, solver.hpp:36-37), GPULineData / GPUDebugPointData
+    // layouts (geometry.hpp:48-60): 10 and 8 floats, FP32 as the reference.
+    std::vector<float> lineData, pointData;
+    void add_line(const V3<R>& a, const V3<R>& b, float r, float g, float bl, float al) {
+        const float v[10] = {float(a[0]), float(a[1]), float(a[2]), float(b[0]), float(b[1]), float(b[2]), r, g, bl, al};
+        lineData.insert(lineData.end(), v, v + 10);
+    }
+    void add_point(const V3<R>& p, float r, float g, float bl) {
+        const float v[8] = {float(p[0]), float(p[1]), float(p[2]), 0.f, r, g, bl, 1.f};
+        pointData.insert(pointData.end(), v, v + 8);
+    }
+    // Force::AddLineData / AddDebugPointData after a successful Initialize
+    // (solver.cpp:276-278): manifold.cpp:259-302, joint.cpp:176-190,
+    // spring.cpp:112-125.
+    void force_debug(Force<R>* f) {
+        if (f->kind == ForceKind::Manifold) {
+            auto* m = static_cast<Manifold<R>*>(f);
+            for (int i = 0; i < m->n; ++i) {
+                const Contact<R>& c = m->contacts[i];
+                if (c.penetration <= R(0)) continue;
+                add_line(c.position, c.position + c.normal * c.penetration, 1.f, 0.f, 0.f, 1.f);
+            }
+            const M3<R> RA = rotmat(m->A->rot), RB = rotmat(m->B->rot);
+            for (int i = 0; i < m->n; ++i) {
+                add_point(m->A->pos + mv(RA, m->contacts[i].rA), 1.f, 1.f, 0.f);
+                add_point(m->B->pos + mv(RB, m->contacts[i].rB), 0.f, 1.f, 1.f);
+            }
+        } else if (f->kind == ForceKind::Joint) {
+            auto* j = static_cast<Joint<R>*>(f);
+            add_line(j->A ? j->A->transform_point(j->rA) : j->rA, j->B->transform_point(j->rB), 0.75f, 0.f, 0.f, 1.f);
+        } else {
+            auto* sp = static_cast<Spring<R>*>(f);
+            add_line(sp->A ? sp->A->transform_point(sp->rA) : sp->rA, sp->B->transform_point(sp->rB), 0.f, 1.f, 0.f,
+                     1.f);
+        }
+    }
+    // Energy::AddLineData (solver.cpp:312): the element's edges, coloured
+    // with the mean of the bodies' colours (white here; alpha 1, which the
+    // reference leaves unset); neohookeanfem.cpp:113-133, stvkfem.cpp:89-105.
+    void energy_debug(Energy<R>* e) {
+        if (auto* t = dynamic_cast<NeoHookeanTet<R>*>(e)) {
+            const int ed[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+            for (auto& p : ed) add_line(t->v[p[0]]->pos, t->v[p[1]]->pos, 1.f, 1.f, 1.f, 1.f);
+        } else if (auto* r = dynamic_cast<StvkTri<R>*>(e)) {
+            const int ed[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+            for (auto& p : ed) add_line(r->v[p[0]]->pos, r->v[p[1]]->pos, 1.f, 1.f, 1.f, 1.f);
+        }
+    }
+
     void remove_force(Force<R>* f) {  // Solver::RemoveForce (solver.cpp:143-155)
         int idx = f->index;
         if (idx < 0) return;
@@ -1093,8 +1143,11 @@ struct World {
             std::vector<Force<R>*> fl;
             for (auto& f : forces) fl.push_back(f.get());
             std::vector<Force<R>*> dead;
+            lineData.clear();
+            pointData.clear();
             for (Force<R>* f : fl) {
                 if (!f->initialize()) { dead.push_back(f); continue; }
+                force_debug(f);
                 const int nr = f->num_rows();
                 for (int i = 0; i < nr; ++i) {
                     Row<R>& r = f->rows[i];
@@ -1108,7 +1161,9 @@ struct World {
             for (Force<R>* f : dead) remove_force(f);
         }
         // 3. energies: Initialize() is always true and the warm-start
-        //    ComputeEnergyTerms(nullptr) result is overwritten before use.
+        //    ComputeEnergyTerms(nullptr) result is overwritten before use;
+        //    each adds its edges to the line data.
+        for (auto& e : energies) energy_debug(e.get());
         timings.warm = ms(t0);
         // 4. prediction
         t0 = Clock::now();

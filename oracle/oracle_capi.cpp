@@ -423,6 +423,18 @@ int orc_remove_bodies(void* hp, const int32_t* idx, int n) {
     Handle* h = static_cast<Handle*>(hp);
     return h->fp64 ? remove_listed(h->wd, idx, n) : remove_listed(h->wf, idx, n);
 }
+// Line / point data of the last step: copies at most cap records (10 / 8
+// floats each), writes the full counts.
+int orc_debug(void* hp, float* lines, int64_t lcap, float* points, int64_t pcap, int64_t* nl, int64_t* np) {
+    Handle* h = static_cast<Handle*>(hp);
+    const std::vector<float>& L = h->fp64 ? h->wd.lineData : h->wf.lineData;
+    const std::vector<float>& P = h->fp64 ? h->wd.pointData : h->wf.pointData;
+    *nl = (int64_t)L.size() / 10;
+    *np = (int64_t)P.size() / 8;
+    if (lines) std::copy(L.begin(), L.begin() + std::min<int64_t>(lcap, *nl) * 10, lines);
+    if (points) std::copy(P.begin(), P.begin() + std::min<int64_t>(pcap, *np) * 8, points);
+    return 0;
+}
 int orc_counts(void* hp, avbd_world* out) {
     Handle* h = static_cast<Handle*>(hp);
     if (h->fp64) counts(h->wd, out); else counts(h->wf, out);

@@ -42,6 +42,8 @@ def lib():
         L.orc_set_order.argtypes = [C.c_void_p, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_remove_bodies.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int]
+        L.orc_debug.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.POINTER(C.c_int64),
+                                C.POINTER(C.c_int64)]
         L.orc_counts.argtypes = [C.c_void_p, C.POINTER(AvbdWorld)]
         L.orc_save.argtypes = [C.c_void_p, C.POINTER(AvbdWorld)]
         L.orc_colours.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int]
@@ -92,6 +94,16 @@ class Oracle:
         a = np.ascontiguousarray(idx, np.int32)
         if self.L.orc_remove_bodies(self.h, a.ctypes.data_as(C.POINTER(C.c_int32)), len(a)) != 0:
             raise ValueError("body index out of range")
+
+    def debug(self):
+        """(lines [n, 10], points [n, 8]) of the last step (the renderer's data)."""
+        nl, npt = C.c_int64(), C.c_int64()
+        self.L.orc_debug(self.h, None, 0, None, 0, C.byref(nl), C.byref(npt))
+        lines = np.zeros((nl.value, 10), np.float32)
+        points = np.zeros((npt.value, 8), np.float32)
+        self.L.orc_debug(self.h, lines.ctypes.data, nl.value, points.ctypes.data, npt.value, C.byref(nl),
+                         C.byref(npt))
+        return lines, points
 
     def step(self, n=1):
         self.L.orc_step(self.h, n)

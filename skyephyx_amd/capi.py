@@ -74,6 +74,9 @@ def lib():
         L.avbd_server_context.restype = vp
         L.avbd_remove_bodies.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.avbd_add_bodies.argtypes = [vp, vp, C.c_int32]
+        L.avbd_set_body_colours.argtypes = [vp, vp, C.c_int32]
+        L.avbd_get_debug_lines.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+        L.avbd_get_debug_points.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
         L.avbd_snapshot_write.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
         L.avbd_snapshot_counts.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
         L.avbd_snapshot_read.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
@@ -208,6 +211,23 @@ class Context:
         slots removals freed; AvbdError AVBD_E_CAPACITY when a scene has none."""
         b = np.ascontiguousarray(bodies, BODY_DTYPE)
         _check(self.L.avbd_add_bodies(self.h, b.ctypes.data, len(b)))
+
+    def set_body_colours(self, rgb):
+        """Mesh::color of every live body ([n, 3]), for the element edge lines."""
+        a = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        _check(self.L.avbd_set_body_colours(self.h, a.ctypes.data, len(a)))
+
+    def debug_output(self):
+        """(lines [n, 10], points [n, 8]) of the last step (option debug_output):
+        start xyz, end xyz, rgba / position xyz, 0, rgba."""
+        n = C.c_int64()
+        _check(self.L.avbd_get_debug_lines(self.h, None, 0, C.byref(n)))
+        lines = np.zeros((n.value, 10), np.float32)
+        _check(self.L.avbd_get_debug_lines(self.h, lines.ctypes.data, n.value, C.byref(n)))
+        _check(self.L.avbd_get_debug_points(self.h, None, 0, C.byref(n)))
+        points = np.zeros((n.value, 8), np.float32)
+        _check(self.L.avbd_get_debug_points(self.h, points.ctypes.data, n.value, C.byref(n)))
+        return lines, points
 
     def last_removed(self):
         """(scene, index at removal) of the bodies the last step call removed, in order."""

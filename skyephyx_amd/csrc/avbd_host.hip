@@ -80,6 +80,25 @@ struct avbd_ctx {
     // a scene whose constraint graph changed a little updates its last
     // colouring from the changed bodies instead of colouring from scratch
     bool colour_updates = true;
+    // Debug/render output (option debug_output): counters, offsets and the
+    // contact points come from alloc (an upload frees them; set up again on
+    // the next use), the output arrays are grown here and owned here.
+    bool dbg_on = false;
+    struct Debug {
+        int* lc = nullptr;
+        int* loff = nullptr;
+        int* pc = nullptr;
+        int* poff = nullptr;
+        void* cub = nullptr;
+        size_t cub_bytes = 0;
+        float4* rgb = nullptr;  // per body colour (avbd_set_body_colours), null = white
+        float* lines = nullptr;
+        size_t lcap = 0;
+        float* points = nullptr;
+        size_t pcap = 0;
+        long long nl = 0, np = 0;
+        bool ready = false;
+    } dbg;
     std::vector<int> scene_bodies;      // per scene, at upload
     int max_scene_bodies = 0;
     std::vector<char> scene_breakable;  // per scene: a joint or spring with a finite fracture
@@ -221,6 +240,9 @@ struct avbd_ctx {
         n_items = n_tiles = 0;
         cub_tmp = nullptr;
         have_world = false;
+        if (dbg.lines) hipFree(dbg.lines);
+        if (dbg.points) hipFree(dbg.points);
+        dbg = Debug{};
     }
 };
 
@@ -1071,6 +1093,10 @@ int remove_bodies(avbd_ctx* c, const std::vector<char>& aff, std::vector<int>& s
             hipLaunchKernelGGL(k_gather_f4, dim3(grid_for(K)), dim3(256), 0, st, d_tmp, arr, d_src, K);
             hipLaunchKernelGGL(k_scatter_f4, dim3(grid_for(K)), dim3(256), 0, st, arr, d_tmp, d_dst, K);
         }
+        if (c->dbg.rgb) {
+            hipLaunchKernelGGL(k_gather_f4, dim3(grid_for(K)), dim3(256), 0, st, d_tmp, c->dbg.rgb, d_src, K);
+            hipLaunchKernelGGL(k_scatter_f4, dim3(grid_for(K)), dim3(256), 0, st, c->dbg.rgb, d_tmp, d_dst, K);
+        }
         uint32_t* u32[] = {w.flags, reinterpret_cast<uint32_t*>(w.scene)};
         uint32_t* t32 = reinterpret_cast<uint32_t*>(d_tmp);
         for (uint32_t* arr : u32) {
@@ -1473,6 +1499,10 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
         c->fused_max = int(value);
         return 0;
     }
+    if (n == "debug_output") {
+        c->dbg_on = value != 0;
+        return 0;
+    }
     if (n == "colour_updates") {
         c->colour_updates = value != 0;
         return 0;
@@ -1538,6 +1568,68 @@ int avbd_upload(avbd_ctx* c, const avbd_world* w, int64_t manifold_capacity) {
     return do_upload(c, w, manifold_capacity);
 }
 
+// Debug output buffers for the current world (see DebugOut).
+static int ensure_debug(avbd_ctx* c) {
+    auto& d = c->dbg;
+    if (d.ready) return 0;
+    DevWorld& w = c->w;
+    const size_t slots = (size_t)w.J + w.SP + w.Mcap + w.T + w.TR;
+    int rc;
+    if ((rc = c->alloc(&d.lc, slots + 1)) || (rc = c->alloc(&d.loff, slots + 1)) ||
+        (rc = c->alloc(&d.pc, (size_t)w.Mcap + 1)) || (rc = c->alloc(&d.poff, (size_t)w.Mcap + 1)))
+        return rc;
+    for (ManifoldBuf& m : c->mb)
+        if (!m.cpos && (rc = c->alloc(&m.cpos, (size_t)w.Mcap * 8))) return rc;
+    w.m = c->mb[c->cur];
+    size_t b1 = 0, b2 = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, d.lc, d.loff, (int)slots, c->stream));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, d.pc, d.poff, w.Mcap, c->stream));
+    d.cub_bytes = std::max<size_t>(std::max(b1, b2), 1);
+    unsigned char* cub;
+    if ((rc = c->alloc(&cub, d.cub_bytes))) return rc;
+    d.cub = cub;
+    d.ready = true;
+    return 0;
+}
+
+// The step's line and point data (k_dbg_count / k_dbg_write), after the step
+// and before its bounds removal.
+static int extract_debug(avbd_ctx* c) {
+    int rc;
+    if ((rc = ensure_debug(c))) return rc;
+    auto& d = c->dbg;
+    DevWorld& w = c->w;
+    hipStream_t st = c->stream;
+    const int slots = w.J + w.SP + w.Mcap + w.T + w.TR;
+    hipLaunchKernelGGL(k_dbg_count, dim3(grid_for(slots)), dim3(256), 0, st, w, d.lc, d.pc);
+    size_t bytes = d.cub_bytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(d.cub, bytes, d.lc, d.loff, slots, st));
+    bytes = d.cub_bytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(d.cub, bytes, d.pc, d.poff, w.Mcap, st));
+    int tail[4];
+    HIP_TRY(hipMemcpyAsync(&tail[0], d.loff + slots - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&tail[1], d.lc + slots - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&tail[2], d.poff + w.Mcap - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&tail[3], d.pc + w.Mcap - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    d.nl = (long long)tail[0] + tail[1];
+    d.np = (long long)tail[2] + tail[3];
+    auto grow = [&](float*& buf, size_t& cap, size_t need) -> int {
+        if (need <= cap) return 0;
+        if (buf) HIP_TRY(hipFree(buf));
+        buf = nullptr;
+        cap = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&buf), sizeof(float) * std::max<size_t>(need, 1)));
+        cap = need;
+        return 0;
+    };
+    if ((rc = grow(d.lines, d.lcap, (size_t)d.nl * 10)) || (rc = grow(d.points, d.pcap, (size_t)d.np * 8))) return rc;
+    const DebugOut out{d.lines, d.points, d.loff, d.poff, d.rgb};
+    hipLaunchKernelGGL(k_dbg_write, dim3(grid_for(slots)), dim3(256), 0, st, w, out, (const int*)d.lc, (const int*)d.pc);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 static int step_impl(avbd_ctx* c, int32_t n_steps) {
     if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
     if (n_steps < 0) return fail(AVBD_E_ARG, "negative step count");
@@ -1546,13 +1638,17 @@ static int step_impl(avbd_ctx* c, int32_t n_steps) {
     c->removed_scene.clear();
     c->removed_index.clear();
     for (int k = 0; k < n_steps; ++k) {
-        int rc = do_step(c);
+        int rc;
+        // the contact points are recorded by the step's warm start
+        if (c->dbg_on && (rc = ensure_debug(c))) return rc;
+        rc = do_step(c);
         if (rc) return rc;
         int g[G_WORDS];
         HIP_TRY(hipMemcpyAsync(g, c->w.g, sizeof(g), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         c->collect_kernel_times();
         c->tuner.collect();
+        if (c->dbg_on && (rc = extract_debug(c))) return rc;
         if (g[G_ANY_OOB] && (rc = remove_out_of_bounds(c))) return rc;
         acc.broad_phase_ms = elapsed(c->ev[0], c->ev[1]);
         acc.warmstart_ms = elapsed(c->ev[1], c->ev[2]);
@@ -1850,6 +1946,8 @@ int avbd_add_bodies(avbd_ctx* c, const avbd_body* bodies, int32_t n) {
         for (int a = 0; a < 14; ++a)
             HIP_TRY(hipMemcpyAsync(dst[a] + i, &f4s[a], sizeof(float4), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(w.flags + i, &b.flags, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        const float4 white = make_float4(1.f, 1.f, 1.f, 0.f);
+        if (c->dbg.rgb) HIP_TRY(hipMemcpyAsync(c->dbg.rgb + i, &white, sizeof(float4), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));  // the sources are on this frame
         c->body_dead[i] = 0;
         stw[b.scene * ST_WORDS + ST_DIRTY] = DIRTY_FULL;
@@ -1857,6 +1955,44 @@ int avbd_add_bodies(avbd_ctx* c, const avbd_body* bodies, int32_t n) {
     if ((rc = upload_vec(c, w.st, stw))) return rc;
     if ((rc = build_incidence(c))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int avbd_set_body_colours(avbd_ctx* c, const float* rgb, int32_t n) {
+    if (!c || (!rgb && n > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    DevWorld& w = c->w;
+    std::vector<float4> col(w.N, make_float4(1.f, 1.f, 1.f, 0.f));
+    int k = 0;
+    for (int i = 0; i < w.N; ++i)
+        if (!c->body_dead[i]) {
+            if (k >= n) return fail(AVBD_E_ARG, "colour count differs from the live body count");
+            col[i] = make_float4(rgb[3 * k], rgb[3 * k + 1], rgb[3 * k + 2], 0.f);
+            ++k;
+        }
+    if (k != n) return fail(AVBD_E_ARG, "colour count differs from the live body count");
+    int rc;
+    if (!c->dbg.rgb && (rc = c->alloc(&c->dbg.rgb, (size_t)w.N))) return rc;
+    if ((rc = upload_vec(c, c->dbg.rgb, col))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int avbd_get_debug_lines(avbd_ctx* c, avbd_line* out, int64_t cap, int64_t* n) {
+    if (!c || !n || (!out && cap > 0)) return fail(AVBD_E_ARG, "null argument");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *n = c->dbg.nl;
+    const long long k = std::min<long long>(cap, c->dbg.nl);
+    if (k > 0) HIP_TRY(hipMemcpy(out, c->dbg.lines, sizeof(avbd_line) * (size_t)k, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int avbd_get_debug_points(avbd_ctx* c, avbd_debug_point* out, int64_t cap, int64_t* n) {
+    if (!c || !n || (!out && cap > 0)) return fail(AVBD_E_ARG, "null argument");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *n = c->dbg.np;
+    const long long k = std::min<long long>(cap, c->dbg.np);
+    if (k > 0) HIP_TRY(hipMemcpy(out, c->dbg.points, sizeof(avbd_debug_point) * (size_t)k, hipMemcpyDeviceToHost));
     return 0;
 }
 

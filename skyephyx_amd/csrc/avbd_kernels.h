@@ -40,3 +40,5 @@ __global__ void k_remap_forces(DevWorld w, BodyRemap rm);
 __global__ void k_remap_manifolds(DevWorld w, BodyRemap rm);
 __global__ void k_kill_bodies(DevWorld w, const int* list, int n);
 __global__ void k_remap_elements(DevWorld w, BodyRemap rm);
+__global__ void k_dbg_count(DevWorld w, int* lc, int* pc);
+__global__ void k_dbg_write(DevWorld w, DebugOut d, const int* lc, const int* pc);

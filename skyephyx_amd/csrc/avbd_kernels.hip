@@ -392,6 +392,7 @@ __global__ void k_manifold_init(DevWorld w) {
             cold[i * CONTACT_COLD_F4 + 0] = make_float4(__uint_as_float(c.id), (float)mstick[i], c.rA.x, c.rA.y);
             cold[i * CONTACT_COLD_F4 + 1] = make_float4(c.rA.z, c.rB.x, c.rB.y, c.rB.z);
             cold[i * CONTACT_COLD_F4 + 2] = v4(c.normal, c.penetration);
+            if (w.m.cpos) w.m.cpos[(size_t)m * 8 + i] = v4(c.position, 0.f);
         }
     }
 }
@@ -477,6 +478,8 @@ __global__ void k_compact(DevWorld w, ManifoldBuf dst) {
         const float4* cs = w.m.cold + (size_t)m * 8 * CONTACT_COLD_F4;
         float4* cd = dst.cold + (size_t)d * 8 * CONTACT_COLD_F4;
         for (int k = 0; k < n * CONTACT_COLD_F4; ++k) cd[k] = cs[k];
+        if (dst.cpos && w.m.cpos)
+            for (int k = 0; k < n; ++k) dst.cpos[(size_t)d * 8 + k] = w.m.cpos[(size_t)m * 8 + k];
     }
 }
 
@@ -1982,6 +1985,131 @@ __global__ void k_remap_manifolds(DevWorld w, BodyRemap rm) {
         if (keep) {
             w.m.a[m] = na;
             w.m.b[m] = nb;
+        }
+    }
+}
+
+// ------------------------------------------------ debug / render output
+// The line and point data the reference's Step collects at its warm start
+// (solver.cpp:266-313): per force after Initialize, Manifold::AddLineData
+// (one line per contact with positive penetration, from the contact point
+// along the normal, red; manifold.cpp:259-277), Manifold::AddDebugPointData
+// (both contact points, A yellow, B cyan; manifold.cpp:280-302),
+// Joint::AddLineData (joint.cpp:176-190, dark red) and Spring::AddLineData
+// (spring.cpp:112-125, green) between the world anchor points; per energy,
+// the element's edges coloured with the mean of their bodies' colours
+// (neohookeanfem.cpp:113-133: six edges, stvkfem.cpp:89-105: three).  Run
+// after the step, before the bounds removal: the poses are the step-start
+// ones (last_pos / last_rot), the manifolds those the warm start kept, a
+// joint or spring that broke during this step still draws.  Scenes that did
+// not run are skipped.  Slots are joints, springs, manifolds (Mcap), tets,
+// tris; lines come out in that order (the reference draws the same set in
+// its force-vector order).
+AVBD_DEV v3 start_point(const DevWorld& w, int b, v3 r) {  // Transform::TransformPoint at step start
+    const v3 s = f3(w.scale[b]);
+    return add(qrot(fq(w.last_rot[b]), v3{r.x * s.x, r.y * s.y, r.z * s.z}), f3(w.last_pos[b]));
+}
+
+AVBD_DEV bool dbg_force_alive(const DevWorld& w, uint32_t flags, int b) {
+    return !(flags & FORCE_DEAD) && scene_runs(w, w.scene[b]);
+}
+
+__global__ void k_dbg_count(DevWorld w, int* lc, int* pc) {
+    const int M = w.g[G_MANIFOLDS];
+    const int J = w.J, SP = w.SP, MC = w.Mcap, T = w.T, TR = w.TR;
+    GRID_STRIDE(t, J + SP + MC + T + TR) {
+        int n = 0;
+        if (t < J) {
+            n = dbg_force_alive(w, w.j_flags[t], w.j_b[t]) ? 1 : 0;
+        } else if (t < J + SP) {
+            const int k = t - J;
+            n = dbg_force_alive(w, w.s_flags[k], w.s_b[k]) ? 1 : 0;
+        } else if (t < J + SP + MC) {
+            const int m = t - J - SP;
+            int np = 0;
+            if (m < M && scene_runs(w, w.scene[w.m.a[m]])) {
+                const int nc = w.m.n[m];
+                np = 2 * nc;
+                for (int c = 0; c < nc; ++c)
+                    if (w.m.cold[((size_t)m * 8 + c) * CONTACT_COLD_F4 + 2].w > 0.f) ++n;
+            }
+            pc[m] = np;
+        } else if (t < J + SP + MC + T) {
+            const int4 v = w.t_v[t - J - SP - MC];
+            n = (v.x >= 0 && v.y >= 0 && v.z >= 0 && v.w >= 0 && scene_runs(w, w.scene[v.x])) ? 6 : 0;
+        } else {
+            const int4 v = w.r_v[t - J - SP - MC - T];
+            n = (v.x >= 0 && v.y >= 0 && v.z >= 0 && scene_runs(w, w.scene[v.x])) ? 3 : 0;
+        }
+        lc[t] = n;
+    }
+}
+
+AVBD_DEV void put_line(float* o, v3 a, v3 b, float r, float g, float bl, float al) {
+    o[0] = a.x; o[1] = a.y; o[2] = a.z;
+    o[3] = b.x; o[4] = b.y; o[5] = b.z;
+    o[6] = r; o[7] = g; o[8] = bl; o[9] = al;
+}
+
+AVBD_DEV float4 body_rgb(const DebugOut& d, int b) {
+    return d.rgb ? d.rgb[b] : make_float4(1.f, 1.f, 1.f, 0.f);
+}
+
+AVBD_DEV void edge_line(const DevWorld& w, const DebugOut& d, float* o, int a, int b) {
+    const float4 ca = body_rgb(d, a), cb = body_rgb(d, b);
+    // (color_a + color_b) * 0.5; the reference leaves alpha unset, here 1
+    put_line(o, f3(w.last_pos[a]), f3(w.last_pos[b]), (ca.x + cb.x) * 0.5f, (ca.y + cb.y) * 0.5f,
+             (ca.z + cb.z) * 0.5f, 1.f);
+}
+
+__global__ void k_dbg_write(DevWorld w, DebugOut d, const int* lc, const int* pc) {
+    const int J = w.J, SP = w.SP, MC = w.Mcap, T = w.T, TR = w.TR;
+    GRID_STRIDE(t, J + SP + MC + T + TR) {
+        const bool has_lines = lc[t] > 0;
+        const bool has_points = t >= J + SP && t < J + SP + MC && pc[t - J - SP] > 0;
+        if (!has_lines && !has_points) continue;
+        float* o = d.lines + (size_t)d.loff[t] * 10;
+        if (t < J + SP) {
+            const bool jt = t < J;
+            const int k = jt ? t : t - J;
+            const int a = jt ? w.j_a[k] : w.s_a[k], b = jt ? w.j_b[k] : w.s_b[k];
+            const v3 ra = f3(jt ? w.j_ra[k] : w.s_ra[k]), rb = f3(jt ? w.j_rb[k] : w.s_rb[k]);
+            const v3 pa = a >= 0 ? start_point(w, a, ra) : ra;
+            const v3 pb = start_point(w, b, rb);
+            if (jt) put_line(o, pa, pb, 0.75f, 0.f, 0.f, 1.f);
+            else put_line(o, pa, pb, 0.f, 1.f, 0.f, 1.f);
+        } else if (t < J + SP + MC) {
+            const int m = t - J - SP;
+            const int a = w.m.a[m], b = w.m.b[m];
+            const int nc = w.m.n[m];
+            const qt qa = fq(w.last_rot[a]), qb = fq(w.last_rot[b]);
+            const v3 xa = f3(w.last_pos[a]), xb = f3(w.last_pos[b]);
+            float* po = d.points + (size_t)d.poff[m] * 8;
+            for (int c = 0; c < nc; ++c) {
+                const float4* cold = w.m.cold + ((size_t)m * 8 + c) * CONTACT_COLD_F4;
+                const float4 c0 = cold[0], c1 = cold[1], c2 = cold[2];
+                if (c2.w > 0.f) {
+                    const v3 s = w.m.cpos ? f3(w.m.cpos[(size_t)m * 8 + c]) : v3{0.f, 0.f, 0.f};
+                    put_line(o, s, add(s, mul(f3(c2), c2.w)), 1.f, 0.f, 0.f, 1.f);
+                    o += 10;
+                }
+                // points: posA + R_A rA (yellow), posB + R_B rB (cyan)
+                const v3 wa = add(xa, mv(rotmat(qa), v3{c0.z, c0.w, c1.x}));
+                const v3 wb = add(xb, mv(rotmat(qb), v3{c1.y, c1.z, c1.w}));
+                po[0] = wa.x; po[1] = wa.y; po[2] = wa.z; po[3] = 0.f;
+                po[4] = 1.f; po[5] = 1.f; po[6] = 0.f; po[7] = 1.f;
+                po[8] = wb.x; po[9] = wb.y; po[10] = wb.z; po[11] = 0.f;
+                po[12] = 0.f; po[13] = 1.f; po[14] = 1.f; po[15] = 1.f;
+                po += 16;
+            }
+        } else if (t < J + SP + MC + T) {
+            const int4 v = w.t_v[t - J - SP - MC];
+            const int e[6][2] = {{v.x, v.y}, {v.x, v.z}, {v.x, v.w}, {v.y, v.z}, {v.y, v.w}, {v.z, v.w}};
+            for (int k = 0; k < 6; ++k) edge_line(w, d, o + 10 * k, e[k][0], e[k][1]);
+        } else {
+            const int4 v = w.r_v[t - J - SP - MC - T];
+            const int e[3][2] = {{v.x, v.y}, {v.y, v.z}, {v.z, v.x}};
+            for (int k = 0; k < 3; ++k) edge_line(w, d, o + 10 * k, e[k][0], e[k][1]);
         }
     }
 }

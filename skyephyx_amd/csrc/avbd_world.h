@@ -85,6 +85,7 @@ struct ManifoldBuf {
     float* fric;
     float4* hot;   // [cap * 8 * CONTACT_HOT_F4]
     float4* cold;  // [cap * 8 * CONTACT_COLD_F4]
+    float4* cpos;  // [cap * 8] world contact points, only with debug output on (else null)
 };
 
 struct DevWorld {
@@ -185,6 +186,17 @@ struct DevWorld {
     int* fracture_bodies;  // [2 * (J + SP) + 2]
     int* g;             // global status words [G_WORDS]
     long long* next_serial;
+};
+
+// Debug/render output of a step (option debug_output): per line 10 floats
+// (start xyz, end xyz, colour rgba), per point 8 (position xyz, 0, colour
+// rgba), the layouts of the reference's GPULineData / GPUDebugPointData.
+struct DebugOut {
+    float* lines;
+    float* points;
+    const int* loff;    // [J + SP + Mcap + T + TR]: first line of each force / element slot
+    const int* poff;    // [Mcap]: first point of each manifold slot
+    const float4* rgb;  // per body colour (Mesh::color), or null for white
 };
 
 // Body renumbering of a bounds removal, which permutes bodies only inside the
