@@ -244,6 +244,9 @@ def main():
     ap.add_argument("--serve-load", type=float, nargs="+", default=[0.8],
                     help="served load points as fractions of the served (closed-loop) saturation")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per served load point")
+    ap.add_argument("--debug-output", action="store_true",
+                    help="make the renderer's line/point data after every step (option debug_output; "
+                         "reports their counts; the timed steps include it)")
     ap.add_argument("--snapshot-dir", default=None,
                     help="after the timed steps, save the device world as a snapshot file in this folder and "
                          "load it into a fresh context (reports bytes and both rates; the file is removed)")
@@ -265,6 +268,8 @@ def main():
         ctx.set_option("step_graphs", 0)
     if args.fused:
         ctx.set_option("fused_scenes", 1)
+    if args.debug_output:
+        ctx.set_option("debug_output", 1)
     ctx.upload(world)
     log("[bench] uploaded")
     for i in range(args.warmup):
@@ -293,6 +298,11 @@ def main():
     em, _ = ctx.status(len(state.scenes))
     tune = ctx.tune_table()
     ctx_bytes = ctx.memory()
+    debug = None
+    if args.debug_output:
+        lines, points = ctx.debug_output()
+        debug = {"lines_per_step": int(len(lines)), "points_per_step": int(len(points)),
+                 "bytes_per_step": int(lines.nbytes + points.nbytes)}
     snapshot = None
     if args.snapshot_dir:
         path = os.path.join(args.snapshot_dir, f"{args.workload}.avbdsnap")
@@ -430,6 +440,8 @@ def main():
             res["gpu_over_cpu_all_cores"] = round(scene_steps / rate_all, 2)
     if snapshot:
         res["snapshot"] = snapshot
+    if debug:
+        res["debug_output"] = debug
     print(json.dumps(res), flush=True)
 
 

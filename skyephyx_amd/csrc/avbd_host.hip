@@ -1614,18 +1614,22 @@ static int extract_debug(avbd_ctx* c) {
     HIP_TRY(hipStreamSynchronize(st));
     d.nl = (long long)tail[0] + tail[1];
     d.np = (long long)tail[2] + tail[3];
+    // grown by a quarter beyond the need: the counts drift up step by step
+    // as contacts appear, and a reallocation per step would dominate
     auto grow = [&](float*& buf, size_t& cap, size_t need) -> int {
         if (need <= cap) return 0;
         if (buf) HIP_TRY(hipFree(buf));
         buf = nullptr;
         cap = 0;
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&buf), sizeof(float) * std::max<size_t>(need, 1)));
-        cap = need;
+        const size_t want = std::max<size_t>(need + need / 4, 1024);
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&buf), sizeof(float) * want));
+        cap = want;
         return 0;
     };
     if ((rc = grow(d.lines, d.lcap, (size_t)d.nl * 10)) || (rc = grow(d.points, d.pcap, (size_t)d.np * 8))) return rc;
     const DebugOut out{d.lines, d.points, d.loff, d.poff, d.rgb};
-    hipLaunchKernelGGL(k_dbg_write, dim3(grid_for(slots)), dim3(256), 0, st, w, out, (const int*)d.lc, (const int*)d.pc);
+    const int blocks = std::max(1, std::min((slots + 127) / 128, 65536));
+    hipLaunchKernelGGL(k_dbg_write, dim3(blocks), dim3(128), 0, st, w, out, (const int*)d.lc, (const int*)d.pc);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -2062,54 +2062,73 @@ AVBD_DEV void edge_line(const DevWorld& w, const DebugOut& d, float* o, int a, i
              (ca.z + cb.z) * 0.5f, 1.f);
 }
 
-__global__ void k_dbg_write(DevWorld w, DebugOut d, const int* lc, const int* pc) {
+// One workgroup per DBG_SLOTS consecutive slots: their lines are contiguous
+// in the output (the offsets are a scan over the slots), so they are built
+// in LDS and then stored with consecutive 8-byte stores (a line is 40 bytes;
+// one thread writing its slot's lines directly would store 40-240 bytes per
+// lane, scattered across a wave).
+#define DBG_SLOTS 128
+#define DBG_MAX_LINES (8 * DBG_SLOTS)  // a manifold slot has at most 8 lines
+__global__ __launch_bounds__(DBG_SLOTS) void k_dbg_write(DevWorld w, DebugOut d, const int* lc, const int* pc) {
+    __shared__ float sl[DBG_MAX_LINES * 10];
     const int J = w.J, SP = w.SP, MC = w.Mcap, T = w.T, TR = w.TR;
-    GRID_STRIDE(t, J + SP + MC + T + TR) {
-        const bool has_lines = lc[t] > 0;
-        const bool has_points = t >= J + SP && t < J + SP + MC && pc[t - J - SP] > 0;
-        if (!has_lines && !has_points) continue;
-        float* o = d.lines + (size_t)d.loff[t] * 10;
-        if (t < J + SP) {
-            const bool jt = t < J;
-            const int k = jt ? t : t - J;
-            const int a = jt ? w.j_a[k] : w.s_a[k], b = jt ? w.j_b[k] : w.s_b[k];
-            const v3 ra = f3(jt ? w.j_ra[k] : w.s_ra[k]), rb = f3(jt ? w.j_rb[k] : w.s_rb[k]);
-            const v3 pa = a >= 0 ? start_point(w, a, ra) : ra;
-            const v3 pb = start_point(w, b, rb);
-            if (jt) put_line(o, pa, pb, 0.75f, 0.f, 0.f, 1.f);
-            else put_line(o, pa, pb, 0.f, 1.f, 0.f, 1.f);
-        } else if (t < J + SP + MC) {
-            const int m = t - J - SP;
-            const int a = w.m.a[m], b = w.m.b[m];
-            const int nc = w.m.n[m];
-            const qt qa = fq(w.last_rot[a]), qb = fq(w.last_rot[b]);
-            const v3 xa = f3(w.last_pos[a]), xb = f3(w.last_pos[b]);
-            float* po = d.points + (size_t)d.poff[m] * 8;
-            for (int c = 0; c < nc; ++c) {
-                const float4* cold = w.m.cold + ((size_t)m * 8 + c) * CONTACT_COLD_F4;
-                const float4 c0 = cold[0], c1 = cold[1], c2 = cold[2];
-                if (c2.w > 0.f) {
-                    const v3 s = w.m.cpos ? f3(w.m.cpos[(size_t)m * 8 + c]) : v3{0.f, 0.f, 0.f};
-                    put_line(o, s, add(s, mul(f3(c2), c2.w)), 1.f, 0.f, 0.f, 1.f);
-                    o += 10;
+    const int slots = J + SP + MC + T + TR;
+    for (int b0 = blockIdx.x * DBG_SLOTS; b0 < slots; b0 += gridDim.x * DBG_SLOTS) {
+        const int b1 = min(b0 + DBG_SLOTS, slots);
+        const int base = d.loff[b0];
+        const int total = d.loff[b1 - 1] + lc[b1 - 1] - base;
+        const int t = b0 + threadIdx.x;
+        if (t < b1 && (lc[t] > 0 || (t >= J + SP && t < J + SP + MC && pc[t - J - SP] > 0))) {
+            float* o = sl + (size_t)(d.loff[t] - base) * 10;
+            if (t < J + SP) {
+                const bool jt = t < J;
+                const int k = jt ? t : t - J;
+                const int a = jt ? w.j_a[k] : w.s_a[k], b = jt ? w.j_b[k] : w.s_b[k];
+                const v3 ra = f3(jt ? w.j_ra[k] : w.s_ra[k]), rb = f3(jt ? w.j_rb[k] : w.s_rb[k]);
+                const v3 pa = a >= 0 ? start_point(w, a, ra) : ra;
+                const v3 pb = start_point(w, b, rb);
+                if (jt) put_line(o, pa, pb, 0.75f, 0.f, 0.f, 1.f);
+                else put_line(o, pa, pb, 0.f, 1.f, 0.f, 1.f);
+            } else if (t < J + SP + MC) {
+                const int m = t - J - SP;
+                const int a = w.m.a[m], b = w.m.b[m];
+                const int nc = w.m.n[m];
+                const qt qa = fq(w.last_rot[a]), qb = fq(w.last_rot[b]);
+                const v3 xa = f3(w.last_pos[a]), xb = f3(w.last_pos[b]);
+                float4* po = reinterpret_cast<float4*>(d.points) + (size_t)d.poff[m] * 2;
+                for (int c = 0; c < nc; ++c) {
+                    const float4* cold = w.m.cold + ((size_t)m * 8 + c) * CONTACT_COLD_F4;
+                    const float4 c0 = cold[0], c1 = cold[1], c2 = cold[2];
+                    if (c2.w > 0.f) {
+                        const v3 s = w.m.cpos ? f3(w.m.cpos[(size_t)m * 8 + c]) : v3{0.f, 0.f, 0.f};
+                        put_line(o, s, add(s, mul(f3(c2), c2.w)), 1.f, 0.f, 0.f, 1.f);
+                        o += 10;
+                    }
+                    // points: posA + R_A rA (yellow), posB + R_B rB (cyan); 32-byte records
+                    const v3 wa = add(xa, mv(rotmat(qa), v3{c0.z, c0.w, c1.x}));
+                    const v3 wb = add(xb, mv(rotmat(qb), v3{c1.y, c1.z, c1.w}));
+                    po[0] = make_float4(wa.x, wa.y, wa.z, 0.f);
+                    po[1] = make_float4(1.f, 1.f, 0.f, 1.f);
+                    po[2] = make_float4(wb.x, wb.y, wb.z, 0.f);
+                    po[3] = make_float4(0.f, 1.f, 1.f, 1.f);
+                    po += 4;
                 }
-                // points: posA + R_A rA (yellow), posB + R_B rB (cyan)
-                const v3 wa = add(xa, mv(rotmat(qa), v3{c0.z, c0.w, c1.x}));
-                const v3 wb = add(xb, mv(rotmat(qb), v3{c1.y, c1.z, c1.w}));
-                po[0] = wa.x; po[1] = wa.y; po[2] = wa.z; po[3] = 0.f;
-                po[4] = 1.f; po[5] = 1.f; po[6] = 0.f; po[7] = 1.f;
-                po[8] = wb.x; po[9] = wb.y; po[10] = wb.z; po[11] = 0.f;
-                po[12] = 0.f; po[13] = 1.f; po[14] = 1.f; po[15] = 1.f;
-                po += 16;
+            } else if (t < J + SP + MC + T) {
+                const int4 v = w.t_v[t - J - SP - MC];
+                const int e[6][2] = {{v.x, v.y}, {v.x, v.z}, {v.x, v.w}, {v.y, v.z}, {v.y, v.w}, {v.z, v.w}};
+                for (int k = 0; k < 6; ++k) edge_line(w, d, o + 10 * k, e[k][0], e[k][1]);
+            } else {
+                const int4 v = w.r_v[t - J - SP - MC - T];
+                const int e[3][2] = {{v.x, v.y}, {v.y, v.z}, {v.z, v.x}};
+                for (int k = 0; k < 3; ++k) edge_line(w, d, o + 10 * k, e[k][0], e[k][1]);
             }
-        } else if (t < J + SP + MC + T) {
-            const int4 v = w.t_v[t - J - SP - MC];
-            const int e[6][2] = {{v.x, v.y}, {v.x, v.z}, {v.x, v.w}, {v.y, v.z}, {v.y, v.w}, {v.z, v.w}};
-            for (int k = 0; k < 6; ++k) edge_line(w, d, o + 10 * k, e[k][0], e[k][1]);
-        } else {
-            const int4 v = w.r_v[t - J - SP - MC - T];
-            const int e[3][2] = {{v.x, v.y}, {v.y, v.z}, {v.z, v.x}};
-            for (int k = 0; k < 3; ++k) edge_line(w, d, o + 10 * k, e[k][0], e[k][1]);
         }
+        __syncthreads();
+        // a line is 40 bytes, so every line (and the block's first) starts
+        // 8-byte aligned
+        const float2* src = reinterpret_cast<const float2*>(sl);
+        float2* dst = reinterpret_cast<float2*>(d.lines + (size_t)base * 10);
+        for (int i = threadIdx.x; i < total * 5; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
     }
 }

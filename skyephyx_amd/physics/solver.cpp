@@ -486,7 +486,29 @@ void Solver::Step() {
     } else if (state_changed) {
         check(avbd_upload_bodies(ctx_, bodies.data(), (int32_t)bodies.size()), "avbd_upload_bodies");
     }
+    check(avbd_set_option(ctx_, "debug_output", debugOutput ? 1 : 0), "avbd_set_option");
+    if (debugOutput) {
+        std::vector<float> rgb(3 * solverBodies.size());
+        for (size_t i = 0; i < solverBodies.size(); ++i)
+            for (int k = 0; k < 3; ++k) rgb[3 * i + k] = solverBodies[i]->color[k];
+        check(avbd_set_body_colours(ctx_, rgb.data(), (int32_t)solverBodies.size()), "avbd_set_body_colours");
+    }
     check(avbd_step(ctx_, 1), "avbd_step");
+    lineData.clear();
+    debugPointData.clear();
+    if (debugOutput) {
+        static_assert(sizeof(GPULineData) == sizeof(avbd_line), "line record layouts");
+        static_assert(sizeof(GPUDebugPointData) == sizeof(avbd_debug_point), "point record layouts");
+        int64_t nl = 0, np = 0;
+        check(avbd_get_debug_lines(ctx_, nullptr, 0, &nl), "avbd_get_debug_lines");
+        lineData.resize((size_t)nl);
+        check(avbd_get_debug_lines(ctx_, reinterpret_cast<avbd_line*>(lineData.data()), nl, &nl),
+              "avbd_get_debug_lines");
+        check(avbd_get_debug_points(ctx_, nullptr, 0, &np), "avbd_get_debug_points");
+        debugPointData.resize((size_t)np);
+        check(avbd_get_debug_points(ctx_, reinterpret_cast<avbd_debug_point*>(debugPointData.data()), np, &np),
+              "avbd_get_debug_points");
+    }
     int32_t em = 0, removed = 0;
     check(avbd_get_status(ctx_, &em, &removed, 1), "avbd_get_status");
     emergencyStop = em != 0;

@@ -67,10 +67,12 @@ public:
     std::vector<Force*> forcePtrs;
     std::vector<Energy*> energyPtrs;
 
-    // Debug geometry of the reference's renderer; the headless facade leaves
-    // them empty.
+    // Debug geometry of the reference's renderer, filled after each Step from
+    // the device when debugOutput is on (avbd_get_debug_lines / _points;
+    // element edges take the bodies' Mesh::color), else left empty.
     std::vector<GPULineData> lineData;
     std::vector<GPUDebugPointData> debugPointData;
+    bool debugOutput = false;
 
     void Start();
     void Clear();

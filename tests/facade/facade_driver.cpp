@@ -179,6 +179,29 @@ int main(int argc, char** argv) {
             std::printf("uploads %lld\n", (long long)s.uploads);
             return write_records(s, argv[5]);
         }
+        if (mode == "debug" && argc == 5) {
+            // debug <scene> <steps> <out>: lineData and debugPointData of the
+            // last step, then the bodies' colours
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            avbd_ctx* ctx = s.DeviceContext();
+            if (avbd_set_option(ctx, "lanes", 1) || avbd_set_option(ctx, "fem_lanes", 1)) return 3;
+            s.debugOutput = true;
+            for (int k = 0, n = std::atoi(argv[3]); k < n; ++k) s.Step();
+            FILE* f = std::fopen(argv[4], "wb");
+            if (!f) return 5;
+            const int64_t head[3] = {(int64_t)s.lineData.size(), (int64_t)s.debugPointData.size(),
+                                     (int64_t)s.bodyPtrs.size()};
+            std::fwrite(head, sizeof(head), 1, f);
+            write_array(f, s.lineData);
+            write_array(f, s.debugPointData);
+            for (Mesh* m : s.bodyPtrs) {
+                const float c[3] = {m->color[0], m->color[1], m->color[2]};
+                std::fwrite(c, sizeof(c), 1, f);
+            }
+            std::fclose(f);
+            return 0;
+        }
         if (mode == "snapshot" && argc == 4) {
             Solver s;
             if (!build(s, argv[2])) return 2;

@@ -232,3 +232,37 @@ def test_spawn_into_freed_slots_stays_on_the_device(driver, tmp_path):
     assert len(fac.bodies) == len(ref.bodies) == len(built.bodies) - len(ids) + n_add
     _same(fac.bodies, ref.bodies, ("pos", "rot", "vel", "ang_vel", "flags", "mass"), "bodies")
     assert len(fac.manifolds) == len(ref.manifolds)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["pyramid", "spring_bridge", "cloth", "fem_block"])
+def test_debug_geometry_fills_line_data(driver, tmp_path, name):
+    """Solver::lineData / debugPointData with debugOutput on: the device's
+    line and point data of the last step, with the bodies' colours, equal
+    to the C ABI's for the same steps."""
+    from skyephyx_amd.capi import Context
+
+    steps = 30
+    out = str(tmp_path / "d.bin")
+    r = subprocess.run([driver, "debug", name, str(steps), out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(out, "rb").read()
+    nl, npt, nb = np.frombuffer(raw[:24], np.int64)
+    at = 24
+    lines = np.frombuffer(raw[at:at + nl * 40], np.float32).reshape(-1, 10)
+    at += nl * 40
+    points = np.frombuffer(raw[at:at + npt * 32], np.float32).reshape(-1, 8)
+    at += npt * 32
+    rgb = np.frombuffer(raw[at:at + nb * 12], np.float32).reshape(-1, 3)
+    subprocess.run([driver, "records", name, str(tmp_path / "w.bin")], check=True)
+    built = read_records(str(tmp_path / "w.bin"))
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.set_option("debug_output", 1)
+        ctx.upload(built)
+        for _ in range(steps):
+            ctx.set_body_colours(rgb)
+            ctx.step(1)
+        gl, gp = ctx.debug_output()
+    assert len(lines) > 0
+    np.testing.assert_array_equal(lines, gl)
+    np.testing.assert_array_equal(points, gp)
