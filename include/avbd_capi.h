@@ -42,6 +42,8 @@
  *   avbd_get_debug_lines/_points,
  *   avbd_set_body_colours     -> Solver::lineData / debugPointData
  *                                (solver.hpp:36-37), Mesh::color
+ *   avbd_get_instances        -> RenderEngine::UpdateInstanceBuffer
+ *                                (RenderEngine.cpp:951-1029)
  *   avbd_footprint, avbd_get_memory, avbd_set_option, avbd_tune_*,
  *   avbd_kernel_stats         -> (no counterpart: device memory, tuning and
  *                                measurement)
@@ -321,6 +323,21 @@ typedef struct avbd_debug_point {
     float pad;
     float color[4];
 } avbd_debug_point;
+/* Renderer instance data (RenderEngine::UpdateInstanceBuffer,
+ * RenderEngine.cpp:951-1029; GPUInstanceData, RenderEngine.hpp:43-49), one
+ * per live body in body order from the current poses: column-major model
+ * matrix [R diag(scale) | position] (Transform::GetModelMatrix), normal
+ * matrix R diag(1/scale) in three 4-float columns (GetNormalMatrix), colour
+ * (Mesh::color from avbd_set_body_colours, else white) with alpha 1.  The
+ * reference batches instances by model type; the caller, which knows the
+ * types, does that. */
+typedef struct avbd_instance {
+    float model[16];
+    float normal[12];
+    float color[4];
+} avbd_instance;
+/* Copies at most cap instances; *n = the live body count. */
+int avbd_get_instances(avbd_ctx* ctx, avbd_instance* out, int32_t cap, int32_t* n);
 /* rgb: 3 floats per live body (avbd_download's order) */
 int avbd_set_body_colours(avbd_ctx* ctx, const float* rgb, int32_t n);
 /* The last step's lines / points: copies at most cap, *n = the full count. */

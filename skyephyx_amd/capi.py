@@ -75,6 +75,7 @@ def lib():
         L.avbd_remove_bodies.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.avbd_add_bodies.argtypes = [vp, vp, C.c_int32]
         L.avbd_set_body_colours.argtypes = [vp, vp, C.c_int32]
+        L.avbd_get_instances.argtypes = [vp, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.avbd_get_debug_lines.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
         L.avbd_get_debug_points.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
         L.avbd_snapshot_write.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
@@ -216,6 +217,15 @@ class Context:
         """Mesh::color of every live body ([n, 3]), for the element edge lines."""
         a = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
         _check(self.L.avbd_set_body_colours(self.h, a.ctypes.data, len(a)))
+
+    def instances(self):
+        """Renderer instance data per live body ([n, 32]: model matrix 16
+        column-major, normal matrix 3 columns of 4, colour 4)."""
+        n = C.c_int32()
+        _check(self.L.avbd_get_instances(self.h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 32), np.float32)
+        _check(self.L.avbd_get_instances(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out
 
     def debug_output(self):
         """(lines [n, 10], points [n, 8]) of the last step (option debug_output):

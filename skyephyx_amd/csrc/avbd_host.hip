@@ -1991,6 +1991,37 @@ int avbd_get_debug_lines(avbd_ctx* c, avbd_line* out, int64_t cap, int64_t* n) {
     return 0;
 }
 
+int avbd_get_instances(avbd_ctx* c, avbd_instance* out, int32_t cap, int32_t* n) {
+    if (!c || !n || (!out && cap > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    HIP_TRY(hipSetDevice(c->device));
+    DevWorld& w = c->w;
+    const int n_live = w.N - (int)std::count(c->body_dead.begin(), c->body_dead.end(), 1);
+    *n = n_live;
+    if (!out || cap <= 0 || n_live == 0) return 0;
+    hipStream_t st = c->stream;
+    struct Scratch {
+        void* p = nullptr;
+        ~Scratch() { if (p) hipFree(p); }
+    } scratch;
+    size_t cub_bytes = 0;
+    int* nul = nullptr;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, cub_bytes, nul, nul, w.N, st));
+    const size_t ints = 2 * (size_t)w.N + 4;
+    HIP_TRY(hipMalloc(&scratch.p, sizeof(avbd_instance) * (size_t)n_live + sizeof(int) * ints + cub_bytes + 256));
+    float4* d_out = static_cast<float4*>(scratch.p);
+    int* flag = reinterpret_cast<int*>(d_out + (size_t)n_live * 8);
+    int* off = flag + w.N + 2;
+    void* cub = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(off + w.N + 2) + 255) & ~(uintptr_t)255);
+    hipLaunchKernelGGL(k_live_flag, dim3(grid_for(w.N)), dim3(256), 0, st, w, flag);
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cub, cub_bytes, flag, off, w.N, st));
+    hipLaunchKernelGGL(k_instances, dim3(grid_for(w.N)), dim3(256), 0, st, w, (const int*)flag, (const int*)off,
+                       (const float4*)c->dbg.rgb, d_out);
+    HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(avbd_instance) * (size_t)std::min(cap, n_live), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
 int avbd_get_debug_points(avbd_ctx* c, avbd_debug_point* out, int64_t cap, int64_t* n) {
     if (!c || !n || (!out && cap > 0)) return fail(AVBD_E_ARG, "null argument");
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -2132,3 +2132,33 @@ __global__ __launch_bounds__(DBG_SLOTS) void k_dbg_write(DevWorld w, DebugOut d,
         __syncthreads();
     }
 }
+
+// Instance data of the renderer (RenderEngine::UpdateInstanceBuffer,
+// RenderEngine.cpp:951-1029; GPUInstanceData, RenderEngine.hpp:43-49): per
+// live body the column-major model matrix [R diag(scale) | position]
+// (Transform::GetModelMatrix, geometry.hpp:112-122), the normal matrix
+// (inverse transpose of its 3x3 block = R diag(1 / scale) for a rotation R;
+// GetNormalMatrix, geometry.hpp:125-139, columns padded to 4) and the colour
+// with alpha 1; 32 floats.  live_off: index among the live bodies (dead
+// slots: scale.w = -1e30, marked -1 by the caller's scan of k_live_flag).
+__global__ void k_live_flag(DevWorld w, int* flag) {
+    GRID_STRIDE(b, w.N) flag[b] = w.scale[b].w > -1e29f ? 1 : 0;
+}
+
+__global__ void k_instances(DevWorld w, const int* flag, const int* live_off, const float4* rgb, float4* out) {
+    GRID_STRIDE(b, w.N) {
+        if (!flag[b]) continue;
+        const m3 R = rotmat(fq(w.rot[b]));
+        const float4 s = w.scale[b];
+        const float4 p = w.pos[b];
+        const float4 c = rgb ? rgb[b] : make_float4(1.f, 1.f, 1.f, 0.f);
+        const float sc[3] = {s.x, s.y, s.z};
+        float4* o = out + (size_t)live_off[b] * 8;
+        for (int j = 0; j < 3; ++j)  // model matrix columns 0-2: R e_j s_j
+            o[j] = make_float4(R.m[0][j] * sc[j], R.m[1][j] * sc[j], R.m[2][j] * sc[j], 0.f);
+        o[3] = make_float4(p.x, p.y, p.z, 1.f);
+        for (int j = 0; j < 3; ++j)  // normal matrix columns: R e_j / s_j
+            o[4 + j] = make_float4(R.m[0][j] / sc[j], R.m[1][j] / sc[j], R.m[2][j] / sc[j], 0.f);
+        o[7] = make_float4(c.x, c.y, c.z, 1.f);
+    }
+}

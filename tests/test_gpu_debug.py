@@ -97,3 +97,40 @@ def test_debug_output_colours_and_scenes():
     want = np.array([(rgb[t[i]] + rgb[t[j]]) * 0.5 for t in tets for i, j in edges], np.float32)
     np.testing.assert_allclose(lines[len(world.joints):, 6:9], want, rtol=0, atol=1e-7)
     assert np.all(lines[:, 9] == 1.0)
+
+
+def _rotmat(q):
+    w, x, y, z = (float(v) for v in q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_instances_match_model_and_normal_matrices():
+    """avbd_get_instances: per live body (dead slots skipped, after a removal)
+    the model matrix [R diag(scale) | t] column-major and the normal matrix
+    as the reference computes it, inverse(3x3 block) transposed (numpy's
+    general inverse here), with the body's colour."""
+    world = _warm(scenes.box_lattice(shape=(3, 3, 2), spacing=3.0, n_chain=4, n_springs=3, bounds=50.0), 10)
+    with Context(lanes=4, fem_lanes=4) as ctx:
+        ctx.upload(world)
+        ctx.remove_bodies([4, 9])
+        n = len(world.bodies) - 2
+        rgb = np.random.default_rng(1).random((n, 3)).astype(np.float32)
+        ctx.set_body_colours(rgb)
+        ctx.step(2)
+        inst = ctx.instances()
+        bodies = ctx.download().bodies
+    assert inst.shape == (n, 32)
+    for i, b in enumerate(bodies):
+        RS = _rotmat(b["rot"]) @ np.diag(b["scale"].astype(np.float64))
+        model = np.eye(4)
+        model[:3, :3] = RS
+        model[:3, 3] = b["pos"]
+        np.testing.assert_allclose(inst[i, :16].reshape(4, 4).T, model, rtol=0, atol=2e-6 * max(1.0, np.abs(b["pos"]).max()))
+        normal = np.linalg.inv(RS).T
+        got = inst[i, 16:28].reshape(3, 4)  # three columns of 4
+        np.testing.assert_allclose(got[:, :3].T, normal, rtol=0, atol=1e-5 * np.abs(normal).max())
+        assert np.all(got[:, 3] == 0)
+        np.testing.assert_array_equal(inst[i, 28:31], rgb[i])
+        assert inst[i, 31] == 1.0
