@@ -356,9 +356,10 @@ int avbd_remove_bodies(avbd_ctx* ctx, const int32_t* bodies, int32_t n);
 
 /* Solver::AddBody / AddParticle (solver.cpp:50-124) on the device-resident
  * world, without a re-upload: each record (its .scene names the scene) is
- * appended to its scene's body order, into a slot a removal freed.
- * AVBD_E_CAPACITY when a scene has no such slot left (upload the world
- * again).  Forces and energies are added with a full upload. */
+ * appended to its scene's body order, into a free slot: one a removal
+ * freed or a spare one (option "spare_bodies").  AVBD_E_CAPACITY when a
+ * scene has no free slot left (upload the world again).  Forces and
+ * energies are added with a full upload. */
 int avbd_add_bodies(avbd_ctx* ctx, const avbd_body* bodies, int32_t n);
 
 /* Bodies the last avbd_step / avbd_step_scenes call removed for leaving the
@@ -409,6 +410,10 @@ int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
  *                    its own stream beside the bucket launches) when "lanes"
  *                    equals "fem_lanes"; 0 (default) = every scene on the
  *                    per-colour bucket launches.  Results are identical.
+ *   "spare_bodies"   k >= 0 (default 0): the next avbd_upload gives every
+ *                    scene k free body slots after its bodies, for
+ *                    avbd_add_bodies (bodies added between steps without a
+ *                    re-upload); they take device memory, not body indices.
  *   "debug_output"   1 = after every step, make its line and point data
  *                    (avbd_get_debug_lines / _points); 0 (default) = none.
  *   "colour_updates" 1 (default) = a scene whose constraint graph gained or

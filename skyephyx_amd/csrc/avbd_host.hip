@@ -80,6 +80,7 @@ struct avbd_ctx {
     // a scene whose constraint graph changed a little updates its last
     // colouring from the changed bodies instead of colouring from scratch
     bool colour_updates = true;
+    int spare_bodies = 0;  // dead slots per scene added at upload (avbd_add_bodies fills them)
     // Debug/render output (option debug_output): counters, offsets and the
     // contact points come from alloc (an upload frees them; set up again on
     // the next use), the output arrays are grown here and owned here.
@@ -1499,6 +1500,11 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
         c->fused_max = int(value);
         return 0;
     }
+    if (n == "spare_bodies") {
+        if (value < 0 || value > (1 << 16)) return fail(AVBD_E_ARG, "spare_bodies out of range");
+        c->spare_bodies = int(value);
+        return 0;
+    }
     if (n == "debug_output") {
         c->dbg_on = value != 0;
         return 0;
@@ -1565,7 +1571,59 @@ int avbd_kernel_stats(avbd_ctx* c, const char* kernel, int64_t* launches, double
 int avbd_upload(avbd_ctx* c, const avbd_world* w, int64_t manifold_capacity) {
     if (!c || !w) return fail(AVBD_E_ARG, "null argument");
     HIP_TRY(hipSetDevice(c->device));
-    return do_upload(c, w, manifold_capacity);
+    const int k = c->spare_bodies;
+    if (k == 0) return do_upload(c, w, manifold_capacity);
+    // Option spare_bodies: every scene gets k dead slots after its bodies
+    // (static, no broad-phase sphere), which avbd_add_bodies can fill.  The
+    // records are expanded on the host and every body index shifted by k per
+    // earlier scene.
+    if (w->n_scenes < 1 || w->n_bodies < 0 || (w->n_bodies > 0 && !w->bodies)) return fail(AVBD_E_ARG, "bad world");
+    const int S = w->n_scenes, N = w->n_bodies;
+    for (int i = 0; i < N; ++i)
+        if (w->bodies[i].scene < 0 || w->bodies[i].scene >= S || (i > 0 && w->bodies[i].scene < w->bodies[i - 1].scene))
+            return fail(AVBD_E_ARG, "bodies must be grouped by scene in ascending order");
+    auto shift = [&](int i) { return i < 0 || i >= N ? i : i + k * w->bodies[i].scene; };
+    std::vector<avbd_body> bodies;
+    bodies.reserve((size_t)N + (size_t)k * S);
+    std::vector<int> spare;
+    int i = 0;
+    for (int s = 0; s < S; ++s) {
+        for (; i < N && w->bodies[i].scene == s; ++i) bodies.push_back(w->bodies[i]);
+        for (int q = 0; q < k; ++q) {
+            avbd_body d{};
+            d.rot[0] = 1.f;
+            d.scale[0] = d.scale[1] = d.scale[2] = 1.f;
+            d.radius = -1e30f;
+            d.flags = AVBD_BODY_STATIC;
+            d.scene = s;
+            spare.push_back((int)bodies.size());
+            bodies.push_back(d);
+        }
+    }
+    std::vector<avbd_joint> joints(w->joints, w->joints + w->n_joints);
+    for (avbd_joint& j : joints) { j.body_a = shift(j.body_a); j.body_b = shift(j.body_b); }
+    std::vector<avbd_spring> springs(w->springs, w->springs + w->n_springs);
+    for (avbd_spring& sp : springs) { sp.body_a = shift(sp.body_a); sp.body_b = shift(sp.body_b); }
+    std::vector<avbd_manifold> mans(w->manifolds, w->manifolds + w->n_manifolds);
+    for (avbd_manifold& m : mans) { m.body_a = shift(m.body_a); m.body_b = shift(m.body_b); }
+    std::vector<avbd_tet> tets(w->tets, w->tets + w->n_tets);
+    for (avbd_tet& t : tets)
+        for (int q = 0; q < 4; ++q) t.v[q] = shift(t.v[q]);
+    std::vector<avbd_tri> tris(w->tris, w->tris + w->n_tris);
+    for (avbd_tri& t : tris)
+        for (int q = 0; q < 3; ++q) t.v[q] = shift(t.v[q]);
+    avbd_world x = *w;
+    x.n_bodies = (int32_t)bodies.size();
+    x.bodies = bodies.data();
+    x.joints = joints.data();
+    x.springs = springs.data();
+    x.manifolds = mans.data();
+    x.tets = tets.data();
+    x.tris = tris.data();
+    const int rc = do_upload(c, &x, manifold_capacity);
+    if (rc) return rc;
+    for (int d : spare) c->body_dead[d] = 1;
+    return 0;
 }
 
 // Debug output buffers for the current world (see DebugOut).
@@ -1928,8 +1986,8 @@ int avbd_add_bodies(avbd_ctx* c, const avbd_body* bodies, int32_t n) {
             while (next[s] < e && !c->body_dead[next[s]]) ++next[s];
         }
         if (next[s] >= e)
-            return fail(AVBD_E_CAPACITY, "scene has no free body slot (bodies can be added into slots left by "
-                                         "removals; otherwise upload the world again)");
+            return fail(AVBD_E_CAPACITY, "scene has no free body slot (slots come from removals and from the "
+                                         "spare_bodies option at upload; otherwise upload the world again)");
         slots[k] = next[s]++;
     }
     std::vector<int> stw;

@@ -427,6 +427,7 @@ void Solver::Upload() {
     ExportWorld(rec);
     avbd_world w = rec.view();
     EnsureContext();
+    check(avbd_set_option(ctx_, "spare_bodies", spareBodies), "avbd_set_option");
     check(avbd_upload(ctx_, &w, 0), "avbd_upload");
     ++uploads;
     pendingAdds_.clear();

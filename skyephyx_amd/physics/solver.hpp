@@ -73,6 +73,7 @@ public:
     std::vector<GPULineData> lineData;
     std::vector<GPUDebugPointData> debugPointData;
     bool debugOutput = false;
+    int spareBodies = 0;  // free body slots per upload (option spare_bodies): AddBody between steps fills them
 
     void Start();
     void Clear();

@@ -167,6 +167,7 @@ int main(int argc, char** argv) {
             if (!build(s, argv[2])) return 2;
             avbd_ctx* ctx = s.DeviceContext();
             if (avbd_set_option(ctx, "lanes", 1) || avbd_set_option(ctx, "fem_lanes", 1)) return 3;
+            if (const char* sp = std::getenv("SPARE_BODIES")) s.spareBodies = std::atoi(sp);
             for (int k = 0, n = std::atoi(argv[3]); k < n; ++k) s.Step();
             std::vector<Mesh*> victims;
             for (int a = 8; a < argc; ++a) victims.push_back(s.bodyPtrs.at(std::atoi(argv[a])));

@@ -266,3 +266,19 @@ def test_debug_geometry_fills_line_data(driver, tmp_path, name):
     assert len(lines) > 0
     np.testing.assert_array_equal(lines, gl)
     np.testing.assert_array_equal(points, gp)
+
+
+@pytest.mark.gpu
+def test_spawn_into_spare_slots_without_removals(driver, tmp_path):
+    """With Solver::spareBodies, AddBody between steps needs no removal and
+    no upload."""
+    before, after, n_add = 6, 4, 2
+    out, added = str(tmp_path / "r.bin"), str(tmp_path / "a.bin")
+    env = dict(os.environ, SPARE_BODIES="2")
+    r = subprocess.run([driver, "spawn", "pyramid", str(before), str(after), out, added, str(n_add)],
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "uploads 1" in r.stdout, r.stdout
+    built = read_records(str(tmp_path / "a.bin"))
+    fac = read_records(out)
+    assert len(fac.bodies) == len(built.bodies)

@@ -194,3 +194,46 @@ def test_add_bodies_into_freed_slots():
             res = summary(g, r)
             assert res["pairs_equal"] and res["n_equal"] and res["ids_equal"], res
             assert res["pos_rel"] <= POS_TOL and res["rot"] <= POS_TOL, res
+
+
+def test_spare_slots_change_nothing_and_take_added_bodies():
+    """Option spare_bodies: free slots after every scene's bodies.  They are
+    invisible (downloads, counts, removals) and the steps are bit-identical
+    to a context without them; avbd_add_bodies fills them without any
+    removal, as the oracle's world with the record appended."""
+    world = World.concat([_warm(scenes.pyramid_2d(rows=6), 20), scenes.fem_block(n=3, cell=0.5),
+                          scenes.joint_chain(4)])
+    with Context(lanes=1, fem_lanes=1) as sp, Context(lanes=1, fem_lanes=1) as plain:
+        sp.set_option("spare_bodies", 3)
+        sp.upload(world)
+        plain.upload(world)
+        g = sp.download()
+        for f in ("bodies", "joints", "tets", "manifolds"):
+            assert getattr(g, f).tobytes() == getattr(world, f).tobytes(), f
+        for _ in range(6):
+            sp.step(1)
+            plain.step(1)
+        a, b = sp.download(), plain.download()
+        for f in ("bodies", "joints", "springs", "tets"):
+            assert getattr(a, f).tobytes() == getattr(b, f).tobytes(), f
+        assert len(a.manifolds) == len(b.manifolds)
+        # three boxes into scene 0's spare slots; a fourth has no slot
+        new = a.bodies[[3, 4, 5]].copy()
+        new["pos"] = [(0.0, 12.0, 0.0), (0.5, 14.0, 0.0), (-0.5, 16.0, 0.0)]
+        for f in ("vel", "ang_vel", "prev_vel", "prev_ang_vel"):
+            new[f] = 0.0
+        sp.add_bodies(new)
+        with pytest.raises(Exception) as e:
+            sp.add_bodies(new[:1])
+        assert getattr(e.value, "code", None) == -3
+        got = sp.download()
+        n0 = int((a.bodies["scene"] == 0).sum())
+        want = np.concatenate([a.bodies[:n0], new, a.bodies[n0:]])
+        assert got.bodies.tobytes() == want.tobytes()
+        scene0 = got.extract_scene(0)
+        ref = O.Oracle(scene0, fp64=False, order=O.COLOUR_MAJOR)
+        sp.step(1)
+        ref.step(1)
+        res = summary(sp.download().extract_scene(0), ref.save())
+        assert res["pairs_equal"] and res["n_equal"] and res["ids_equal"], res
+        assert res["pos_rel"] <= POS_TOL and res["rot"] <= POS_TOL, res
