@@ -99,6 +99,8 @@ struct avbd_ctx {
         size_t pcap = 0;
         long long nl = 0, np = 0;
         bool ready = false;
+        void* inst = nullptr;  // avbd_get_instances scratch (owned here)
+        size_t inst_bytes = 0;
     } dbg;
     std::vector<int> scene_bodies;      // per scene, at upload
     int max_scene_bodies = 0;
@@ -243,6 +245,7 @@ struct avbd_ctx {
         have_world = false;
         if (dbg.lines) hipFree(dbg.lines);
         if (dbg.points) hipFree(dbg.points);
+        if (dbg.inst) hipFree(dbg.inst);
         dbg = Debug{};
     }
 };
@@ -1632,6 +1635,7 @@ static int ensure_debug(avbd_ctx* c) {
     if (d.ready) return 0;
     DevWorld& w = c->w;
     const size_t slots = (size_t)w.J + w.SP + w.Mcap + w.T + w.TR;
+    if (slots >= (size_t)INT32_MAX) return fail(AVBD_E_CAPACITY, "world too large for debug output");
     int rc;
     if ((rc = c->alloc(&d.lc, slots + 1)) || (rc = c->alloc(&d.loff, slots + 1)) ||
         (rc = c->alloc(&d.pc, (size_t)w.Mcap + 1)) || (rc = c->alloc(&d.poff, (size_t)w.Mcap + 1)))
@@ -2058,16 +2062,22 @@ int avbd_get_instances(avbd_ctx* c, avbd_instance* out, int32_t cap, int32_t* n)
     *n = n_live;
     if (!out || cap <= 0 || n_live == 0) return 0;
     hipStream_t st = c->stream;
-    struct Scratch {
-        void* p = nullptr;
-        ~Scratch() { if (p) hipFree(p); }
-    } scratch;
     size_t cub_bytes = 0;
     int* nul = nullptr;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, cub_bytes, nul, nul, w.N, st));
     const size_t ints = 2 * (size_t)w.N + 4;
-    HIP_TRY(hipMalloc(&scratch.p, sizeof(avbd_instance) * (size_t)n_live + sizeof(int) * ints + cub_bytes + 256));
-    float4* d_out = static_cast<float4*>(scratch.p);
+    // one scratch buffer for the instances, the live flags, their scan and
+    // the scan's temporary storage; kept for the next call (a renderer asks
+    // every frame)
+    const size_t need = sizeof(avbd_instance) * (size_t)n_live + sizeof(int) * ints + cub_bytes + 256;
+    if (need > c->dbg.inst_bytes) {
+        if (c->dbg.inst) HIP_TRY(hipFree(c->dbg.inst));
+        c->dbg.inst = nullptr;
+        c->dbg.inst_bytes = 0;
+        HIP_TRY(hipMalloc(&c->dbg.inst, need + need / 4));
+        c->dbg.inst_bytes = need + need / 4;
+    }
+    float4* d_out = static_cast<float4*>(c->dbg.inst);
     int* flag = reinterpret_cast<int*>(d_out + (size_t)n_live * 8);
     int* off = flag + w.N + 2;
     void* cub = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(off + w.N + 2) + 255) & ~(uintptr_t)255);
