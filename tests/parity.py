@@ -35,7 +35,7 @@ def compare_manifolds(g, o):
     res = {"pairs_equal": set(tg) == set(to), "ids_equal": True, "n_equal": True, "lambda_rel": 0.0,
            "n_manifolds": len(to), "missing": sorted(set(to) - set(tg))[:5], "extra": sorted(set(tg) - set(to))[:5]}
     lam_all = [np.abs(m["contacts"]["lambda"][: m["n_contacts"]]).max() for m in o.manifolds if m["n_contacts"]]
-    lam_scale = max(lam_all) if lam_all else 1.0
+    lam_scale = max(lam_all) if lam_all and max(lam_all) > 0 else 1.0  # all zero: static pairs only
     worst = 0.0
     d2 = r2 = 0.0
     for key, mo in to.items():

@@ -1,0 +1,95 @@
+"""Edge cases of the device step: a world of static bodies only, an empty
+scene between others, and a scene whose colouring needs more than 64
+colours (the colouring's search beyond its 64-bit mask).  Each against the
+oracle (colours bit-exact, positions within the parity bar)."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from parity import POS_TOL, colouring_ok, summary
+from skyephyx_amd import scenes
+from skyephyx_amd.capi import Context
+from skyephyx_amd.world import MODEL_CUBE, World, WorldBuilder
+
+pytestmark = pytest.mark.gpu
+
+
+def _static_only():
+    b = WorldBuilder()
+    b.add_body(MODEL_CUBE, 1.0, 0.5, (0.0, -10.0, 0.0), (40.0, 1.0, 40.0), is_static=True)
+    b.add_body(MODEL_CUBE, 1.0, 0.5, (0.0, -9.0, 0.0), (1.0, 1.0, 1.0), is_static=True)
+    return b.build()
+
+
+def _empty_scene():
+    b = WorldBuilder()
+    w = b.build()  # a scene with no bodies
+    assert len(w.bodies) == 0
+    return w
+
+
+def _spring_clique(n=70):
+    """n boxes on a 10 x k grid (3 m apart, inside the scene's 50 m bounds),
+    every pair tied by a spring: the greedy colouring gives every box its own
+    colour."""
+    b = WorldBuilder()
+    ids = [b.add_body(MODEL_CUBE, 1.0, 0.5, (3.0 * (i % 10) - 15.0, 3.0 * (i // 10), 0.0), (1.0, 1.0, 1.0))
+           for i in range(n)]
+    for i in range(n):
+        for j in range(i + 1, n):
+            b.add_spring(ids[i], (0.0, 0.0, 0.0), ids[j], (0.0, 0.0, 0.0), 10.0)
+    return b.build()
+
+
+def test_static_only_world():
+    """Nothing moves; the broad phase still pairs touching static boxes (the
+    reference's loop has no static test, solver.cpp:233-256), as the oracle."""
+    world = _static_only()
+    o = O.Oracle(world, fp64=False, order=O.COLOUR_MAJOR)
+    o.step(3)
+    ref = o.save()
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(world)
+        ctx.step(3)
+        g = ctx.download()
+        np.testing.assert_array_equal(g.bodies["pos"], world.bodies["pos"])
+        np.testing.assert_array_equal(g.bodies["rot"], world.bodies["rot"])
+        res = summary(g, ref)
+        assert res["pairs_equal"] and res["n_equal"] and res["ids_equal"], res
+        assert len(g.manifolds) == len(ref.manifolds) == 1
+        assert np.all(ctx.colours(len(g.bodies)) == -1)
+
+
+def test_empty_scene_between_scenes():
+    a, c = scenes.pyramid_2d(rows=4), scenes.joint_chain(3)
+    world = World.concat([a, _empty_scene(), c])
+    assert len(world.scenes) == 3
+    with Context(lanes=1, fem_lanes=1) as ctx, Context(lanes=1, fem_lanes=1) as alone:
+        ctx.upload(world)
+        alone.upload(c)
+        ctx.step(4)
+        alone.step(4)
+        g = ctx.download()
+        ctx.step_scenes([1], 2)  # only the empty scene: nothing moves
+        h = ctx.download()
+        solo = alone.download()
+    np.testing.assert_array_equal(g.bodies["pos"], h.bodies["pos"])
+    np.testing.assert_array_equal(g.extract_scene(2).bodies["pos"], solo.bodies["pos"])
+
+
+def test_more_than_64_colours():
+    world = _spring_clique(70)
+    o = O.Oracle(world, fp64=False, order=O.COLOUR_MAJOR)
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(world)
+        for k in range(3):
+            ctx.step(1)
+            o.step(1)
+            g, ref = ctx.download(), o.save()
+            gcol = ctx.colours(len(g.bodies))
+            ocol, nc = o.colours(len(ref.bodies))
+            assert nc == 70 and gcol.max() == 69
+            np.testing.assert_array_equal(gcol, ocol)
+            assert colouring_ok(g, gcol)
+            res = summary(g, ref)
+            assert res["pos_rel"] <= POS_TOL and res["rot"] <= POS_TOL, (k, res)
