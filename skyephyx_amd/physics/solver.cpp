@@ -68,6 +68,7 @@ void Solver::Clear() {
     emergencyStop = false;
     forcesStale_ = false;
     structureDirty_ = true;
+    pendingAdds_.clear();
     timingAccumulator.clear();
 }
 
@@ -307,11 +308,200 @@ avbd_ctx* Solver::DeviceContext() {
     return ctx_;
 }
 
+namespace {
+
+// A manifold's contact points and row state from its record.
+void set_contacts(Manifold* m, const avbd_manifold& r) {
+    m->numContactPoints = r.n_contacts;
+    m->friction = r.friction;
+    for (int c = 0; c < 8; ++c) {
+        const avbd_contact& ct = r.contacts[c];
+        ContactPoint& cp = m->contactPoints[c];
+        cp.id = ct.id;
+        cp.rA = get3(ct.r_a);
+        cp.rB = get3(ct.r_b);
+        cp.normal = get3(ct.normal);
+        cp.penetration = ct.penetration;
+        m->contactInfos[c].stick = ct.stick != 0;
+        for (int k = 0; k < 3; ++k) {
+            m->constraintPoints[3 * c + k].lambda = c < r.n_contacts ? ct.lambda[k] : 0.f;
+            m->constraintPoints[3 * c + k].penalty = c < r.n_contacts ? ct.penalty[k] : 0.f;
+        }
+    }
+}
+
+void set_row(ConstraintPointProperties& p, float stiffness, float fracture, float fmin, float fmax, float lambda,
+             float penalty) {
+    p.stiffness = stiffness;
+    p.fracture = fracture;
+    p.fminMagnitude = fmin;
+    p.fmaxMagnitude = fmax;
+    p.lambda = lambda;
+    p.penalty = penalty;
+}
+
+}  // namespace
+
 void Solver::SaveSnapshot(const std::string& path) {
     WorldRecords r;
     ExportWorld(r);
     avbd_world w = r.view();
     if (avbd_snapshot_write(path.c_str(), &w) != 0) throw std::runtime_error(avbd_last_error());
+}
+
+void Solver::LoadSnapshot(const std::string& path) {
+    avbd_world cnt{};
+    if (avbd_snapshot_counts(path.c_str(), &cnt) != 0) throw std::runtime_error(avbd_last_error());
+    if (cnt.n_scenes != 1)
+        throw std::runtime_error("Solver::LoadSnapshot: a solver holds one scene, the snapshot has " +
+                                 std::to_string(cnt.n_scenes));
+    WorldRecords r;
+    r.bodies.resize(cnt.n_bodies);
+    r.joints.resize(cnt.n_joints);
+    r.springs.resize(cnt.n_springs);
+    r.manifolds.resize(cnt.n_manifolds);
+    r.tets.resize(cnt.n_tets);
+    r.tris.resize(cnt.n_tris);
+    avbd_world w = r.view();
+    if (avbd_snapshot_read(path.c_str(), &w) != 0) throw std::runtime_error(avbd_last_error());
+    const int nb = cnt.n_bodies;
+    auto body = [&](int32_t i, bool may_be_world) -> Mesh* {
+        if (i == -1 && may_be_world) return nullptr;
+        if (i < 0 || i >= nb) throw std::runtime_error("Solver::LoadSnapshot: a record names a body out of range");
+        return solverBodies[i].get();
+    };
+    for (const avbd_body& b : r.bodies)
+        if (b.scene != 0) throw std::runtime_error("Solver::LoadSnapshot: a body of another scene");
+
+    Clear();
+    const avbd_scene& sc = r.scene;
+    stepValue = sc.dt;
+    numIterations = sc.iterations;
+    alpha = sc.alpha;
+    beta = sc.beta;
+    gamma = sc.gamma;
+    onPenetrationPenalty = sc.on_penetration_penalty;
+    postStabilization = sc.post_stabilization != 0;
+    projectionMode = static_cast<EigenProjectionMode>(sc.projection_mode);
+    gravity = get3(sc.gravity);
+    boundsMin = get3(sc.bounds_min);
+    boundsMax = get3(sc.bounds_max);
+    emergencyStop = sc.emergency_stop != 0;
+
+    // The objects take the records' serials (the constructors' serial
+    // requests are answered with -1 while mirroring_ is set).
+    mirroring_ = true;
+    for (const avbd_body& b : r.bodies) {
+        auto mesh = std::make_unique<Mesh>(this, ModelType_Cube);
+        Mesh& m = *mesh;
+        m.transform.SetPosition(get3(b.pos));
+        m.transform.SetRotation(Quaternionf(b.rot[0], b.rot[1], b.rot[2], b.rot[3]));
+        m.transform.SetScale(get3(b.scale));
+        m.velocity = get3(b.vel);
+        m.angularVelocity = get3(b.ang_vel);
+        m.prevVelocity = get3(b.prev_vel);
+        m.prevAngularVelocity = get3(b.prev_ang_vel);
+        m.isStatic = (b.flags & AVBD_BODY_STATIC) != 0;
+        m.isParticle = (b.flags & AVBD_BODY_PARTICLE) != 0;
+        m.mass = b.mass;
+        const float vol = (b.scale[0] * b.scale[1]) * b.scale[2];
+        m.density = vol > 0.f ? b.mass / vol : 0.f;
+        m.friction = b.friction;
+        m.detectionRadius = b.radius;
+        m.inertiaTensorBody = Matrix3f::Zero();
+        m.inertiaTensorBodyInv = Matrix3f::Zero();
+        for (int k = 0; k < 3; ++k) {
+            m.inertiaTensorBody(k, k) = b.inertia[k];
+            m.inertiaTensorBodyInv(k, k) = b.inertia[k] != 0.f ? 1.f / b.inertia[k] : 0.f;
+        }
+        m.solverIndex = static_cast<int>(solverBodies.size());
+        solverBodies.push_back(std::move(mesh));
+    }
+    // Forces in creation order, so that each body's list is in serial order
+    // as it would be had they been created one by one.
+    struct Item {
+        int64_t serial;
+        int kind, i;
+    };
+    std::vector<Item> order;
+    for (int i = 0; i < (int)r.joints.size(); ++i) order.push_back({r.joints[i].serial, 0, i});
+    for (int i = 0; i < (int)r.springs.size(); ++i) order.push_back({r.springs[i].serial, 1, i});
+    for (int i = 0; i < (int)r.manifolds.size(); ++i) order.push_back({r.manifolds[i].serial, 2, i});
+    std::sort(order.begin(), order.end(), [](const Item& a, const Item& b) { return a.serial < b.serial; });
+    int64_t max_force = -1;
+    for (const Item& it : order) {
+        Force* f = nullptr;
+        if (it.kind == 0) {
+            const avbd_joint& j = r.joints[it.i];
+            auto owned = std::make_unique<Joint>(this, body(j.body_a, true), get3(j.r_a), body(j.body_b, false),
+                                                 get3(j.r_b));
+            owned->restRotation = Quaternionf(j.rest_rot[0], j.rest_rot[1], j.rest_rot[2], j.rest_rot[3]);
+            owned->torqueArm = j.torque_arm;
+            for (int k = 0; k < 6; ++k)
+                set_row(owned->constraintPoints[k], j.stiffness[k], j.fracture[k], j.fmin[k], j.fmax[k], j.lambda[k],
+                        j.penalty[k]);
+            owned->disabled = (j.flags & AVBD_FORCE_DISABLED) != 0;
+            f = AddForce(std::move(owned));
+        } else if (it.kind == 1) {
+            const avbd_spring& s = r.springs[it.i];
+            auto owned = std::make_unique<Spring>(this, body(s.body_a, true), get3(s.r_a), body(s.body_b, false),
+                                                  get3(s.r_b), s.stiffness, s.rest_length, s.fmin == 0.f);
+            set_row(owned->constraintPoints[0], s.stiffness, s.fracture, s.fmin, s.fmax, s.lambda, s.penalty);
+            owned->disabled = (s.flags & AVBD_FORCE_DISABLED) != 0;
+            f = AddForce(std::move(owned));
+        } else {
+            const avbd_manifold& mr = r.manifolds[it.i];
+            auto owned = std::make_unique<Manifold>(this, body(mr.body_a, false), body(mr.body_b, false));
+            set_contacts(owned.get(), mr);
+            f = AddForce(std::move(owned));
+        }
+        f->serial = it.serial;
+        max_force = std::max<int64_t>(max_force, it.serial);
+    }
+    // Energies likewise: the constructors derive the rest state from the
+    // current positions, the records' rest state replaces it.
+    struct EItem {
+        int32_t serial;
+        int kind, i;
+    };
+    std::vector<EItem> eorder;
+    for (int i = 0; i < (int)r.tets.size(); ++i) eorder.push_back({r.tets[i].serial, 0, i});
+    for (int i = 0; i < (int)r.tris.size(); ++i) eorder.push_back({r.tris[i].serial, 1, i});
+    std::sort(eorder.begin(), eorder.end(), [](const EItem& a, const EItem& b) { return a.serial < b.serial; });
+    // Young's modulus and Poisson's ratio back from the Lame parameters
+    auto young = [](float mu, float lam) { return mu + lam != 0.f ? mu * (3.f * lam + 2.f * mu) / (lam + mu) : 0.f; };
+    auto poisson = [](float mu, float lam) { return mu + lam != 0.f ? lam / (2.f * (lam + mu)) : 0.f; };
+    int32_t max_energy = -1;
+    for (const EItem& it : eorder) {
+        Energy* e = nullptr;
+        if (it.kind == 0) {
+            const avbd_tet& t = r.tets[it.i];
+            auto owned = std::make_unique<NeoHookeanFEM>(this, body(t.v[0], false), body(t.v[1], false),
+                                                         body(t.v[2], false), body(t.v[3], false),
+                                                         young(t.mu, t.lambda), poisson(t.mu, t.lambda));
+            std::memcpy(owned->DmInv, t.dm_inv, sizeof(owned->DmInv));
+            owned->restVolume = t.rest_volume;
+            owned->lameMu = t.mu;
+            owned->lameLambda = t.lambda;
+            e = AddEnergy(std::move(owned));
+        } else {
+            const avbd_tri& t = r.tris[it.i];
+            auto owned = std::make_unique<STVKFEM>(this, body(t.v[0], false), body(t.v[1], false), body(t.v[2], false),
+                                                   young(t.mu, t.lambda), poisson(t.mu, t.lambda));
+            std::memcpy(owned->DmInv2D, t.dm_inv, sizeof(owned->DmInv2D));
+            owned->restArea = t.rest_area;
+            owned->lameMu = t.mu;
+            owned->lameLambda = t.lambda;
+            e = AddEnergy(std::move(owned));
+        }
+        e->serial = it.serial;
+        max_energy = std::max(max_energy, it.serial);
+    }
+    mirroring_ = false;
+    nextForceSerial_ = max_force + 1;
+    nextEnergySerial_ = max_energy + 1;
+    structureDirty_ = true;
+    RebuildPtrCaches();
 }
 
 void Solver::ExportWorld(WorldRecords& out) {
@@ -618,22 +808,7 @@ void Solver::SyncForces() {
         }
         if (!m) continue;
         seen[r.serial] = true;
-        m->numContactPoints = r.n_contacts;
-        m->friction = r.friction;
-        for (int c = 0; c < 8; ++c) {
-            const avbd_contact& ct = r.contacts[c];
-            ContactPoint& cp = m->contactPoints[c];
-            cp.id = ct.id;
-            cp.rA = get3(ct.r_a);
-            cp.rB = get3(ct.r_b);
-            cp.normal = get3(ct.normal);
-            cp.penetration = ct.penetration;
-            m->contactInfos[c].stick = ct.stick != 0;
-            for (int k = 0; k < 3; ++k) {
-                m->constraintPoints[3 * c + k].lambda = c < r.n_contacts ? ct.lambda[k] : 0.f;
-                m->constraintPoints[3 * c + k].penalty = c < r.n_contacts ? ct.penalty[k] : 0.f;
-            }
-        }
+        set_contacts(m, r);
     }
     // forces the device retired (contacts without points, fractured springs
     // and joints, forces of removed bodies)

@@ -132,6 +132,13 @@ public:
     // ExportWorld written as a world snapshot file (avbd_snapshot_write);
     // throws std::runtime_error with the library's message on failure.
     void SaveSnapshot(const std::string& path);
+    // Replaces the world with a one-scene snapshot file (avbd_snapshot_read):
+    // scene parameters, bodies (Mesh::modelType is not recorded: cubes),
+    // joints, springs, contact manifolds with their row state, and energies,
+    // each keeping its creation serial, so that ExportWorld gives the file's
+    // records back.  The next Step uploads the world.  Throws
+    // std::runtime_error on a bad file or a snapshot of several scenes.
+    void LoadSnapshot(const std::string& path);
 
     // Used by the Force / Energy constructors and the Mesh destructor.
     int64_t TakeForceSerial();

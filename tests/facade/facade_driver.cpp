@@ -203,6 +203,25 @@ int main(int argc, char** argv) {
             std::fclose(f);
             return 0;
         }
+        if (mode == "reload" && argc == 4) {
+            // reload <snapshot> <out>: Solver::LoadSnapshot, then the records
+            Solver s;
+            s.LoadSnapshot(argv[2]);
+            std::printf("bodies %zu forces %zu energies %zu\n", s.bodyPtrs.size(), s.forcePtrs.size(),
+                        s.energyPtrs.size());
+            return write_records(s, argv[3]);
+        }
+        if (mode == "reloadstep" && argc == 7) {
+            // reloadstep <snapshot> <steps> <lanes> <fem lanes> <out>
+            Solver s;
+            s.LoadSnapshot(argv[2]);
+            avbd_ctx* ctx = s.DeviceContext();
+            if (avbd_set_option(ctx, "lanes", std::atoi(argv[4])) || avbd_set_option(ctx, "fem_lanes", std::atoi(argv[5])))
+                return 3;
+            for (int k = 0, n = std::atoi(argv[3]); k < n; ++k) s.Step();
+            std::printf("uploads %lld\n", (long long)s.uploads);
+            return write_records(s, argv[6]);
+        }
         if (mode == "snapshot" && argc == 4) {
             Solver s;
             if (!build(s, argv[2])) return 2;

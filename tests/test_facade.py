@@ -19,6 +19,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import oracle_py as O
 from skyephyx_amd import capi, scenes
 from skyephyx_amd.world import (BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, SCENE_DTYPE, SPRING_DTYPE, TET_DTYPE,
                                 TRI_DTYPE, World)
@@ -91,6 +92,46 @@ def test_facade_snapshot_holds_its_records(driver, tmp_path, name):
         assert getattr(c, arr).tobytes() == getattr(s, arr).tobytes(), arr
 
 
+def _warm(world, steps):
+    o = O.Oracle(world, fp64=False, order=O.NATURAL)
+    o.step(steps)
+    return o.save()
+
+
+# one-scene worlds in mid-flight: contacts with warm-started rows, joints,
+# springs (one a rope), tets and triangles
+RELOAD = {
+    "pyramid_settled": lambda: _warm(scenes.pyramid_2d(), 20),
+    "spring_bridge": lambda: _warm(scenes.spring_bridge(4), 5),
+    "lattice_mix": lambda: _warm(scenes.box_lattice(shape=(3, 2, 2), spacing=3.0, n_chain=4, n_springs=3,
+                                                    bounds=50.0), 10),
+    "fem_block": lambda: _warm(scenes.fem_block(n=3, cell=0.5), 3),
+    "cloth": lambda: _warm(scenes.cloth_sheet(n=4, spacing=0.5), 3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(RELOAD))
+def test_facade_load_snapshot_gives_its_records_back(driver, tmp_path, name):
+    """Solver::LoadSnapshot rebuilds the objects (bodies, forces with their
+    row state, contacts, energies with their rest state, creation serials):
+    ExportWorld then gives the file's records byte for byte."""
+    w = RELOAD[name]()
+    snap, out = str(tmp_path / "w.avbdsnap"), str(tmp_path / "r.bin")
+    capi.save_snapshot(w, snap)
+    r = subprocess.run([driver, "reload", snap, out], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    back = read_records(out)
+    for arr in ("scenes", "bodies", "joints", "springs", "manifolds", "tets", "tris"):
+        assert getattr(back, arr).tobytes() == getattr(w, arr).tobytes(), arr
+
+
+def test_facade_load_snapshot_refuses_several_scenes(driver, tmp_path):
+    snap = str(tmp_path / "two.avbdsnap")
+    capi.save_snapshot(World.concat([scenes.pyramid_2d(rows=3), scenes.joint_chain(3)]), snap)
+    r = subprocess.run([driver, "reload", snap, str(tmp_path / "r.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 4 and "one scene" in r.stdout, r.stdout
+
+
 @pytest.mark.parametrize("name", SCENES)
 def test_records_match_the_python_builder(driver, tmp_path, name):
     out = str(tmp_path / "w.bin")
@@ -144,6 +185,34 @@ def test_step_matches_the_c_abi(driver, tmp_path, name):
     _same(fac.joints, ref.joints, ("body_a", "body_b", "serial", "flags", "lambda", "penalty"), "joints")
     _same(fac.springs, ref.springs, ("body_a", "body_b", "serial", "flags", "lambda", "penalty"), "springs")
     assert len(fac.tets) == len(ref.tets) and len(fac.tris) == len(ref.tris)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["lattice_mix", "pyramid_settled"])
+def test_reloaded_solver_steps_like_the_c_abi(driver, tmp_path, name):
+    """A world reloaded with Solver::LoadSnapshot continues exactly as the
+    same snapshot loaded into a context of the C ABI."""
+    from skyephyx_amd.capi import Context
+
+    steps = 6
+    snap, out = str(tmp_path / "w.avbdsnap"), str(tmp_path / "s.bin")
+    capi.save_snapshot(RELOAD[name](), snap)
+    r = subprocess.run([driver, "reloadstep", snap, str(steps), "1", "1", out], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "uploads 1" in r.stdout, r.stdout
+    fac = read_records(out)
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.load_snapshot(snap)
+        for _ in range(steps):
+            ctx.step(1)
+        ref = ctx.download()
+    assert len(fac.bodies) == len(ref.bodies)
+    _same(fac.bodies, ref.bodies, ("pos", "rot", "vel", "ang_vel", "prev_vel", "prev_ang_vel"), "bodies")
+    assert len(fac.manifolds) == len(ref.manifolds)
+    _same(fac.manifolds, ref.manifolds, ("body_a", "body_b", "serial", "n_contacts"), "manifolds")
+    _same(fac.joints, ref.joints, ("body_a", "body_b", "serial", "flags", "lambda", "penalty"), "joints")
+    _same(fac.springs, ref.springs, ("body_a", "body_b", "serial", "flags", "lambda", "penalty"), "springs")
 
 
 @pytest.mark.gpu
