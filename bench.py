@@ -270,8 +270,10 @@ def main():
         ctx.set_option("fused_scenes", 1)
     if args.debug_output:
         ctx.set_option("debug_output", 1)
+    t0 = time.perf_counter()
     ctx.upload(world)
-    log("[bench] uploaded")
+    ctx.synchronize()
+    log(f"[bench] uploaded in {time.perf_counter() - t0:.2f}s")
     for i in range(args.warmup):
         ctx.step(1)
     ctx.synchronize()

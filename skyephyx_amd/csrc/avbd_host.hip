@@ -557,32 +557,27 @@ int upload_energies(avbd_ctx* c, int N, const avbd_tet* tets, int T, const avbd_
         rd[i * 2 + 0] = make_float4(t.dm_inv[0], t.dm_inv[1], t.dm_inv[2], t.dm_inv[3]);
         rd[i * 2 + 1] = make_float4(t.rest_area, t.mu, t.lambda, 0.f);
     }
-    // the element stream in incidence order
-    const size_t nE = ent.size();
-    std::vector<float4> ea(nE), eb(nE), ec(nE, make_float4(0.f, 0.f, 0.f, 0.f));
-    std::vector<int4> evv(nE, make_int4(0, 0, 0, 0));
-    for (int k = 0; k < eoff[N]; ++k) {
-        const uint32_t e = ent[k];
-        const uint32_t idx = e & 0x1FFFFFFFu;
-        if (e >> 31) {
-            ea[k] = rd[idx * 2 + 0];
-            eb[k] = rd[idx * 2 + 1];
-            evv[k] = rv[idx];
-        } else {
-            ea[k] = td[idx * 3 + 0];
-            eb[k] = td[idx * 3 + 1];
-            ec[k] = td[idx * 3 + 2];
-            evv[k] = tv[idx];
-        }
-    }
     int rc;
     if ((rc = upload_vec(c, const_cast<int*>(w.e_off), eoff)) || (rc = upload_vec(c, const_cast<uint32_t*>(w.e_ent), ent)) ||
         (rc = upload_vec(c, w.t_v, tv)) || (rc = upload_vec(c, w.t_d, td)) || (rc = upload_vec(c, w.r_v, rv)) ||
-        (rc = upload_vec(c, w.r_d, rd)) || (rc = upload_vec(c, w.ev_a, ea)) || (rc = upload_vec(c, w.ev_b, eb)) ||
-        (rc = upload_vec(c, w.ev_c, ec)) || (rc = upload_vec(c, w.ev_v, evv)))
+        (rc = upload_vec(c, w.r_d, rd)))
         return rc;
     w.T = T;
     w.TR = TR;
+    // the element stream in incidence order, gathered on the device (a
+    // quarter of the host work and of the bytes over PCIe: each element's
+    // records cross once, not once per vertex)
+    if (eoff[N] > 0) {
+        hipLaunchKernelGGL(k_element_stream, dim3(grid_for(eoff[N])), dim3(256), 0, c->stream, w, eoff[N]);
+        HIP_TRY(hipGetLastError());
+    } else {
+        // no elements: the one padding entry of each stream array is zero
+        const std::vector<float4> z4(1, make_float4(0.f, 0.f, 0.f, 0.f));
+        const std::vector<int4> zi(1, make_int4(0, 0, 0, 0));
+        if ((rc = upload_vec(c, w.ev_a, z4)) || (rc = upload_vec(c, w.ev_b, z4)) || (rc = upload_vec(c, w.ev_c, z4)) ||
+            (rc = upload_vec(c, w.ev_v, zi)))
+            return rc;
+    }
     c->tet_serial.resize(T);
     c->tri_serial.resize(TR);
     c->tet_dead.assign(T, 0);

@@ -1961,6 +1961,27 @@ __global__ void k_kill_bodies(DevWorld w, const int* list, int n) {
 
 // Element vertex ids renumbered (-1 for a removed body, kept by later
 // removals; such an element is removed with its body and no longer read).
+// The element stream in incidence order from the element records: entry k
+// of e_ent (kind, local vertex, element index) takes its element's rest-shape
+// records and vertex ids.  Run once per upload over the n stream entries.
+__global__ void k_element_stream(DevWorld w, int n) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t e = w.e_ent[k];
+        const uint32_t idx = e & 0x1FFFFFFFu;
+        if (EENT_TRI(e)) {
+            w.ev_a[k] = w.r_d[(size_t)idx * 2 + 0];
+            w.ev_b[k] = w.r_d[(size_t)idx * 2 + 1];
+            w.ev_c[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            w.ev_v[k] = w.r_v[idx];
+        } else {
+            w.ev_a[k] = w.t_d[(size_t)idx * 3 + 0];
+            w.ev_b[k] = w.t_d[(size_t)idx * 3 + 1];
+            w.ev_c[k] = w.t_d[(size_t)idx * 3 + 2];
+            w.ev_v[k] = w.t_v[idx];
+        }
+    }
+}
+
 __global__ void k_remap_elements(DevWorld w, BodyRemap rm) {
     GRID_STRIDE(t, w.T + w.TR) {
         int4* v = t < w.T ? &w.t_v[t] : &w.r_v[t - w.T];
