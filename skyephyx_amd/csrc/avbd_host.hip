@@ -524,7 +524,9 @@ int upload_energies(avbd_ctx* c, int N, const avbd_tet* tets, int T, const avbd_
     refs.reserve(T + TR);
     for (int i = 0; i < T; ++i) refs.push_back({tets[i].serial, 0, i});
     for (int i = 0; i < TR; ++i) refs.push_back({tris[i].serial, 1, i});
-    std::stable_sort(refs.begin(), refs.end(), [](const ERef& x, const ERef& y) { return x.serial < y.serial; });
+    const auto by_serial = [](const ERef& x, const ERef& y) { return x.serial < y.serial; };
+    if (!std::is_sorted(refs.begin(), refs.end(), by_serial))  // builders write them in creation order
+        std::stable_sort(refs.begin(), refs.end(), by_serial);
     std::vector<int> cnt(N + 1, 0);
     for (const ERef& r : refs) {
         if (r.kind == 0) for (int k = 0; k < 4; ++k) cnt[tets[r.idx].v[k]]++;
@@ -774,8 +776,8 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
         // manifolds must be in creation order
         std::vector<int> order(M);
         for (int i = 0; i < M; ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int x, int y) { return in->manifolds[x].serial < in->manifolds[y].serial; });
+        const auto by_serial = [&](int x, int y) { return in->manifolds[x].serial < in->manifolds[y].serial; };
+        if (!std::is_sorted(order.begin(), order.end(), by_serial)) std::stable_sort(order.begin(), order.end(), by_serial);
         std::vector<int> a(M), b(M), n(M);
         std::vector<long long> ser(M);
         std::vector<float> fr(M);
