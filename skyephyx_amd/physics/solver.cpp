@@ -364,14 +364,21 @@ void Solver::LoadSnapshot(const std::string& path) {
     r.tris.resize(cnt.n_tris);
     avbd_world w = r.view();
     if (avbd_snapshot_read(path.c_str(), &w) != 0) throw std::runtime_error(avbd_last_error());
+    // Every check before the world is replaced: a bad file leaves it as it was.
     const int nb = cnt.n_bodies;
-    auto body = [&](int32_t i, bool may_be_world) -> Mesh* {
-        if (i == -1 && may_be_world) return nullptr;
-        if (i < 0 || i >= nb) throw std::runtime_error("Solver::LoadSnapshot: a record names a body out of range");
-        return solverBodies[i].get();
-    };
-    for (const avbd_body& b : r.bodies)
-        if (b.scene != 0) throw std::runtime_error("Solver::LoadSnapshot: a body of another scene");
+    auto in_range = [&](int32_t i, bool may_be_world) { return (i == -1 && may_be_world) || (i >= 0 && i < nb); };
+    bool ok = true;
+    for (const avbd_body& b : r.bodies) ok = ok && b.scene == 0;
+    for (const avbd_joint& j : r.joints) ok = ok && in_range(j.body_a, true) && in_range(j.body_b, false);
+    for (const avbd_spring& sp : r.springs) ok = ok && in_range(sp.body_a, true) && in_range(sp.body_b, false);
+    for (const avbd_manifold& m : r.manifolds)
+        ok = ok && in_range(m.body_a, false) && in_range(m.body_b, false) && m.n_contacts >= 0 && m.n_contacts <= 8;
+    for (const avbd_tet& t : r.tets)
+        for (int k = 0; k < 4; ++k) ok = ok && in_range(t.v[k], false);
+    for (const avbd_tri& t : r.tris)
+        for (int k = 0; k < 3; ++k) ok = ok && in_range(t.v[k], false);
+    if (!ok) throw std::runtime_error("Solver::LoadSnapshot: a record names a body out of range or of another scene");
+    auto body = [&](int32_t i) -> Mesh* { return i == -1 ? nullptr : solverBodies[i].get(); };
 
     Clear();
     const avbd_scene& sc = r.scene;
@@ -433,7 +440,7 @@ void Solver::LoadSnapshot(const std::string& path) {
         Force* f = nullptr;
         if (it.kind == 0) {
             const avbd_joint& j = r.joints[it.i];
-            auto owned = std::make_unique<Joint>(this, body(j.body_a, true), get3(j.r_a), body(j.body_b, false),
+            auto owned = std::make_unique<Joint>(this, body(j.body_a), get3(j.r_a), body(j.body_b),
                                                  get3(j.r_b));
             owned->restRotation = Quaternionf(j.rest_rot[0], j.rest_rot[1], j.rest_rot[2], j.rest_rot[3]);
             owned->torqueArm = j.torque_arm;
@@ -444,14 +451,14 @@ void Solver::LoadSnapshot(const std::string& path) {
             f = AddForce(std::move(owned));
         } else if (it.kind == 1) {
             const avbd_spring& s = r.springs[it.i];
-            auto owned = std::make_unique<Spring>(this, body(s.body_a, true), get3(s.r_a), body(s.body_b, false),
+            auto owned = std::make_unique<Spring>(this, body(s.body_a), get3(s.r_a), body(s.body_b),
                                                   get3(s.r_b), s.stiffness, s.rest_length, s.fmin == 0.f);
             set_row(owned->constraintPoints[0], s.stiffness, s.fracture, s.fmin, s.fmax, s.lambda, s.penalty);
             owned->disabled = (s.flags & AVBD_FORCE_DISABLED) != 0;
             f = AddForce(std::move(owned));
         } else {
             const avbd_manifold& mr = r.manifolds[it.i];
-            auto owned = std::make_unique<Manifold>(this, body(mr.body_a, false), body(mr.body_b, false));
+            auto owned = std::make_unique<Manifold>(this, body(mr.body_a), body(mr.body_b));
             set_contacts(owned.get(), mr);
             f = AddForce(std::move(owned));
         }
@@ -476,8 +483,8 @@ void Solver::LoadSnapshot(const std::string& path) {
         Energy* e = nullptr;
         if (it.kind == 0) {
             const avbd_tet& t = r.tets[it.i];
-            auto owned = std::make_unique<NeoHookeanFEM>(this, body(t.v[0], false), body(t.v[1], false),
-                                                         body(t.v[2], false), body(t.v[3], false),
+            auto owned = std::make_unique<NeoHookeanFEM>(this, body(t.v[0]), body(t.v[1]),
+                                                         body(t.v[2]), body(t.v[3]),
                                                          young(t.mu, t.lambda), poisson(t.mu, t.lambda));
             std::memcpy(owned->DmInv, t.dm_inv, sizeof(owned->DmInv));
             owned->restVolume = t.rest_volume;
@@ -486,7 +493,7 @@ void Solver::LoadSnapshot(const std::string& path) {
             e = AddEnergy(std::move(owned));
         } else {
             const avbd_tri& t = r.tris[it.i];
-            auto owned = std::make_unique<STVKFEM>(this, body(t.v[0], false), body(t.v[1], false), body(t.v[2], false),
+            auto owned = std::make_unique<STVKFEM>(this, body(t.v[0]), body(t.v[1]), body(t.v[2]),
                                                    young(t.mu, t.lambda), poisson(t.mu, t.lambda));
             std::memcpy(owned->DmInv2D, t.dm_inv, sizeof(owned->DmInv2D));
             owned->restArea = t.rest_area;

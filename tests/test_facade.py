@@ -125,11 +125,16 @@ def test_facade_load_snapshot_gives_its_records_back(driver, tmp_path, name):
         assert getattr(back, arr).tobytes() == getattr(w, arr).tobytes(), arr
 
 
-def test_facade_load_snapshot_refuses_several_scenes(driver, tmp_path):
+def test_facade_load_snapshot_refuses_bad_worlds(driver, tmp_path):
     snap = str(tmp_path / "two.avbdsnap")
     capi.save_snapshot(World.concat([scenes.pyramid_2d(rows=3), scenes.joint_chain(3)]), snap)
     r = subprocess.run([driver, "reload", snap, str(tmp_path / "r.bin")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 4 and "one scene" in r.stdout, r.stdout
+    bad = scenes.joint_chain(3)
+    bad.joints["body_b"][1] = len(bad.bodies)  # names a body the file does not hold
+    capi.save_snapshot(bad, snap)
+    r = subprocess.run([driver, "reload", snap, str(tmp_path / "r.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 4 and "out of range" in r.stdout, r.stdout
 
 
 @pytest.mark.parametrize("name", SCENES)
