@@ -25,8 +25,9 @@ struct LaneTuner {
     static constexpr int kSamples = 2;
 
     struct Entry {
-        int best = 0;  // chosen width, 0 while tuning
-        int next = 0;  // next candidate index to sample
+        int best = 0;       // chosen width, 0 while tuning
+        int next = 0;       // next candidate index to sample
+        unsigned mask = 0;  // candidates sampled (bit c: kCands[c]); set at first use
         int count[kNumCands] = {0};
         double us_per_body[kNumCands] = {0.0};
     };
@@ -68,14 +69,34 @@ struct LaneTuner {
     // events the caller records around the launch; else -1.
     int choose(int cls, int n, int* measure) {
         *measure = -1;
-        Entry& e = table[key(cls, n)];
+        const int k = key(cls, n);
+        Entry& e = table[k];
         if (!enabled || e.best) return e.best ? e.best : 8;
-        const int ci = e.next % kNumCands;
-        e.next++;
+        if (!e.mask) e.mask = seed_mask(k);
+        int ci = e.next % kNumCands;
+        while (!((e.mask >> ci) & 1u)) ci = (ci + 1) % kNumCands;
+        e.next = ci + 1;
         Pending p{key(cls, n), ci, n, event(), event()};
         pending.push_back(p);
         *measure = int(pending.size()) - 1;
         return kCands[ci];
+    }
+
+    // Candidates for a key first seen: next to a tuned key of the same class
+    // (launches within a factor of two in size) only that key's width and
+    // its half and double, so a bucket that crosses a power of two while the
+    // world runs costs a few sampled launches, not a sweep of every width.
+    unsigned seed_mask(int k) const {
+        for (int d : {-1, 1}) {
+            auto it = table.find(k + d);
+            if (it == table.end() || !it->second.best || (k + d) / 64 != k / 64) continue;
+            const int b = it->second.best;
+            unsigned m = 0;
+            for (int c = 0; c < kNumCands; ++c)
+                if (kCands[c] == b || 2 * kCands[c] == b || kCands[c] == 2 * b) m |= 1u << c;
+            return m;
+        }
+        return (1u << kNumCands) - 1;
     }
 
     // The width choose() would return without sampling: 0 while the key is
@@ -97,11 +118,13 @@ struct LaneTuner {
             e.count[p.cand]++;
             e.us_per_body[p.cand] += 1e3 * ms / (p.n > 0 ? p.n : 1);
             bool done = true;
-            for (int c = 0; c < kNumCands; ++c) done = done && e.count[c] >= kSamples;
+            for (int c = 0; c < kNumCands; ++c) done = done && (!((e.mask >> c) & 1u) || e.count[c] >= kSamples);
             if (done) {
-                int bi = 0;
-                for (int c = 1; c < kNumCands; ++c)
-                    if (e.us_per_body[c] / e.count[c] < e.us_per_body[bi] / e.count[bi]) bi = c;
+                int bi = -1;
+                for (int c = 0; c < kNumCands; ++c)
+                    if (((e.mask >> c) & 1u) &&
+                        (bi < 0 || e.us_per_body[c] / e.count[c] < e.us_per_body[bi] / e.count[bi]))
+                        bi = c;
                 e.best = kCands[bi];
             }
         }
