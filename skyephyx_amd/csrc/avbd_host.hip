@@ -618,6 +618,9 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     const int M = in->n_manifolds, T = in->n_tets, TR = in->n_tris;
     if (S < 1) return fail(AVBD_E_ARG, "world needs at least one scene");
     if (N < 0 || J < 0 || SP < 0 || M < 0 || T < 0 || TR < 0) return fail(AVBD_E_ARG, "negative count");
+    if (!in->scenes || (N > 0 && !in->bodies) || (J > 0 && !in->joints) || (SP > 0 && !in->springs) ||
+        (M > 0 && !in->manifolds) || (T > 0 && !in->tets) || (TR > 0 && !in->tris))
+        return fail(AVBD_E_ARG, "null record array");
     // bodies of a scene are contiguous, scenes in order
     std::vector<int> off(S + 1, 0);
     for (int i = 0; i < N; ++i) {
@@ -646,12 +649,20 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
         if (in->bodies[m.body_a].scene != in->bodies[m.body_b].scene) return fail(AVBD_E_ARG, "manifold links two scenes");
         if (m.n_contacts < 0 || m.n_contacts > 8) return fail(AVBD_E_ARG, "manifold contact count");
     }
+    // an element's vertices lie in one scene: colouring, removal and the
+    // energy lists index a scene's own body range
     for (int i = 0; i < T; ++i)
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 4; ++k) {
             if (!body_ok(in->tets[i].v[k])) return fail(AVBD_E_ARG, "tet vertex index");
+            if (in->bodies[in->tets[i].v[k]].scene != in->bodies[in->tets[i].v[0]].scene)
+                return fail(AVBD_E_ARG, "tet links two scenes");
+        }
     for (int i = 0; i < TR; ++i)
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 3; ++k) {
             if (!body_ok(in->tris[i].v[k])) return fail(AVBD_E_ARG, "triangle vertex index");
+            if (in->bodies[in->tris[i].v[k]].scene != in->bodies[in->tris[i].v[0]].scene)
+                return fail(AVBD_E_ARG, "triangle links two scenes");
+        }
 
     c->release();
     WorldDims dims;
@@ -1578,11 +1589,37 @@ int avbd_upload(avbd_ctx* c, const avbd_world* w, int64_t manifold_capacity) {
     // records are expanded on the host and every body index shifted by k per
     // earlier scene.
     if (w->n_scenes < 1 || w->n_bodies < 0 || (w->n_bodies > 0 && !w->bodies)) return fail(AVBD_E_ARG, "bad world");
+    if (w->n_joints < 0 || w->n_springs < 0 || w->n_manifolds < 0 || w->n_tets < 0 || w->n_tris < 0)
+        return fail(AVBD_E_ARG, "negative count");
+    if ((w->n_joints > 0 && !w->joints) || (w->n_springs > 0 && !w->springs) || (w->n_manifolds > 0 && !w->manifolds) ||
+        (w->n_tets > 0 && !w->tets) || (w->n_tris > 0 && !w->tris))
+        return fail(AVBD_E_ARG, "null record array");
     const int S = w->n_scenes, N = w->n_bodies;
+    if ((long long)N + (long long)k * S >= INT32_MAX) return fail(AVBD_E_ARG, "too many bodies with spare slots");
     for (int i = 0; i < N; ++i)
         if (w->bodies[i].scene < 0 || w->bodies[i].scene >= S || (i > 0 && w->bodies[i].scene < w->bodies[i - 1].scene))
             return fail(AVBD_E_ARG, "bodies must be grouped by scene in ascending order");
-    auto shift = [&](int i) { return i < 0 || i >= N ? i : i + k * w->bodies[i].scene; };
+    // Every index is checked against the caller's N before the expansion: an
+    // index in [N, N + k S) would otherwise name a spare or shifted body and
+    // pass the range checks of the expanded world.  -1 (the world anchor of
+    // a joint or spring) is the only negative index kept.
+    auto in_range = [&](int b, bool anchor_ok) { return (b >= 0 && b < N) || (anchor_ok && b == -1); };
+    for (int q = 0; q < w->n_joints; ++q)
+        if (!in_range(w->joints[q].body_a, true) || !in_range(w->joints[q].body_b, false))
+            return fail(AVBD_E_ARG, "joint body index");
+    for (int q = 0; q < w->n_springs; ++q)
+        if (!in_range(w->springs[q].body_a, true) || !in_range(w->springs[q].body_b, false))
+            return fail(AVBD_E_ARG, "spring body index");
+    for (int q = 0; q < w->n_manifolds; ++q)
+        if (!in_range(w->manifolds[q].body_a, false) || !in_range(w->manifolds[q].body_b, false))
+            return fail(AVBD_E_ARG, "manifold body index");
+    for (int q = 0; q < w->n_tets; ++q)
+        for (int v = 0; v < 4; ++v)
+            if (!in_range(w->tets[q].v[v], false)) return fail(AVBD_E_ARG, "tet vertex index");
+    for (int q = 0; q < w->n_tris; ++q)
+        for (int v = 0; v < 3; ++v)
+            if (!in_range(w->tris[q].v[v], false)) return fail(AVBD_E_ARG, "triangle vertex index");
+    auto shift = [&](int i) { return i < 0 ? i : i + k * w->bodies[i].scene; };
     std::vector<avbd_body> bodies;
     bodies.reserve((size_t)N + (size_t)k * S);
     std::vector<int> spare;
@@ -1632,7 +1669,13 @@ static int ensure_debug(avbd_ctx* c) {
     if (d.ready) return 0;
     DevWorld& w = c->w;
     const size_t slots = (size_t)w.J + w.SP + w.Mcap + w.T + w.TR;
-    if (slots >= (size_t)INT32_MAX) return fail(AVBD_E_CAPACITY, "world too large for debug output");
+    // the line and point offsets are 32-bit scans over per-slot counts: a
+    // joint or spring draws 1 line, a manifold up to 8 lines and 16 points,
+    // a tet 6 lines, a triangle 3
+    const size_t most_lines = (size_t)w.J + w.SP + 8 * (size_t)w.Mcap + 6 * (size_t)w.T + 3 * (size_t)w.TR;
+    const size_t most_points = 16 * (size_t)w.Mcap;
+    if (slots >= (size_t)INT32_MAX || most_lines >= (size_t)INT32_MAX || most_points >= (size_t)INT32_MAX)
+        return fail(AVBD_E_CAPACITY, "world too large for debug output");
     int rc;
     if ((rc = c->alloc(&d.lc, slots + 1)) || (rc = c->alloc(&d.loff, slots + 1)) ||
         (rc = c->alloc(&d.pc, (size_t)w.Mcap + 1)) || (rc = c->alloc(&d.poff, (size_t)w.Mcap + 1)))

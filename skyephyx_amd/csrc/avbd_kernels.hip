@@ -1959,8 +1959,6 @@ __global__ void k_kill_bodies(DevWorld w, const int* list, int n) {
     }
 }
 
-// Element vertex ids renumbered (-1 for a removed body, kept by later
-// removals; such an element is removed with its body and no longer read).
 // The element stream in incidence order from the element records: entry k
 // of e_ent (kind, local vertex, element index) takes its element's rest-shape
 // records and vertex ids.  Run once per upload over the n stream entries.
@@ -1982,6 +1980,8 @@ __global__ void k_element_stream(DevWorld w, int n) {
     }
 }
 
+// Element vertex ids renumbered (-1 for a removed body, kept by later
+// removals; such an element is removed with its body and no longer read).
 __global__ void k_remap_elements(DevWorld w, BodyRemap rm) {
     GRID_STRIDE(t, w.T + w.TR) {
         int4* v = t < w.T ? &w.t_v[t] : &w.r_v[t - w.T];

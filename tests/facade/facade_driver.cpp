@@ -211,6 +211,23 @@ int main(int argc, char** argv) {
                         s.energyPtrs.size());
             return write_records(s, argv[3]);
         }
+        if (mode == "keep" && argc == 6) {
+            // keep <scene> <bad snapshot> <before out> <after out>: a populated
+            // solver tries LoadSnapshot on a file it must refuse; the records
+            // are exported before and after the attempt
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            if (write_records(s, argv[4])) return 5;
+            bool refused = false;
+            try {
+                s.LoadSnapshot(argv[3]);
+            } catch (const std::exception& e) {
+                refused = true;
+                std::printf("refused: %s\n", e.what());
+            }
+            if (!refused) return 6;
+            return write_records(s, argv[5]);
+        }
         if (mode == "reloadstep" && argc == 7) {
             // reloadstep <snapshot> <steps> <lanes> <fem lanes> <out>
             Solver s;

@@ -137,6 +137,24 @@ def test_facade_load_snapshot_refuses_bad_worlds(driver, tmp_path):
     assert r.returncode == 4 and "out of range" in r.stdout, r.stdout
 
 
+@pytest.mark.parametrize("scene", ["joint_chain", "fem_block"])
+def test_facade_refused_load_keeps_the_world(driver, tmp_path, scene):
+    """A populated solver that refuses a snapshot (two scenes, or a record
+    naming a body the file does not hold) keeps its own records byte for
+    byte: LoadSnapshot checks every record before replacing anything."""
+    two = str(tmp_path / "two.avbdsnap")
+    capi.save_snapshot(World.concat([scenes.pyramid_2d(rows=3), scenes.joint_chain(3)]), two)
+    bad = scenes.joint_chain(3)
+    bad.joints["body_b"][1] = len(bad.bodies)
+    oob = str(tmp_path / "oob.avbdsnap")
+    capi.save_snapshot(bad, oob)
+    for snap, why in ((two, "one scene"), (oob, "out of range")):
+        before, after = str(tmp_path / "before.bin"), str(tmp_path / "after.bin")
+        r = subprocess.run([driver, "keep", scene, snap, before, after], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0 and why in r.stdout, r.stdout + r.stderr
+        assert open(before, "rb").read() == open(after, "rb").read()
+
+
 @pytest.mark.parametrize("name", SCENES)
 def test_records_match_the_python_builder(driver, tmp_path, name):
     out = str(tmp_path / "w.bin")

@@ -114,3 +114,52 @@ def test_element_records_in_any_order():
     assert len(world.tets) > 0 and len(world.tris) > 0
     np.testing.assert_array_equal(out[0].bodies["pos"], out[1].bodies["pos"])
     np.testing.assert_array_equal(out[0].bodies["rot"], out[1].bodies["rot"])
+
+
+def _two_fem_scenes_with_a_crossing_tet():
+    w = World.concat([scenes.fem_block(n=2, cell=0.5), scenes.fem_block(n=2, cell=0.5, origin=(3.0, 0.0, 0.0))])
+    first = w.bodies["scene"] == 0
+    other = int(np.nonzero(~first)[0][0])
+    t = int(np.nonzero(np.all(np.isin(w.tets["v"], np.nonzero(first)[0]), axis=1))[0][0])
+    w.tets["v"][t][3] = other  # one vertex in the second scene
+    return w
+
+
+def test_cross_scene_element_is_refused_before_any_launch():
+    """An element whose vertices lie in two scenes is refused by the upload
+    (AVBD_E_ARG) before anything is freed or launched: the world uploaded
+    before keeps stepping as a context that never saw the bad upload."""
+    from skyephyx_amd.capi import AvbdError
+
+    good = scenes.fem_block(n=3, cell=0.5)
+    with Context(lanes=1, fem_lanes=1) as a, Context(lanes=1, fem_lanes=1) as b:
+        a.upload(good)
+        b.upload(good)
+        a.step(2)
+        with pytest.raises(AvbdError, match="two scenes"):
+            a.upload(_two_fem_scenes_with_a_crossing_tet())
+        cloth = World.concat([scenes.cloth_sheet(n=3, spacing=0.5), scenes.cloth_sheet(n=3, spacing=0.5)])
+        cloth.tris["v"][0][2] = int(np.nonzero(cloth.bodies["scene"] == 1)[0][0])
+        with pytest.raises(AvbdError, match="two scenes"):
+            a.upload(cloth)
+        a.step(2)
+        b.step(4)
+        ga, gb = a.download(), b.download()
+        assert ga.bodies.tobytes() == gb.bodies.tobytes()
+
+
+def test_spare_slots_do_not_widen_the_index_range():
+    """With spare body slots, an index at or beyond the caller's body count
+    (which would name a spare or shifted body after the expansion) is
+    refused, as without spare slots."""
+    from skyephyx_amd.capi import AvbdError
+
+    w = World.concat([scenes.joint_chain(3), scenes.joint_chain(3)])
+    w.joints["body_b"][0] = len(w.bodies)  # one past the last body
+    with Context() as ctx:
+        ctx.set_option("spare_bodies", 4)
+        with pytest.raises(AvbdError, match="joint body index"):
+            ctx.upload(w)
+        w.joints["body_b"][0] = -2
+        with pytest.raises(AvbdError, match="joint body index"):
+            ctx.upload(w)
