@@ -338,6 +338,19 @@ typedef struct avbd_instance {
 } avbd_instance;
 /* Copies at most cap instances; *n = the live body count. */
 int avbd_get_instances(avbd_ctx* ctx, avbd_instance* out, int32_t cap, int32_t* n);
+
+/* avbd_get_instances without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (a render
+ * loop under a frame budget, a serving loop, code that forbids hipMalloc).
+ * Same arguments, results and errors as avbd_get_instances, but the
+ * instance, flag and offset staging lives in a file-scope device array of
+ * the library (one per GPU, AVBD_NOALLOC_MAX_BODIES body slots, about
+ * 17 MiB, held until the process ends; its address is looked up by
+ * avbd_create).  Worlds with more body slots (dead and spare slots count)
+ * get AVBD_E_CAPACITY.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU, from any context, race. */
+#define AVBD_NOALLOC_MAX_BODIES 131072
+int avbd_get_instances_noalloc(avbd_ctx* ctx, avbd_instance* out, int32_t cap, int32_t* n);
 /* rgb: 3 floats per live body (avbd_download's order) */
 int avbd_set_body_colours(avbd_ctx* ctx, const float* rgb, int32_t n);
 /* The last step's lines / points: copies at most cap, *n = the full count. */

@@ -20,6 +20,9 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "avbd_capi.h")
 _lib = None
 
 
+NOALLOC_MAX_BODIES = 131072  # AVBD_NOALLOC_MAX_BODIES
+
+
 class AvbdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"avbd error {code}: {msg}")
@@ -76,6 +79,7 @@ def lib():
         L.avbd_add_bodies.argtypes = [vp, vp, C.c_int32]
         L.avbd_set_body_colours.argtypes = [vp, vp, C.c_int32]
         L.avbd_get_instances.argtypes = [vp, vp, C.c_int32, C.POINTER(C.c_int32)]
+        L.avbd_get_instances_noalloc.argtypes = [vp, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.avbd_get_debug_lines.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
         L.avbd_get_debug_points.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
         L.avbd_snapshot_write.argtypes = [C.c_char_p, C.POINTER(AvbdWorld)]
@@ -218,13 +222,16 @@ class Context:
         a = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
         _check(self.L.avbd_set_body_colours(self.h, a.ctypes.data, len(a)))
 
-    def instances(self):
+    def instances(self, noalloc: bool = False):
         """Renderer instance data per live body ([n, 32]: model matrix 16
-        column-major, normal matrix 3 columns of 4, colour 4)."""
+        column-major, normal matrix 3 columns of 4, colour 4).  noalloc: the
+        entry point that allocates no device memory (at most
+        NOALLOC_MAX_BODIES body slots)."""
+        f = self.L.avbd_get_instances_noalloc if noalloc else self.L.avbd_get_instances
         n = C.c_int32()
-        _check(self.L.avbd_get_instances(self.h, None, 0, C.byref(n)))
+        _check(f(self.h, None, 0, C.byref(n)))
         out = np.zeros((n.value, 32), np.float32)
-        _check(self.L.avbd_get_instances(self.h, out.ctypes.data, n.value, C.byref(n)))
+        _check(f(self.h, out.ctypes.data, n.value, C.byref(n)))
         return out
 
     def debug_output(self):

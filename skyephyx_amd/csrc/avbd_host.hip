@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -60,6 +61,42 @@ int grid_for(long long n, int block = 256, int max_blocks = 4096) {
     return int(b);
 }
 
+}  // namespace
+
+// Device scratch of avbd_get_instances_noalloc: a file-scope array, one
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol): its device address is looked up once per device, with that device
+// current, by noalloc_scratch_init (called from avbd_create), and kept in
+// g_noalloc_ptr.  Layout for n body slots: n instances (128 bytes each), then
+// n live flags, n live offsets and one total (ints).
+constexpr int kNoallocMaxBodies = AVBD_NOALLOC_MAX_BODIES;
+constexpr size_t kNoallocBytes =
+    (size_t)kNoallocMaxBodies * (sizeof(avbd_instance) + 2 * sizeof(int)) + 256;
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// Called with `device` current.
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(AVBD_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return 0;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(AVBD_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[device] = p;
+    return 0;
+}
+
+void* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    return device >= 0 && device < kMaxDevices ? g_noalloc_ptr[device] : nullptr;
+}
 }  // namespace
 
 struct avbd_ctx {
@@ -1441,6 +1478,7 @@ int avbd_create(int32_t device, avbd_ctx** out) {
     if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
         return fail(AVBD_E_NOGPU, std::string("library built for gfx950, device is ") + prop.gcnArchName);
     HIP_TRY(hipSetDevice(device));
+    if (int rc = noalloc_scratch_init(device)) return rc;
     avbd_ctx* c = new avbd_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -2125,6 +2163,36 @@ int avbd_get_instances(avbd_ctx* c, avbd_instance* out, int32_t cap, int32_t* n)
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cub, cub_bytes, flag, off, w.N, st));
     hipLaunchKernelGGL(k_instances, dim3(grid_for(w.N)), dim3(256), 0, st, w, (const int*)flag, (const int*)off,
                        (const float4*)c->dbg.rgb, d_out);
+    HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(avbd_instance) * (size_t)std::min(cap, n_live), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int avbd_get_instances_noalloc(avbd_ctx* c, avbd_instance* out, int32_t cap, int32_t* n) {
+    if (!c || !n || (!out && cap > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    DevWorld& w = c->w;
+    if (w.N > kNoallocMaxBodies)
+        return fail(AVBD_E_CAPACITY, "avbd_get_instances_noalloc: the world has " + std::to_string(w.N) +
+                                         " body slots, at most " + std::to_string(kNoallocMaxBodies) +
+                                         " fit its scratch (use avbd_get_instances)");
+    HIP_TRY(hipSetDevice(c->device));
+    unsigned char* base = static_cast<unsigned char*>(noalloc_scratch(c->device));
+    if (!base) return fail(AVBD_E_STATE, "scratch of this device not set up");
+    const int n_live = w.N - (int)std::count(c->body_dead.begin(), c->body_dead.end(), 1);
+    *n = n_live;
+    if (!out || cap <= 0 || n_live == 0) return 0;
+    hipStream_t st = c->stream;
+    // every word read below is written first by this call: nothing to clear
+    float4* d_out = reinterpret_cast<float4*>(base);
+    int* flag = reinterpret_cast<int*>(base + sizeof(avbd_instance) * (size_t)kNoallocMaxBodies);
+    int* off = flag + kNoallocMaxBodies;
+    int* total = off + kNoallocMaxBodies;
+    hipLaunchKernelGGL(k_live_flag, dim3(grid_for(w.N)), dim3(256), 0, st, w, flag);
+    hipLaunchKernelGGL(k_scan_flags_1wg, dim3(1), dim3(1024), 0, st, (const int*)flag, off, w.N, total);
+    hipLaunchKernelGGL(k_instances, dim3(grid_for(w.N)), dim3(256), 0, st, w, (const int*)flag, (const int*)off,
+                       (const float4*)c->dbg.rgb, d_out);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(avbd_instance) * (size_t)std::min(cap, n_live), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return 0;

@@ -43,5 +43,6 @@ __global__ void k_element_stream(DevWorld w, int n);
 __global__ void k_remap_elements(DevWorld w, BodyRemap rm);
 __global__ void k_dbg_count(DevWorld w, int* lc, int* pc);
 __global__ void k_live_flag(DevWorld w, int* flag);
+__global__ void k_scan_flags_1wg(const int* flag, int* off, int n, int* total);
 __global__ void k_instances(DevWorld w, const int* flag, const int* live_off, const float4* rgb, float4* out);
 __global__ void k_dbg_write(DevWorld w, DebugOut d, const int* lc, const int* pc);

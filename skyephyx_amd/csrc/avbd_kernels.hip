@@ -2166,6 +2166,44 @@ __global__ void k_live_flag(DevWorld w, int* flag) {
     GRID_STRIDE(b, w.N) flag[b] = w.scale[b].w > -1e29f ? 1 : 0;
 }
 
+// Exclusive scan of n flags by one workgroup of 1024 threads (no temporary
+// storage beyond LDS): 1024 flags per pass, a wave-level scan with
+// __shfl_up, the 16 wave totals scanned in LDS, a running carry between
+// passes.  *total = the sum.  Used where no device allocation may happen.
+__global__ __launch_bounds__(1024) void k_scan_flags_1wg(const int* flag, int* off, int n, int* total) {
+    __shared__ int wave_sum[16];
+    __shared__ int carry_s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + (int)threadIdx.x;
+        const int v = i < n ? flag[i] : 0;
+        int x = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wave_sum[wave] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int k = 0; k < 16; ++k) {
+                const int t = wave_sum[k];
+                wave_sum[k] = acc;
+                acc += t;
+            }
+        }
+        __syncthreads();
+        const int carry = carry_s;
+        if (i < n) off[i] = carry + wave_sum[wave] + x - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry_s = carry + wave_sum[15] + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry_s;
+}
+
 __global__ void k_instances(DevWorld w, const int* flag, const int* live_off, const float4* rgb, float4* out) {
     GRID_STRIDE(b, w.N) {
         if (!flag[b]) continue;
