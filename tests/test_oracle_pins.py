@@ -516,3 +516,326 @@ def test_oracle_remove_bodies_is_swap_with_last():
                           if keep(int(m["body_a"]), int(m["body_b"])))
     pairs_after = sorted((int(m["body_a"]), int(m["body_b"])) for m in after.manifolds)
     assert pairs_before == pairs_after
+
+
+# ----------------------------------------------------------- friction (manifold.cpp:220-234)
+G = 9.81
+MU = 0.5  # sqrt(0.5 * 0.5): both bodies at friction 0.5
+
+
+def _box_on_incline(theta, gap=0.002):
+    """A unit box resting on a static slab tilted by theta about z (rotated
+    with it); returns the world and the downhill unit vector."""
+    q = quat_wxyz(Rotation.from_euler("z", theta))
+    R = Rotation.from_euler("z", theta).as_matrix()
+    normal = R @ np.array([0.0, 1.0, 0.0])
+    b = WorldBuilder()
+    b.add_body(MODEL_CUBE, 1.0, 0.5, (0.0, 0.0, 0.0), (40.0, 1.0, 10.0), rotation=q, is_static=True)
+    b.add_body(MODEL_CUBE, 1.0, 0.5, tuple(normal * (1.0 + gap)), (1, 1, 1), rotation=q)
+    return b.build(), -(R @ np.array([1.0, 0.0, 0.0]))
+
+
+def _track(o, steps, body, axis):
+    """(displacement along axis, velocity along axis) per step."""
+    x0 = o.save().bodies["pos"][body].astype(np.float64)
+    xs, vs = [], []
+    for _ in range(steps):
+        o.step(1)
+        s = o.save()
+        xs.append(float((s.bodies["pos"][body] - x0) @ axis))
+        vs.append(float(s.bodies["vel"][body] @ axis))
+    return np.array(xs), np.array(vs)
+
+
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
+def test_box_on_incline_below_friction_angle_stays(fp64):
+    """Coulomb: below the friction angle (tan 20 deg = 0.36 < mu = 0.5) the
+    tangential force needed, m g sin(theta), fits inside mu |lambda_n| and
+    the box sticks."""
+    w, down = _box_on_incline(math.radians(20.0))
+    xs, _ = _track(O.Oracle(w, fp64=fp64), 120, 1, down)
+    assert abs(xs[-1]) < 2e-3, xs[-1]
+
+
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
+def test_box_on_incline_above_friction_angle_slides(fp64):
+    """Above it (tan 35 deg = 0.70 > 0.5) the box slides with
+    a = g (sin(theta) - mu cos(theta)) = 1.609 m/s^2: friction bounded by
+    mu |lambda_n| with mu = sqrt(mu_A mu_B).  (Bounds taken from the
+    tangential lambda, or mu = mu_A mu_B, give 3.6 m/s^2 or other values.)"""
+    th = math.radians(35.0)
+    w, down = _box_on_incline(th)
+    dt = 1.0 / 60.0
+    _, vs = _track(O.Oracle(w, fp64=fp64), 90, 1, down)
+    t = np.arange(1, len(vs) + 1) * dt
+    a = np.polyfit(t[20:], vs[20:], 1)[0]
+    np.testing.assert_allclose(a, G * (math.sin(th) - MU * math.cos(th)), rtol=0.03)
+
+
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
+def test_box_launched_along_the_ground_saturates_the_friction_bound(fp64):
+    """A box sliding along the ground: the tangent rows along the motion sit
+    on their bounds +-mu |lambda_n| (each row bounded on its own,
+    manifold.cpp:220-226, mu = sqrt(mu_A mu_B)), so over the four contacts
+    they sum to mu m g, the rows across the motion carry nothing, and the
+    box slows down and stops.  (Per contact the bound comes from the
+    previous iteration's lambda_n, so only the sum is fixed exactly.  The
+    deceleration itself comes out below mu g: the primal keeps k J J^T of
+    a clamped friction row in its block, so ten iterations stop short of
+    the full Coulomb impulse; that is the reference's algorithm, not a
+    closed form.)"""
+    v0 = 3.0
+    w = single_box((0.0, -9.0 - 0.001, 0.0), vel=(v0, 0.0, 0.0))
+    o = O.Oracle(w, fp64=fp64)
+    v_prev = v0
+    for k in range(60):
+        o.step(1)
+        s = o.save()
+        v = float(s.bodies["vel"][1][0])
+        if v_prev > 0.01:
+            assert v <= v_prev + 1e-6  # slowing down while it slides
+        m = s.manifolds[0]
+        n = int(m["n_contacts"])
+        lam = m["contacts"]["lambda"][:n].astype(np.float64)
+        if 2 <= k and v > 0.3:  # sliding along x: one tangent row saturates, the other is idle
+            # (within 0.5%: the bounds lag the normal rows by one iteration
+            # while the landing's normal-force transient of the first steps decays)
+            np.testing.assert_allclose(lam[:, 0].sum(), -G, rtol=5e-3)
+            along = int(np.argmax([np.abs(lam[:, 1]).sum(), np.abs(lam[:, 2]).sum()])) + 1
+            across = 3 - along
+            np.testing.assert_allclose(np.abs(lam[:, along]).sum(), MU * np.abs(lam[:, 0].sum()), rtol=5e-3)
+            assert np.abs(lam[:, across]).max() < 1e-6
+            assert np.all(np.sign(lam[:, along]) == np.sign(lam[0, along]))
+        v_prev = v
+    assert abs(v_prev) < 1e-3
+
+
+# ----------------------------------------------------------- FACE_B contacts (collision.cpp:416-433)
+def test_face_b_contacts_swap_lever_arms():
+    """Box first, ground second: the 1e-6 padding of |C| (collision.cpp:291-294)
+    inflates the A-face penetrations by the big ground's half extents, so the
+    ground's face (B) is the reference face.  The clipped points are built
+    with B as reference and their lever arms swapped back: rA is a bottom
+    corner of the box (box frame), rB its projection on the ground's top
+    face (ground frame), 0.01 apart along the normal.  Without the swap rA
+    would hold ground-frame points (x near 3, y = +0.5)."""
+    cube = box((3.0, -9.0 - 0.01, 1.0))
+    ground = box((0, -10, 0), scale=(20, 1, 20))
+    n, c, ids = O.collide(cube, ground, fp64=True)
+    assert n == 4
+    assert all(int(i) & 3 == 1 for i in ids)  # FACE_B
+    assert all((int(i) >> 5) & 7 == 1 for i in ids)  # B's axis 1 (y)
+    np.testing.assert_allclose(c[:, 3:6], np.tile([0, -1, 0], (4, 1)), atol=1e-6)  # A -> B: down
+    ra, rb = c[:, 7:10].astype(np.float64), c[:, 10:13].astype(np.float64)
+    np.testing.assert_allclose(np.abs(ra), 0.5, atol=1e-6)  # corners of the unit box
+    np.testing.assert_allclose(ra[:, 1], -0.5, atol=1e-6)  # its bottom face
+    np.testing.assert_allclose(rb[:, 1], 0.5, atol=1e-6)  # the ground's top face
+    # world points: the corner 0.01 below the ground top, its projection on it
+    wa = np.array([3.0, -9.01, 1.0]) + ra
+    wb = np.array([0.0, -10.0, 0.0]) + rb
+    np.testing.assert_allclose(wb - wa, np.tile([0, 0.01, 0], (4, 1)), atol=1e-5)
+    np.testing.assert_allclose(c[:, 0:3], wb, atol=1e-5)
+
+
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
+def test_resting_box_on_ground_as_body_b(fp64):
+    """The resting-box balance through the FACE_B path: sum(lambda_n) = -m g,
+    every lambda_n <= 0, and the box does not rotate."""
+    b = WorldBuilder(alpha=0.99)
+    b.add_body(MODEL_CUBE, 1.0, 0.5, (0.3, -8.99, 0.2), (1, 1, 1))
+    scenes.add_ground(b)
+    o = O.Oracle(b.build(), fp64=True)
+    o.step(200)
+    s = o.save()
+    m = s.manifolds[0]
+    assert (int(m["body_a"]), int(m["body_b"])) == (0, 1)
+    assert all(int(i) & 3 == 1 for i in m["contacts"]["id"][: m["n_contacts"]])
+    lam = m["contacts"]["lambda"][: m["n_contacts"], 0]
+    assert np.all(lam <= 0)
+    np.testing.assert_allclose(lam.sum(), -1.0 * G, rtol=1e-4)
+    np.testing.assert_allclose(s.bodies["rot"][0], [1, 0, 0, 0], atol=1e-6)
+    np.testing.assert_allclose(s.bodies["vel"][0], 0, atol=1e-6)
+
+
+# ----------------------------------------------------------- edge-edge contacts (collision.cpp:186-267)
+def _box_edges(b, axis):
+    """The 4 edges of box b parallel to its local axis: (start, end) in world."""
+    p, q, s = b[:3].astype(np.float64), b[3:7].astype(np.float64), b[7:].astype(np.float64)
+    R = Rotation.from_quat([q[1], q[2], q[3], q[0]]).as_matrix()
+    u, v = (axis + 1) % 3, (axis + 2) % 3
+    out = []
+    for su in (-1, 1):
+        for sv in (-1, 1):
+            mid = p + R[:, u] * su * s[u] / 2 + R[:, v] * sv * s[v] / 2
+            out.append((mid - R[:, axis] * s[axis] / 2, mid + R[:, axis] * s[axis] / 2))
+    return out, p, R
+
+
+def _closest_points_brute(a0, a1, b0, b1, n=2001):
+    """Closest points of two segments by a dense FP64 grid refined twice."""
+    lo_s, hi_s, lo_t, hi_t = 0.0, 1.0, 0.0, 1.0
+    for _ in range(3):
+        s = np.linspace(lo_s, hi_s, n)[:, None]
+        t = np.linspace(lo_t, hi_t, n)[None, :]
+        pa = a0[None, None, :] + (a1 - a0)[None, None, :] * s[..., None]
+        pb = b0[None, None, :] + (b1 - b0)[None, None, :] * t[..., None]
+        d = np.linalg.norm(pa - pb, axis=-1)
+        i, j = np.unravel_index(np.argmin(d), d.shape)
+        si, tj = float(s[i, 0]), float(t[0, j])
+        hs, ht = (hi_s - lo_s) * 4 / n, (hi_t - lo_t) * 4 / n
+        lo_s, hi_s = max(0.0, si - hs), min(1.0, si + hs)
+        lo_t, hi_t = max(0.0, tj - ht), min(1.0, tj + ht)
+    return a0 + (a1 - a0) * si, b0 + (b1 - b0) * tj, float(d[i, j])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_edge_edge_contact_against_brute_force_closest_points(seed):
+    """Crossed edges, randomly perturbed: the contact's lever arms put its two
+    points on the closest points of the penetrating edge pair, found by brute
+    force over all 16 edge pairs parallel to the axes the contact id names;
+    the position is their midpoint and the penetration their distance."""
+    rng = np.random.default_rng(seed)
+    ra = Rotation.from_euler("zy", [45 + rng.uniform(-6, 6), rng.uniform(-6, 6)], degrees=True)
+    rb = Rotation.from_euler("xy", [45 + rng.uniform(-6, 6), rng.uniform(-6, 6)], degrees=True)
+    sa = rng.uniform(0.8, 1.4, 3)
+    sb = rng.uniform(0.8, 1.4, 3)
+    a = np.concatenate([[0, 0, 0], quat_wxyz(ra), sa]).astype(np.float32)
+    # lift B until the boxes just touch (FP64 SAT), then push it 5 mm in
+    lo, hi = 0.0, 5.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        b = np.concatenate([[rng.uniform(-0.02, 0.02) * 0, mid, 0], quat_wxyz(rb), sb]).astype(np.float32)
+        if sat_separated(a, b) > 0:
+            hi = mid
+        else:
+            lo = mid
+    b = np.concatenate([[0, hi - 0.005, 0], quat_wxyz(rb), sb]).astype(np.float32)
+    n, c, ids = O.collide(a, b, fp64=True)
+    if n != 1 or int(ids[0]) & 3 != 2:
+        pytest.skip("this perturbation gives a face contact")
+    axis_a, axis_b = (int(ids[0]) >> 2) & 7, (int(ids[0]) >> 5) & 7
+    ea, pa, RA = _box_edges(a, axis_a)
+    eb, pb, RB = _box_edges(b, axis_b)
+    best = None
+    for a0, a1 in ea:
+        for b0, b1 in eb:
+            qa, qb, d = _closest_points_brute(a0, a1, b0, b1, n=401)
+            if best is None or d < best[2]:
+                best = (qa, qb, d)
+    qa, qb, d = best
+    wa = pa + RA @ c[0, 7:10].astype(np.float64)
+    wb = pb + RB @ c[0, 10:13].astype(np.float64)
+    np.testing.assert_allclose(wa, qa, atol=2e-4)
+    np.testing.assert_allclose(wb, qb, atol=2e-4)
+    np.testing.assert_allclose(c[0, 0:3], 0.5 * (qa + qb), atol=2e-4)
+    np.testing.assert_allclose(c[0, 6], d, atol=2e-4)
+
+
+# ----------------------------------------------------------- joint angular rows (joint.cpp:13-21,112-163)
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
+def test_fixed_joint_cantilever_balances_gravity_torque(fp64):
+    """A 2 x 0.5 x 0.5 box held at its -x end face by a fixed joint to the
+    world: at rest it keeps its rest rotation; the position rows carry its
+    weight (lambda_y = m g) and the angular rows the moment of that force
+    about the centre of mass: torque_arm * lambda_ang = r x F with
+    r = (-1, 0, 0) m, so lambda_ang_z = -m g * 1 m / |s_B|^2 (torque_arm =
+    |s_A + s_B|^2 with no body A).  A sign slip in the angular Jacobian
+    (the reference fixed one, joint.cpp:139,161) turns the box away."""
+    b = WorldBuilder()
+    i = b.add_body(MODEL_CUBE, 1.0, 0.5, (0.0, 0.0, 0.0), (2.0, 0.5, 0.5))
+    b.add_joint(None, (-1.0, 0.0, 0.0), i, (-0.5, 0.0, 0.0))  # all six rows hard
+    o = O.Oracle(b.build(), fp64=fp64)
+    o.step(240)
+    s = o.save()
+    m = 1.0 * 2.0 * 0.5 * 0.5
+    arm = 2.0 ** 2 + 0.5 ** 2 + 0.5 ** 2
+    lam = s.joints["lambda"][0].astype(np.float64)
+    np.testing.assert_allclose(float(s.joints["torque_arm"][0]), arm, rtol=1e-6)
+    np.testing.assert_allclose(lam[:3], [0.0, m * G, 0.0], atol=1e-3 * m * G)
+    np.testing.assert_allclose(lam[3:], [0.0, 0.0, -m * G * 1.0 / arm], atol=2e-3 * m * G / arm)
+    q = s.bodies["rot"][0].astype(np.float64)
+    assert 2 * math.acos(min(1.0, abs(q[0]))) < 1e-3  # rest rotation kept
+    np.testing.assert_allclose(s.bodies["pos"][0], [0, 0, 0], atol=1e-3)
+
+
+# ----------------------------------------------------------- rotational prediction (solver.cpp:337-351,371-383)
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32", "fp64"])
+def test_torque_free_spin_follows_the_integrator_closed_form(fp64):
+    """A free box spinning about a principal axis (world axis R0 e_y of a
+    box of distinct edge lengths, R0 not the identity): each step rotates
+    the pose by q <- normalize(q + dt/2 (0, w) q), a left rotation by
+    theta_n = 2 atan(|w_n| dt / 2) about w, and the velocity update reads
+    back w_{n+1} = theta_n / dt along the same axis.  Twenty steps against
+    that recursion in FP64: pins the world-frame (left) product, the sign
+    and the first-order form.  A start above the clamp (50 rad/s) is cut to
+    it first."""
+    dt = 1.0 / 60.0
+    r0 = Rotation.from_euler("xyz", [0.4, -0.7, 1.1])
+    axis = r0.apply([0.0, 1.0, 0.0])
+    for w0, w_start in ((3.0, 3.0), (80.0, 50.0)):
+        # along the world y axis for the clamp case (the clamp is per component)
+        ax = axis if w0 < 50 else np.array([0.0, 1.0, 0.0])
+        rot = r0 if w0 < 50 else Rotation.identity()
+        b = WorldBuilder()
+        b.add_body(MODEL_CUBE, 1.0, 0.5, (0.0, 0.0, 0.0), (1.0, 2.0, 3.0), rotation=quat_wxyz(rot),
+                   angular_velocity=tuple(ax * w0))
+        world = b.build()
+        o = O.Oracle(world, fp64=fp64)
+        o.step(20)
+        s = o.save()
+        # from the stored (float) initial state
+        q0 = world.bodies["rot"][0].astype(np.float64)
+        rot = Rotation.from_quat([q0[1], q0[2], q0[3], q0[0]])
+        wv = np.clip(world.bodies["ang_vel"][0].astype(np.float64), -50.0, 50.0)
+        w = float(np.linalg.norm(wv))
+        ax = wv / w
+        assert abs(w - w_start) < 1e-5
+        total = 0.0
+        for _ in range(20):
+            th = 2.0 * math.atan(w * dt / 2.0)
+            total += th
+            w = th / dt
+        expect = Rotation.from_rotvec(ax * total) * rot
+        got = s.bodies["rot"][0].astype(np.float64)
+        ref = quat_wxyz(expect)
+        d = min(np.linalg.norm(got - ref), np.linalg.norm(got + ref))
+        # FP64 arithmetic, results stored as float records: 3e-7.  (An exact
+        # exponential map instead of the first-order form is 2e-4 away here.)
+        assert d < (2e-5 if not fp64 else 3e-7), (w0, d)
+        np.testing.assert_allclose(s.bodies["ang_vel"][0], ax * w, rtol=0, atol=(2e-3 if not fp64 else 1e-5))
+
+
+# ----------------------------------------------------------- fracture reset (solver.cpp:473-481)
+def test_fracture_resets_the_manifolds_of_both_linked_bodies():
+    """When a force breaks, every manifold on its linked bodies is reset
+    (lambda, penalty and C to 0, force.hpp:67-77) and the force disabled.
+    With one iteration the break happens in the step's only dual update and
+    the post-stabilisation that follows does not touch lambda or penalty:
+    after the step the manifolds of the two linked boxes read 0, a third
+    resting box keeps its contact forces."""
+    b = WorldBuilder()
+    scenes.add_ground(b)
+    a1 = b.add_body(MODEL_CUBE, 1.0, 0.5, (-2.0, -9.0 - 0.005, 0.0), (1, 1, 1))
+    a2 = b.add_body(MODEL_CUBE, 1.0, 0.5, (2.0, -9.0 - 0.005, 0.0), (1, 1, 1))
+    b.add_body(MODEL_CUBE, 1.0, 0.5, (6.0, -9.0 - 0.005, 0.0), (1, 1, 1))
+    b.add_spring(a1, (0, 0, 0), a2, (0, 0, 0), 100.0)  # rest length 4: slack
+    o = O.Oracle(b.build())
+    o.step(10)
+    w = o.save()
+    for m in w.manifolds:
+        assert np.any(m["contacts"]["lambda"][: m["n_contacts"]] != 0)
+    w.springs["rest_length"][0] = 2.0  # now stretched by 2 m
+    w.springs["fracture"][0] = 1e-3
+    w.scenes["iterations"][0] = 1
+    o = O.Oracle(w)
+    o.step(1)
+    s = o.save()
+    assert s.springs["flags"][0] & 1 and s.springs["lambda"][0] == 0 and s.springs["penalty"][0] == 0
+    by_body = {int(m["body_b"]): m for m in s.manifolds}  # ground (0) is body A of each
+    for k in (a1, a2):
+        m = by_body[k]
+        n = int(m["n_contacts"])
+        assert n > 0
+        assert np.all(m["contacts"]["lambda"][:n] == 0) and np.all(m["contacts"]["penalty"][:n] == 0), k
+    m = by_body[3]
+    assert np.any(m["contacts"]["lambda"][: m["n_contacts"]] != 0)
