@@ -134,11 +134,12 @@ def random_quaternion(rng):
 
 
 def box_lattice(shape=(40, 25, 20), spacing=3.5, jitter=0.25, edge=(0.5, 2.0), seed=20260101, ground_y=-10.0,
-                n_chain=0, n_springs=0, spring_k=1e4, bounds=200.0, **scene):
+                n_chain=0, n_springs=0, spring_k=1e4, bounds=200.0, chain_density=1000.0, **scene):
     """Jittered lattice of randomly sized and oriented boxes above a large ground.
 
-    Optional: a ball-joint chain of `n_chain` links hanging from a world anchor
-    (Joint stiffness (inf, inf, inf, 0, 0, 0)), and `n_springs` springs between
+    Optional: a ball-joint chain of `n_chain` links (0.1 m cubes of density
+    chain_density) lying on the ground beside the lattice, anchored to the
+    world at one end (Joint stiffness (inf, inf, inf, 0, 0, 0)), and `n_springs` springs between
     seeded random lattice-neighbour box pairs."""
     rng = np.random.default_rng(seed)
     params = dict(bounds_min=(-bounds,) * 3, bounds_max=(bounds,) * 3)
@@ -159,14 +160,23 @@ def box_lattice(shape=(40, 25, 20), spacing=3.5, jitter=0.25, edge=(0.5, 2.0), s
                 scale = tuple(rng.uniform(edge[0], edge[1], 3))
                 lattice[i, j, k] = b.add_body(MODEL_CUBE, 1.0, 0.5, pos, scale, rotation=random_quaternion(rng))
     if n_chain:
+        # The chain lies straight along x on the ground 10 m beside the
+        # lattice, its first link anchored to the world; links are 0.1 m
+        # cubes of density chain_density (1 kg at the default, comparable to
+        # the boxes that tumble into it), 2 mm above the ground top.  Hanging
+        # or released from a height, a chain of 1000 hard ball joints never
+        # converges in a step's iterations, stretches and within a few
+        # seconds spins its links up until the explosion check stops the
+        # scene; 1 g links hit by kilogram boxes do the same.  The
+        # reference's algorithm (the oracle) behaves alike in both cases.
         link = 0.1
-        top = base_y + ny * spacing + 5.0
-        start = np.array([-(n_chain * link) * 0.5, top, -(nz * spacing) * 0.5 - 5.0])
+        y = ground_y + 0.5 + link * 0.5 + 0.002
+        start = np.array([-(n_chain * link) * 0.5, y, -(nz * spacing) * 0.5 - 10.0])
         ball = (INF, INF, INF, F32(0), F32(0), F32(0))
         last = None
         for c in range(n_chain):
             p = start + np.array([(c + 0.5) * link, 0.0, 0.0])
-            body = b.add_body(MODEL_CUBE, 1.0, 0.5, tuple(p), (link, link, link))
+            body = b.add_body(MODEL_CUBE, chain_density, 0.5, tuple(p), (link, link, link))
             if last is None:
                 b.add_joint(None, tuple(start), body, (-0.5, 0.0, 0.0), ball)
             else:

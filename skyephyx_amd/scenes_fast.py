@@ -75,15 +75,9 @@ def tets(positions, idx, E, nu, serial0=0):
     t["serial"] = serial0 + np.arange(T)
     p = positions.astype(np.float64)
     p0 = p[idx[:, 0]]
-    # Dm = [p1 - p0 | p2 - p0 | p3 - p0] (columns); inverse by cofactors in
-    # double, rounded once to float (numpy's batched LU inverse costs ~0.5 us
-    # per matrix, which dominated building the 24.7M-tet benchmark world)
-    c0, c1, c2 = p[idx[:, 1]] - p0, p[idx[:, 2]] - p0, p[idx[:, 3]] - p0
-    r0, r1, r2 = np.cross(c1, c2), np.cross(c2, c0), np.cross(c0, c1)  # rows of adj(Dm)
-    det = np.einsum("ij,ij->i", c0, r0)
-    inv = np.stack([r0, r1, r2], axis=1) / det[:, None, None]
-    t["dm_inv"] = inv.astype(F32).reshape(T, 9)
-    t["rest_volume"] = (np.abs(det) / 6.0).astype(F32)
+    Dm = np.stack([p[idx[:, 1]] - p0, p[idx[:, 2]] - p0, p[idx[:, 3]] - p0], axis=2)
+    t["dm_inv"] = np.linalg.inv(Dm).astype(F32).reshape(T, 9)
+    t["rest_volume"] = (np.abs(np.linalg.det(Dm)) / 6.0).astype(F32)
     mu, lam = WorldBuilder.lame(E, nu)
     t["mu"] = mu
     t["lambda"] = lam

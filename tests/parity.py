@@ -63,21 +63,25 @@ def compare_manifolds(g, o):
 
 
 def compare_rows(g_rows, o_rows):
+    """(per entry max |dl| / (|l| + 1e-3 max|l|), norm-wise ||dl|| / ||l||)."""
     g = np.asarray(g_rows, np.float64)
     o = np.asarray(o_rows, np.float64)
     if g.size == 0:
-        return 0.0
+        return 0.0, 0.0
     scale = np.abs(o).max() if o.size else 1.0
-    return float((np.abs(g - o) / (np.abs(o) + 1e-3 * max(scale, 1e-12))).max())
+    per = float((np.abs(g - o) / (np.abs(o) + 1e-3 * max(scale, 1e-12))).max())
+    r = float(np.sqrt((o ** 2).sum()))
+    d = float(np.sqrt(((g - o) ** 2).sum()))
+    return per, (d / r if r > 0 else d)
 
 
 def summary(g, o):
     s = {"pos_rel": pos_rel_err(g.bodies, o.bodies), "rot": rot_err(g.bodies, o.bodies)}
     s.update(compare_manifolds(g, o))
     if len(o.joints):
-        s["joint_lambda_rel"] = compare_rows(g.joints["lambda"], o.joints["lambda"])
+        s["joint_lambda_rel"], s["joint_lambda_norm_rel"] = compare_rows(g.joints["lambda"], o.joints["lambda"])
     if len(o.springs):
-        s["spring_lambda_rel"] = compare_rows(g.springs["lambda"], o.springs["lambda"])
+        s["spring_lambda_rel"], s["spring_lambda_norm_rel"] = compare_rows(g.springs["lambda"], o.springs["lambda"])
     return s
 
 
@@ -116,3 +120,73 @@ def colouring_ok(world, colours):
         if any(colours[u] == colours[v] for u in nb[v]):
             return False
     return True
+
+
+def one_step_report(state, g, gcol, natural=True):
+    """The GPU's result g (and its colours gcol) of one step from `state`
+    against the FP32 oracle in colour-major order (what the GPU computes),
+    with the FP64 oracle's distance beside it and, as a report only, the
+    distance from the FP32 oracle in natural order (the reference's own
+    bodyPtrs visit order, solver.cpp:392-449)."""
+    import oracle_py as O
+
+    o = O.Oracle(state, fp64=False, order=O.COLOUR_MAJOR)
+    o.step(1)
+    ref = o.save()
+    ocol, _ = o.colours(len(ref.bodies))
+    del o
+    d = O.Oracle(state, fp64=True, order=O.COLOUR_MAJOR)
+    d.step(1)
+    ref64 = d.save()
+    del d
+    res = summary(g, ref)
+    dyn = (g.bodies["flags"] & 1) == 0
+    res["colours_equal"] = bool(len(gcol) == len(ocol) and np.array_equal(gcol[dyn], ocol[dyn]))
+    g64 = summary(g, ref64)
+    res["fp64_pos_rel_gpu"] = g64["pos_rel"]
+    res["fp64_rot_gpu"] = g64["rot"]
+    spread = summary(ref, ref64)  # the FP32 oracle against the FP64 oracle
+    res["fp64_pos_rel_oracle32"] = spread["pos_rel"]
+    res["fp64_rot_oracle32"] = spread["rot"]
+    # the FP32 oracle's own distance from FP64, per lambda figure
+    for k in ("lambda_rel", "lambda_norm_rel", "joint_lambda_rel", "joint_lambda_norm_rel", "spring_lambda_rel",
+              "spring_lambda_norm_rel"):
+        if k in spread:
+            res["spread_" + k] = spread[k]
+    res["oracle32_vs_64_lambda_rel"] = spread["lambda_rel"]
+    if natural:
+        n = O.Oracle(state, fp64=False, order=O.NATURAL)
+        n.step(1)
+        nat = n.save()
+        del n
+        if len(nat.bodies) == len(g.bodies):
+            sn = summary(g, nat)
+            res["natural_pos_rel"] = sn["pos_rel"]
+            res["natural_rot"] = sn["rot"]
+            res["natural_lambda_norm_rel"] = sn["lambda_norm_rel"]
+    return res
+
+
+def assert_parity(res, strict_lambda=False):
+    """The single-step bars.  Positions and orientations within POS_TOL;
+    pairs, contact counts, feature ids and colours bit-exact.  Each lambda
+    figure (contacts, joints, springs; per entry |dl| / (|l| + 1e-3 max|l|)
+    and norm-wise ||dl|| / ||l||) against the FP32 oracle within
+    max(LAMBDA_TOL, k x the FP32 oracle's own distance from FP64 in the same
+    figure): k = 1 with strict_lambda (the GPU accumulating in the oracle's
+    sequential order, so only the rounding of transcendentals differs), else
+    k = 2.  Where FP32 itself is well conditioned (spread below LAMBDA_TOL)
+    that is LAMBDA_TOL outright; where it is not (friction rows of settling
+    piles, the statically indeterminate contacts of a resting box, the
+    near-zero forces of a slack chain), the GPU may not move further from
+    the FP32 oracle than FP32 moves from FP64."""
+    assert res["pairs_equal"], res
+    assert res["n_equal"] and res["ids_equal"], res
+    assert res["colours_equal"], res
+    assert res["pos_rel"] <= POS_TOL, res
+    assert res["rot"] <= POS_TOL, res
+    k = 1.0 if strict_lambda else 2.0
+    for key in ("lambda_rel", "lambda_norm_rel", "joint_lambda_rel", "joint_lambda_norm_rel", "spring_lambda_rel",
+                "spring_lambda_norm_rel"):
+        if key in res:
+            assert res[key] <= max(LAMBDA_TOL, k * res.get("spread_" + key, 0.0)), (key, res)
