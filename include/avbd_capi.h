@@ -224,10 +224,10 @@ typedef struct avbd_world {
     avbd_tri* tris;
 } avbd_world;
 
-/* Per-phase wall time of the last step, milliseconds (SolverTimings).  When
- * the step ran phases 4-7 as a replayed step graph (option "step_graphs"),
- * their time is reported together in primal_dual_ms and prediction,
- * velocity and post-stabilisation read 0. */
+/* Per-phase wall time of the last step, milliseconds (SolverTimings), from
+ * HIP events on the library's stream; a replayed step graph (option
+ * "step_graphs") records the events of phases 4-7 itself, so every phase is
+ * timed either way. */
 typedef struct avbd_timings {
     float broad_phase_ms;
     float warmstart_ms;

@@ -225,13 +225,18 @@ struct avbd_ctx {
         kmarks.clear();
     }
 
-    // Step graphs: phases 4-7 of a step (prediction, iterations, velocities,
-    // post-stabilisation) captured once per launch shape and replayed.  A
-    // shape is captured the second time it occurs, so shapes that never
-    // repeat (a served tick's changing scene set) stay eager.
+    // Step graphs: the launches of phase 5 (the iterations) and phase 7
+    // (post-stabilisation) captured once per launch shape and replayed;
+    // prediction and velocities, one kernel each, launch between them with
+    // the phase events recorded from the host (events recorded inside a
+    // replayed graph do not give ordered timestamps).  A shape is captured
+    // the second time it occurs, so shapes that never repeat (a served
+    // tick's changing scene set) stay eager.
     struct StepGraph {
         std::vector<int> key;
-        hipGraphExec_t exec = nullptr;
+        hipGraphExec_t iters = nullptr;  // phase 5: the iterations
+        hipGraphExec_t post = nullptr;   // phase 7: post-stabilisation (if any scene has it)
+        bool captured = false;
         long long used = 0;
     };
     static constexpr size_t kMaxGraphs = 16;
@@ -239,9 +244,14 @@ struct avbd_ctx {
     std::vector<StepGraph> graphs;
     long long graph_clock = 0;
     long long graph_launches = 0, graph_captures = 0;
+    static void destroy(StepGraph& g) {
+        if (g.iters) hipGraphExecDestroy(g.iters);
+        if (g.post) hipGraphExecDestroy(g.post);
+        g.iters = g.post = nullptr;
+        g.captured = false;
+    }
     void drop_graphs() {
-        for (StepGraph& g : graphs)
-            if (g.exec) hipGraphExecDestroy(g.exec);
+        for (StepGraph& g : graphs) destroy(g);
         graphs.clear();
     }
 
@@ -1208,11 +1218,65 @@ int remove_out_of_bounds(avbd_ctx* c) {
     return poststab_scenes(c, aff);
 }
 
-// Phases 4-7 of a step for the bucket offsets off: prediction, the
-// iterations (per colour the primal buckets, then the dual and the fracture
-// reset), velocities and post-stabilisation.  marks: record the phase events
-// (not under stream capture).
-int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_post, int dgrid, bool marks) {
+// Phase 5 of a step for the bucket offsets off: the iterations, per colour
+// the primal buckets, then the dual and the fracture reset.
+int launch_iterations(avbd_ctx* c, const int* off, int ncol, int max_iters, int dgrid) {
+    DevWorld& w = c->w;
+    hipStream_t st = c->stream;
+    for (int it = 0; it < max_iters; ++it) {
+        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, it, 0);
+        const int mark = c->mark_begin(avbd_ctx::K_DUAL);
+        hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
+        c->mark_end(mark);
+        if (w.J + w.SP > 0) {
+            hipLaunchKernelGGL(k_fracture_reset, dim3(grid_for(2 * (w.J + w.SP) + 2)), dim3(256), 0, st, w);
+            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_FRACTURES, 0);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Phase 7: the post-stabilisation sweep and the stick flags it leaves.
+int launch_poststab(avbd_ctx* c, const int* off, int ncol) {
+    DevWorld& w = c->w;
+    for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, -1, 1);
+    hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, c->stream, w);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Captures the launches `body` makes on the context's stream into an
+// executable graph.
+template <class F>
+int capture(avbd_ctx* c, hipGraphExec_t* exec, F body) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(AVBD_E_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        *exec = nullptr;
+        return fail(AVBD_E_HIP, std::string("step graph instantiate: ") + hipGetErrorString(ei));
+    }
+    return 0;
+}
+
+// Phases 4-7 of a step: prediction, the iterations, velocities and
+// post-stabilisation, with the phase events ev[3] (after the prediction),
+// ev[4] (after the iterations), ev[5] (after the velocities) recorded on
+// the host side of the launches.  sg: the step graph of this launch shape,
+// whose captured iterations and post-stabilisation replace the launches of
+// phases 5 and 7 (prediction and velocities are one kernel each); null =
+// everything eagerly.
+int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_post, int dgrid,
+                 avbd_ctx::StepGraph* sg) {
     DevWorld& w = c->w;
     hipStream_t st = c->stream;
     const int N = w.N;
@@ -1224,28 +1288,28 @@ int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_
         launch_scene_step(c, c->side2, 0);
     }
     hipLaunchKernelGGL(k_predict, dim3(grid_for(N)), dim3(256), 0, st, w);
-    if (marks) HIP_TRY(hipEventRecord(c->ev[3], st));
-    for (int it = 0; it < max_iters; ++it) {
-        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, it, 0);
-        const int mark = c->mark_begin(avbd_ctx::K_DUAL);
-        hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
-        c->mark_end(mark);
-        if (w.J + w.SP > 0) {
-            hipLaunchKernelGGL(k_fracture_reset, dim3(grid_for(2 * (w.J + w.SP) + 2)), dim3(256), 0, st, w);
-            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_FRACTURES, 0);
-        }
+    HIP_TRY(hipEventRecord(c->ev[3], st));
+    int rc;
+    if (sg) {
+        HIP_TRY(hipGraphLaunch(sg->iters, st));
+    } else if ((rc = launch_iterations(c, off, ncol, max_iters, dgrid))) {
+        return rc;
     }
-    if (marks) HIP_TRY(hipEventRecord(c->ev[4], st));
+    HIP_TRY(hipEventRecord(c->ev[4], st));
     hipLaunchKernelGGL(k_velocity, dim3(grid_for(N)), dim3(256), 0, st, w);
-    if (marks) HIP_TRY(hipEventRecord(c->ev[5], st));
+    HIP_TRY(hipEventRecord(c->ev[5], st));
     if (any_post) {
-        for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, -1, 1);
-        hipLaunchKernelGGL(k_stick, dim3(grid_for((long long)w.Mcap * 8)), dim3(256), 0, st, w);
+        if (sg) {
+            HIP_TRY(hipGraphLaunch(sg->post, st));
+        } else if ((rc = launch_poststab(c, off, ncol))) {
+            return rc;
+        }
     }
     if (fused) {
         HIP_TRY(hipEventRecord(c->join2, c->side2));
         HIP_TRY(hipStreamWaitEvent(st, c->join2, 0));
     }
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 
@@ -1332,57 +1396,45 @@ int do_step(avbd_ctx* c) {
         while (p < dgrid) p <<= 1;
         dgrid = std::min(p, 8192);
     }
-    // ---- 4.-7. as one graph when this launch shape has been seen before
+    // ---- 4.-7. with the iterations and post-stabilisation as step graphs
+    // when this launch shape has been seen before
     std::vector<int> key;
+    avbd_ctx::StepGraph* use = nullptr;
     if (c->use_graphs && !c->ktiming && step_graph_key(c, off, ncol, max_iters, any_post, dgrid, &key)) {
         avbd_ctx::StepGraph* hit = nullptr;
         for (avbd_ctx::StepGraph& sg : c->graphs)
             if (sg.key == key) hit = &sg;
-        if (hit && !hit->exec) {
-            hipGraph_t graph = nullptr;
-            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            rc = sweep_phases(c, off, ncol, max_iters, any_post, dgrid, false);
-            hipError_t e = hipStreamEndCapture(st, &graph);
+        if (hit && !hit->captured) {
+            rc = capture(c, &hit->iters, [&] { return launch_iterations(c, off, ncol, max_iters, dgrid); });
+            if (!rc && any_post) rc = capture(c, &hit->post, [&] { return launch_poststab(c, off, ncol); });
             if (rc) {
-                if (graph) hipGraphDestroy(graph);
+                avbd_ctx::destroy(*hit);
                 return rc;
             }
-            if (e != hipSuccess) return fail(AVBD_E_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
-            e = hipGraphInstantiate(&hit->exec, graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
-            if (e != hipSuccess) {
-                hit->exec = nullptr;
-                return fail(AVBD_E_HIP, std::string("step graph instantiate: ") + hipGetErrorString(e));
-            }
+            hit->captured = true;
             ++c->graph_captures;
         }
         if (hit) {
             hit->used = ++c->graph_clock;
-            // phases 4-7 run inside the graph: reported together as primal_dual
-            HIP_TRY(hipEventRecord(c->ev[3], st));
-            HIP_TRY(hipGraphLaunch(hit->exec, st));
+            use = hit;
             ++c->graph_launches;
-            HIP_TRY(hipEventRecord(c->ev[4], st));
-            HIP_TRY(hipEventRecord(c->ev[5], st));
-            HIP_TRY(hipEventRecord(c->ev[6], st));
-            HIP_TRY(hipGetLastError());
-            return 0;
+        } else {
+            // first sighting: remember the shape, run eagerly
+            if (c->graphs.size() >= avbd_ctx::kMaxGraphs) {
+                auto lru = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                            [](const avbd_ctx::StepGraph& a, const avbd_ctx::StepGraph& b) {
+                                                return a.used < b.used;
+                                            });
+                avbd_ctx::destroy(*lru);
+                c->graphs.erase(lru);
+            }
+            avbd_ctx::StepGraph sg;
+            sg.key = key;
+            sg.used = ++c->graph_clock;
+            c->graphs.push_back(std::move(sg));
         }
-        // first sighting: remember the shape, run eagerly
-        if (c->graphs.size() >= avbd_ctx::kMaxGraphs) {
-            auto lru = std::min_element(c->graphs.begin(), c->graphs.end(),
-                                        [](const avbd_ctx::StepGraph& a, const avbd_ctx::StepGraph& b) {
-                                            return a.used < b.used;
-                                        });
-            if (lru->exec) hipGraphExecDestroy(lru->exec);
-            c->graphs.erase(lru);
-        }
-        avbd_ctx::StepGraph sg;
-        sg.key = key;
-        sg.used = ++c->graph_clock;
-        c->graphs.push_back(std::move(sg));
     }
-    if ((rc = sweep_phases(c, off, ncol, max_iters, any_post, dgrid, true))) return rc;
+    if ((rc = sweep_phases(c, off, ncol, max_iters, any_post, dgrid, use))) return rc;
     HIP_TRY(hipEventRecord(c->ev[6], st));
     HIP_TRY(hipGetLastError());
     return 0;

@@ -714,7 +714,9 @@ void Solver::Step() {
     ReadBack();
     if (syncForces) SyncForces();
     else forcesStale_ = true;
-    AccumulateTimings();
+    // CheckExplosion returns from Step before the timing update
+    // (solver.cpp:528-529): the stopping step is not accumulated
+    if (!emergencyStop) AccumulateTimings();
 }
 
 // Bodies the step removed for leaving the bounds: the same RemoveBody calls

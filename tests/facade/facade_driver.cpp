@@ -5,6 +5,12 @@
 //   facade_driver step <scene> <steps> <lanes> <fem_lanes> <out>
 //                                           records after <steps> Step() calls
 //   facade_driver semantics                 host-side API checks (no GPU)
+//   facade_driver keep <scene> <snap> <before> <after>
+//                                           records before and after a refused
+//                                           LoadSnapshot (no GPU)
+//   facade_driver timings <scene> <steps> <explode_at>
+//                                           per step the C ABI's timings and the
+//                                           facade's SolverTimings
 //
 // <out> holds int32 counts (scenes, bodies, joints, springs, manifolds,
 // tets, tris) followed by the record arrays of include/avbd_capi.h in that
@@ -210,6 +216,31 @@ int main(int argc, char** argv) {
             std::printf("bodies %zu forces %zu energies %zu\n", s.bodyPtrs.size(), s.forcePtrs.size(),
                         s.energyPtrs.size());
             return write_records(s, argv[3]);
+        }
+        if (mode == "timings" && argc == 5) {
+            // timings <scene> <steps> <explode_at>: per step the C ABI's
+            // figures of that step and the facade's rolling means; at step
+            // explode_at (if >= 0) the last body is given 700 m/s first
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            const int steps = std::atoi(argv[3]), boom = std::atoi(argv[4]);
+            for (int k = 0; k < steps; ++k) {
+                if (k == boom) s.bodyPtrs.back()->velocity = Vector3f(700.f, 0.f, 0.f);
+                s.Step();
+                avbd_timings t{};
+                if (avbd_get_timings(s.DeviceContext(), &t)) return 3;
+                int64_t replays = 0;
+                double captures = 0.0;
+                if (avbd_kernel_stats(s.DeviceContext(), "step_graph", &replays, &captures, 0)) return 3;
+                const SolverTimings& f = s.timings;
+                std::printf("abi %d %.6g %.6g %.6g %.6g %.6g %.6g %.6g replays %lld\n", k, t.broad_phase_ms,
+                            t.warmstart_ms, t.prediction_ms, t.primal_dual_ms, t.velocity_update_ms, t.post_stab_ms,
+                            t.total_ms, (long long)replays);
+                std::printf("facade %d %.6g %.6g %.6g %.6g %.6g %.6g %.6g avg %.6g emergency %d\n", k, f.broadPhaseMs,
+                            f.warmstartMs, f.predictionMs, f.primalDualMs, f.velocityUpdateMs, f.postStabMs,
+                            f.totalSubstepMs, s.averageStepTime, int(s.emergencyStop));
+            }
+            return 0;
         }
         if (mode == "keep" && argc == 6) {
             // keep <scene> <bad snapshot> <before out> <after out>: a populated

@@ -374,3 +374,49 @@ def test_spawn_into_spare_slots_without_removals(driver, tmp_path):
     built = read_records(str(tmp_path / "a.bin"))
     fac = read_records(out)
     assert len(fac.bodies) == len(built.bodies)
+
+
+def _timing_rows(out):
+    abi, fac = [], []
+    for line in out.splitlines():
+        p = line.split()
+        if p and p[0] == "abi":
+            abi.append(([float(x) for x in p[2:9]], int(p[10])))
+        elif p and p[0] == "facade":
+            fac.append(([float(x) for x in p[2:9]], float(p[10]), int(p[12])))
+    return abi, fac
+
+
+@pytest.mark.gpu
+def test_solver_timings_rolling_mean(driver):
+    """SolverTimings (solver.hpp:10-19, solver.cpp:599-626): every step's
+    seven figures come from events on the device stream, also when the step
+    replays a captured graph (phases 4-7 inside the graph); the phases add
+    up to the total; the facade's timings are the mean over the last 60
+    steps and averageStepTime is its total."""
+    steps = 75
+    r = subprocess.run([driver, "timings", "pyramid", str(steps), "-1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    abi, fac = _timing_rows(r.stdout)
+    assert len(abi) == len(fac) == steps
+    assert abi[-1][1] > 0  # later steps replay step graphs
+    for k, ((t, _), (f, avg, em)) in enumerate(zip(abi, fac)):
+        assert em == 0
+        assert all(x > 0 for x in t), (k, t)  # broad, warm, predict, primal/dual, velocity, post-stab, total
+        np.testing.assert_allclose(sum(t[:6]), t[6], rtol=1e-3)
+        window = [abi[j][0] for j in range(max(0, k - 59), k + 1)]
+        np.testing.assert_allclose(f, np.mean(window, axis=0), rtol=1e-4)
+        assert avg == f[6]
+
+
+@pytest.mark.gpu
+def test_solver_timings_skip_the_stopping_step(driver):
+    """The step that stops on an explosion returns before the timing update
+    (solver.cpp:528-529), and later steps are no-ops: the facade's figures
+    stay those of the steps before."""
+    r = subprocess.run([driver, "timings", "pyramid", "12", "8"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    abi, fac = _timing_rows(r.stdout)
+    assert [e for _, _, e in fac] == [0] * 8 + [1] * 4
+    for k in range(8, 12):
+        assert fac[k][0] == fac[7][0] and fac[k][1] == fac[7][1]
