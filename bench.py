@@ -8,18 +8,23 @@ advanced with one step call over all scenes.  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mix|fem|boxes20k|pyramid]
 
-The default warm-up (150 steps) lets the box piles settle and the hanging
-blocks deform into contact, so the timed steps carry the steady contact load
-(about 3x the contacts of the first steps) rather than the sparse start.
+After the upload the world settles --settle steps (default 150) before the
+W warm-up steps, so the timed steps carry the steady contact load (the box
+piles roughly triple their contacts over the first ~100 steps) whatever W
+is.  The K timed steps run as the product does (step graphs, no per-launch
+events).
 
 value = whole-job scene-steps/s.  The line also carries the roofline of the
-dominant kernel (the primal block-solve sweep, timed per launch with HIP
-events on the library's stream), the served form over the same scenes
-(request server with every scene resident under a 200 GB HBM budget,
-Poisson arrivals at --serve-load fractions of its own closed-loop saturation: request
-latency p50/p99 and tick p99) and a CPU baseline: the FP32 oracle (a
-restatement of the reference's single-threaded Solver::Step) timed on one
-host core over a bounded, size-stratified sample of the same scenes.
+dominant kernel (the primal block-solve sweep) and of the other sweep
+kernels, from an instrumented pass of --roofline-steps steps after the
+timed ones (HIP events around every launch on the stream it runs on), the
+served form over the same scenes (request server with every scene resident
+under a 200 GB HBM budget, Poisson arrivals at --serve-load fractions of its
+own closed-loop saturation: request latency p50/p99 and tick p99), a CPU
+baseline (the FP32 oracle, a restatement of the reference's single-threaded
+Solver::Step, timed on one host core over a bounded, size-stratified sample
+of the same scenes) and the plain GPU step against that CPU step on the
+landed 20k-box world.
 """
 from __future__ import annotations
 
@@ -62,6 +67,7 @@ KERNEL_CLASSES = ("primal_forces", "primal_fem", "dual")
 KERNEL_NAMES = {
     "primal_fem": "k_primal<G, true> (FEM-only bucket of a primal colour sweep)",
     "primal_forces": "k_primal<G, false> (bucket of bodies with forces)",
+    "dual": "k_dual (dual and penalty update of every contact, joint and spring row)",
 }
 
 
@@ -194,6 +200,61 @@ def plain_gpu_rate(world, steps=3, configs=((8, 16), (4, 4))):
     return 1.0 / (t_w / total_w)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def boxes20k_plain_vs_cpu(no_cpu, settle=360, steps=30, cpu_steps=3):
+    """The plain per-scene GPU step against the CPU step on the 20k-box
+    world (40 x 25 x 20 lattice, 1k-link chain, 5k springs) once its boxes
+    have landed: the GPU world settles `settle` steps, then `steps` steps
+    are timed (step graphs); the landed state is downloaded and the FP32
+    oracle (one thread, natural order) times `cpu_steps` steps of it after
+    one untimed step."""
+    from skyephyx_amd import scenes
+    from skyephyx_amd.capi import Context
+
+    world = scenes.box_lattice(shape=(40, 25, 20), spacing=3.5, n_chain=1000, n_springs=5000)
+    with Context() as ctx:
+        ctx.upload(world)
+        for i in range(0, settle, 30):
+            ctx.step(min(30, settle - i))
+        ctx.step(5)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        ctx.synchronize()
+        gpu = steps / (time.perf_counter() - t0)
+        state = ctx.download()
+        em, _ = ctx.status(1)
+    out = {"gpu_steps_per_s": round(gpu, 2), "state": f"after {settle + 5} GPU steps: {len(state.manifolds)} "
+           f"manifolds, {len(state.bodies)} bodies, emergency stop {int(em[0])}", "gpu_steps_timed": steps}
+    log(f"[bench]   20k boxes GPU {gpu:.1f} steps/s ({out['state']})")
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+
+        o = O.Oracle(state, fp64=False, order=O.NATURAL)
+        o.step(1)
+        times = []
+        for _ in range(cpu_steps):
+            t0 = time.perf_counter()
+            o.step(1)
+            times.append(time.perf_counter() - t0)
+        cpu = 1.0 / float(np.median(times))
+        out.update(cpu_steps_per_s=round(cpu, 4), cpu_cores=1, cpu_kind="port",
+                   cpu_sample=f"FP32 oracle, 1 thread, {cpu_steps} steps timed after 1 untimed, median",
+                   gpu_over_cpu=round(gpu / cpu, 1))
+        log(f"[bench]   20k boxes CPU {cpu:.3f} steps/s -> {gpu / cpu:.0f}x")
+    return out
+
+
 def served(world, fractions, duration, budget=200e9):
     """Served form over the same scenes: every scene registered and resident
     under an HBM budget, requests of U{1..8} steps of a random scene.  The
@@ -230,20 +291,28 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle", type=int, default=150,
+                    help="steps run after the upload and before the warm-up, so the timed steps see the settled "
+                         "contact load whatever --warmup is (the mix's piles roughly triple their contacts over "
+                         "the first ~100 steps)")
+    ap.add_argument("--roofline-steps", type=int, default=20,
+                    help="steps of the instrumented pass after the timed steps (eager launches with an event "
+                         "pair around every sweep launch; the source of the roofline figures)")
     ap.add_argument("--workload", default="mix")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body with forces (0 = autotuned)")
     ap.add_argument("--fem-lanes", type=int, default=0, help="lanes per FEM-only body (0 = autotuned)")
-    ap.add_argument("--no-kernel-timing", action="store_true",
-                    help="time the steps without per-launch events (step graphs replay; no roofline figures)")
+    ap.add_argument("--no-roofline", action="store_true", help="skip the instrumented pass (no roofline figures)")
+    ap.add_argument("--no-boxes20k", action="store_true",
+                    help="skip the plain GPU step vs the CPU step on the landed 20k-box world")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel of a step one by one")
     ap.add_argument("--fused", action="store_true", help="small scenes in the per-scene kernel (option fused_scenes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson request) measurement")
-    ap.add_argument("--serve-load", type=float, nargs="+", default=[0.8],
+    ap.add_argument("--serve-load", type=float, nargs="+", default=[0.1, 0.5, 0.8, 1.0],
                     help="served load points as fractions of the served (closed-loop) saturation")
-    ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per served load point")
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival window per served load point")
     ap.add_argument("--debug-output", action="store_true",
                     help="make the renderer's line/point data after every step (option debug_output; "
                          "reports their counts; the timed steps include it)")
@@ -274,15 +343,17 @@ def main():
     ctx.upload(world)
     ctx.synchronize()
     log(f"[bench] uploaded in {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    for i in range(0, args.settle, 10):
+        ctx.step(min(10, args.settle - i))
+    ctx.synchronize()
+    log(f"[bench] settled {args.settle} steps in {time.perf_counter() - t0:.1f}s")
     for i in range(args.warmup):
         ctx.step(1)
     ctx.synchronize()
     log(f"[bench] warmup {args.warmup} done")
-    # per-launch events on the library's stream over the timed region (a
-    # step then launches its kernels one by one instead of replaying a graph)
-    ctx.set_option("kernel_timing", 0 if args.no_kernel_timing else 1)
-    for k in KERNEL_CLASSES + ("step_graph",):
-        ctx.kernel_stats(k, reset=True)
+    # the timed steps: the product's defaults (step graphs, no per-launch events)
+    ctx.kernel_stats("step_graph", reset=True)
     step_ms = []
     ctx.synchronize()
     t_start = time.perf_counter()
@@ -292,12 +363,24 @@ def main():
         step_ms.append((time.perf_counter() - a) * 1e3)
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
-    ks = {k: ctx.kernel_stats(k) for k in KERNEL_CLASSES}
     graph_replays = ctx.kernel_stats("step_graph")[0]
-    ctx.set_option("kernel_timing", 0)
     timings = ctx.timings()
     state = ctx.download()
     em, _ = ctx.status(len(state.scenes))
+    # instrumented pass: HIP events on the stream each sweep launch runs on,
+    # one pair per launch (the steps launch eagerly meanwhile)
+    ks = None
+    if not args.no_roofline and args.roofline_steps > 0:
+        ctx.set_option("kernel_timing", 1)
+        for k in KERNEL_CLASSES:
+            ctx.kernel_stats(k, reset=True)
+        for i in range(args.roofline_steps):
+            ctx.step(1)
+        ctx.synchronize()
+        ks = {k: ctx.kernel_stats(k) for k in KERNEL_CLASSES}
+        ctx.set_option("kernel_timing", 0)
+        log(f"[bench] instrumented pass {args.roofline_steps} steps: " +
+            ", ".join(f"{k} {n} launches {ms:.1f} ms" for k, (n, ms) in ks.items()))
     tune = ctx.tune_table()
     ctx_bytes = ctx.memory()
     debug = None
@@ -331,15 +414,41 @@ def main():
 
     S = len(state.scenes)
     scene_steps = S * args.steps / elapsed
-    primal_bytes, dual_bytes = roofline.step_bytes(state)
     forces_bytes, fem_bytes = roofline.class_step_bytes(state)
+    dual_bytes = roofline.step_bytes(state)[1]
     updates = roofline.body_updates_per_step(state)
-    # dominant kernel: the primal bucket class with the larger device time
-    cls = "primal_fem" if ks["primal_fem"][1] >= ks["primal_forces"][1] else "primal_forces"
-    n_cls, ms_cls = ks[cls]
-    cls_bytes = fem_bytes if cls == "primal_fem" else forces_bytes
-    achieved = cls_bytes * args.steps / (ms_cls * 1e-3) / 1e9 if ms_cls > 0 else 0.0
-    n_dual, ms_dual = ks["dual"]
+
+    def kernel_roofline(cls):
+        """Roofline object of one kernel class over the instrumented pass:
+        algorithmic bytes per step (the state after the timed steps) x the
+        pass's steps / the summed launch durations."""
+        n, ms = ks[cls]
+        step_b = {"primal_fem": fem_bytes, "primal_forces": forces_bytes, "dual": dual_bytes}[cls]
+        ach = step_b * args.roofline_steps / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {
+            "bound": "hbm",
+            "kernel": KERNEL_NAMES[cls],
+            "achieved": round(ach, 2),
+            "peak": roofline.HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(ach / roofline.HBM_PEAK_GBPS, 5),
+            "traffic": measured_traffic(args.workload, args.lanes, cls),
+            "launches": n,
+            "avg_launch_us": round(ms * 1e3 / max(n, 1), 3),
+            "device_ms_per_step": round(ms / args.roofline_steps, 4),
+            "algorithmic_bytes_per_launch": int(step_b * args.roofline_steps / max(n, 1)),
+        }
+
+    roof = None
+    if ks is not None:
+        # dominant kernel: the primal bucket class with the larger device time
+        cls = "primal_fem" if ks["primal_fem"][1] >= ks["primal_forces"][1] else "primal_forces"
+        other = "primal_forces" if cls == "primal_fem" else "primal_fem"
+        roof = kernel_roofline(cls)
+        roof["measured_over"] = (f"{args.roofline_steps} steps right after the timed ones, launched eagerly with "
+                                 f"an event pair around every sweep launch on the stream it runs on")
+        roof["other_bucket"] = kernel_roofline(other) if ks[other][0] else None
+        roof["dual"] = kernel_roofline("dual")
     res = {
         "metric": METRIC,
         "value": round(scene_steps, 3),
@@ -365,32 +474,14 @@ def main():
             "tuned_lanes": {f"{'fem' if k[0] else 'forces'}/2^{k[1]}": v for k, v in sorted(tune.items())},
             "colours_last_step": int(timings["n_colours"]),
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": KERNEL_NAMES[cls],
-            "achieved": round(achieved, 2),
-            "peak": roofline.HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": round(achieved / roofline.HBM_PEAK_GBPS, 5),
-            "traffic": measured_traffic(args.workload, args.lanes, cls),
-            "launches": n_cls,
-            "avg_launch_us": round(ms_cls * 1e3 / max(n_cls, 1), 3),
-            "algorithmic_bytes_per_launch": int(cls_bytes * args.steps / max(n_cls, 1)),
-            "other_bucket": {
-                "kernel": KERNEL_NAMES["primal_forces" if cls == "primal_fem" else "primal_fem"],
-                "launches": ks["primal_forces" if cls == "primal_fem" else "primal_fem"][0],
-                "device_ms": round(ks["primal_forces" if cls == "primal_fem" else "primal_fem"][1], 3),
-            },
-        },
+        "roofline": roof,
         "p50_step_ms": round(float(np.percentile(step_ms, 50)), 4),
         "p99_step_ms": round(float(np.percentile(step_ms, 99)), 4),
         "body_vertex_updates_per_s": round(updates * args.steps / elapsed, 1),
         "phase_ms_last_step": {k: round(v, 4) for k, v in timings.items() if k.endswith("_ms")},
-        "dual_avg_launch_us": round(ms_dual * 1e3 / max(n_dual, 1), 3),
+        "timed_state": f"after {args.settle} settling and {args.warmup} warm-up steps",
         "step_graph_replays": graph_replays,
     }
-    if args.no_kernel_timing:
-        res["roofline"] = None
     if not args.no_plain and S > 1:
         log("[bench] plain per-scene comparison")
         plain = plain_gpu_rate(state)
@@ -410,7 +501,9 @@ def main():
             "points": points,
             "phase_ms_last_tick": {k: round(v, 4) for k, v in stim.items() if k.endswith("_ms")},
         }
-        main_pt = points[0]
+        # headline point: the load nearest 0.8 of saturation
+        main_pt = min(points, key=lambda p: abs(p["load_fraction"] - 0.8))
+        res["served_headline_load"] = main_pt["load_fraction"]
         res["served_p50_latency_ms"] = main_pt["latency_ms_p50"]
         res["served_p99_latency_ms"] = main_pt["latency_ms_p99"]
         res["served_p99_tick_ms"] = main_pt["tick_ms_p99"]
@@ -425,7 +518,7 @@ def main():
             "cores": 1,
             "kind": "port",
             "sample": cpu_desc,
-            "host": platform.processor() or platform.machine(),
+            "host": cpu_model(),
         }
         res["gpu_over_cpu"] = round(scene_steps / cpu_rate, 2)
         if S > 1:
@@ -440,6 +533,9 @@ def main():
                           f"({n_steps} steps); weighted median step time under contention",
             }
             res["gpu_over_cpu_all_cores"] = round(scene_steps / rate_all, 2)
+    if not args.no_boxes20k and args.workload == "mix":
+        log("[bench] 20k boxes: plain GPU step vs the CPU step on the landed world")
+        res["boxes20k_plain_gpu_vs_cpu"] = boxes20k_plain_vs_cpu(args.no_cpu)
     if snapshot:
         res["snapshot"] = snapshot
     if debug:
