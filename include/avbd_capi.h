@@ -33,6 +33,9 @@
  *   avbd_download_bodies,
  *   avbd_upload_bodies        -> per-frame reads/writes of Mesh state
  *                                (geometry.hpp:148-256)
+ *   avbd_set_drag             -> Mesh::isDragged / addedDragVelocity
+ *                                (geometry.hpp:193-194), read by Step's
+ *                                velocity update (solver.cpp:506-510)
  *   avbd_get_timings          -> Solver::timings / SolverTimings
  *                                (solver.hpp:10-19, solver.cpp:599-626)
  *   avbd_step_scenes          -> (no counterpart: batched/served stepping of a
@@ -396,6 +399,16 @@ int avbd_download_bodies(avbd_ctx* ctx, avbd_body* out, int32_t n);
  * geometry.hpp:193-194).  Mass, inertia, scale and flags change only
  * through avbd_upload. */
 int avbd_upload_bodies(avbd_ctx* ctx, const avbd_body* in, int32_t n);
+
+/* Mesh::isDragged and Mesh::addedDragVelocity (geometry.hpp:193-194) of the
+ * listed bodies (indices among the live bodies, as avbd_download_bodies
+ * orders them; added_velocity: 3 floats per body; dragged: 0 or 1).  The
+ * next velocity update adds a dragged body's added velocity to its new
+ * velocity once and clears it (solver.cpp:506-510); the flag stays until set
+ * again.  Bodies not listed keep their state.  Removed and added bodies
+ * start undragged.  Not kept in snapshots. */
+int avbd_set_drag(avbd_ctx* ctx, const int32_t* bodies, const float* added_velocity, const int32_t* dragged,
+                  int32_t n);
 
 int avbd_get_timings(avbd_ctx* ctx, avbd_timings* out);
 

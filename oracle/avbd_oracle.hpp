@@ -45,6 +45,8 @@ struct Body {
     R radius = R(0);
     bool isStatic = false;
     bool isParticle = false;
+    bool isDragged = false;       // Mesh::isDragged (geometry.hpp:193)
+    V3<R> addedDragVelocity;      // Mesh::addedDragVelocity (geometry.hpp:194)
     V3<R> lastPos, inertialPos;
     Q<R> lastRot, inertialRot;
     M3<R> Iw;  // cached world inertia R I R^T (solver.cpp:214-231)
@@ -1244,7 +1246,12 @@ struct World {
             if (b->isStatic) continue;
             b->prevVel = b->vel;
             b->prevAngVel = b->angVel;
-            b->vel = (b->pos - b->lastPos) / dt;
+            V3<R> nv = (b->pos - b->lastPos) / dt;
+            if (b->isDragged) {  // solver.cpp:506-510: one-shot added velocity of a dragged body
+                nv = nv + b->addedDragVelocity;
+                b->addedDragVelocity = V3<R>(R(0), R(0), R(0));
+            }
+            b->vel = nv;
             if (!b->isParticle) b->angVel = rotation_difference(b->rot, b->lastRot) / dt;
             const V3<R>& p = b->pos;
             if (p.x < boundsMin.x || p.x > boundsMax.x || p.y < boundsMin.y || p.y > boundsMax.y ||

@@ -25,6 +25,18 @@ struct Handle {
 
 // Solver::RemoveBody (solver.cpp:104-116) for the bodies at the listed
 // indices (indices before the call), in the listed order.
+// Mesh::isDragged / addedDragVelocity of the listed bodies (velocity: 3 per body).
+template <class R>
+int set_drag(World<R>& w, const int32_t* idx, const float* vel, const int32_t* dragged, int n) {
+    for (int k = 0; k < n; ++k)
+        if (idx[k] < 0 || idx[k] >= int(w.bodies.size())) return -1;
+    for (int k = 0; k < n; ++k) {
+        Body<R>* b = w.bodies[idx[k]].get();
+        b->isDragged = dragged[k] != 0;
+        b->addedDragVelocity = V3<R>(R(vel[3 * k]), R(vel[3 * k + 1]), R(vel[3 * k + 2]));
+    }
+    return 0;
+}
 template <class R>
 int remove_listed(World<R>& w, const int32_t* idx, int n) {
     std::vector<Body<R>*> victims;
@@ -418,6 +430,10 @@ int orc_step(void* hp, int n) {
         if (h->fp64) h->wd.step(); else h->wf.step();
     }
     return 0;
+}
+int orc_set_drag(void* hp, const int32_t* idx, const float* vel, const int32_t* dragged, int n) {
+    Handle* h = static_cast<Handle*>(hp);
+    return h->fp64 ? set_drag(h->wd, idx, vel, dragged, n) : set_drag(h->wf, idx, vel, dragged, n);
 }
 int orc_remove_bodies(void* hp, const int32_t* idx, int n) {
     Handle* h = static_cast<Handle*>(hp);

@@ -42,6 +42,8 @@ def lib():
         L.orc_set_order.argtypes = [C.c_void_p, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_remove_bodies.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int]
+        L.orc_set_drag.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                                   C.c_int]
         L.orc_debug.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.POINTER(C.c_int64),
                                 C.POINTER(C.c_int64)]
         L.orc_counts.argtypes = [C.c_void_p, C.POINTER(AvbdWorld)]
@@ -93,6 +95,17 @@ class Oracle:
         """Solver::RemoveBody of the bodies at these indices (before the call), in order."""
         a = np.ascontiguousarray(idx, np.int32)
         if self.L.orc_remove_bodies(self.h, a.ctypes.data_as(C.POINTER(C.c_int32)), len(a)) != 0:
+            raise ValueError("body index out of range")
+
+    def set_drag(self, idx, added_velocity, dragged=True):
+        """Mesh::isDragged / addedDragVelocity (geometry.hpp:193-194) of the
+        bodies at these indices; the next velocity update adds the velocity
+        once (solver.cpp:506-510)."""
+        i = np.ascontiguousarray(idx, np.int32)
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(added_velocity, np.float32), (len(i), 3)))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dragged, np.int32), (len(i),)))
+        if self.L.orc_set_drag(self.h, i.ctypes.data_as(C.POINTER(C.c_int32)), _fptr(v),
+                               d.ctypes.data_as(C.POINTER(C.c_int32)), len(i)) != 0:
             raise ValueError("body index out of range")
 
     def debug(self):

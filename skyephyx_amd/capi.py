@@ -54,6 +54,7 @@ def lib():
                                          C.POINTER(C.c_int32)]
         L.avbd_download_bodies.argtypes = [vp, vp, C.c_int32]
         L.avbd_upload_bodies.argtypes = [vp, vp, C.c_int32]
+        L.avbd_set_drag.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32]
         L.avbd_get_colours.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.avbd_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
         L.avbd_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int32]
@@ -266,6 +267,16 @@ class Context:
     def upload_bodies(self, bodies: np.ndarray):
         b = np.ascontiguousarray(bodies, BODY_DTYPE)
         _check(self.L.avbd_upload_bodies(self.h, b.ctypes.data, len(b)))
+
+    def set_drag(self, bodies, added_velocity, dragged=True):
+        """Mesh::isDragged / addedDragVelocity of these bodies (live indices):
+        the next velocity update adds the velocity once."""
+        i = np.ascontiguousarray(bodies, np.int32)
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(added_velocity, np.float32), (len(i), 3)))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dragged, np.int32), (len(i),)))
+        _check(self.L.avbd_set_drag(self.h, i.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    v.ctypes.data_as(C.POINTER(C.c_float)), d.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    len(i)))
 
     def timings(self) -> dict:
         t = AvbdTimings()

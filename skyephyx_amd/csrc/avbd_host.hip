@@ -514,6 +514,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     w.sc = *d_sc;
     w.sc_off = *d_off;
     A(&w.pos, N); A(&w.rot, N); A(&w.scale, N); A(&w.vel, N); A(&w.angvel, N); A(&w.pvel, N); A(&w.pangvel, N);
+    A(&w.drag, N);
     A(&w.inertia, N); A(&w.flags, N); A(&w.scene, N); A(&w.last_pos, N); A(&w.last_rot, N); A(&w.inert_pos, N);
     A(&w.inert_rot, N); A(&w.diff0, N); A(&w.diff1, N);
     if ((rc = alloc_manifolds(c, c->mb[0], Mcap))) return rc;
@@ -1147,7 +1148,7 @@ int remove_bodies(avbd_ctx* c, const std::vector<char>& aff, std::vector<int>& s
         return rc;
     {
         float4* f4[] = {w.pos, w.rot, w.scale, w.vel, w.angvel, w.pvel, w.pangvel, w.inertia,
-                        w.last_pos, w.last_rot, w.inert_pos, w.inert_rot, w.diff0, w.diff1};
+                        w.last_pos, w.last_rot, w.inert_pos, w.inert_rot, w.diff0, w.diff1, w.drag};
         for (float4* arr : f4) {
             hipLaunchKernelGGL(k_gather_f4, dim3(grid_for(K)), dim3(256), 0, st, d_tmp, arr, d_src, K);
             hipLaunchKernelGGL(k_scatter_f4, dim3(grid_for(K)), dim3(256), 0, st, arr, d_tmp, d_dst, K);
@@ -2136,10 +2137,10 @@ int avbd_add_bodies(avbd_ctx* c, const avbd_body* bodies, int32_t n) {
         // the record as avbd_upload converts it; the step's caches cleared
         const float4 f4s[] = {f4(b.pos, 0.f), make_float4(b.rot[0], b.rot[1], b.rot[2], b.rot[3]), f4(b.scale, b.radius),
                               f4(b.vel, b.mass), f4(b.ang_vel, b.friction), f4(b.prev_vel, 0.f),
-                              f4(b.prev_ang_vel, 0.f), f4(b.inertia, 0.f), z, z, z, z, z, z};
+                              f4(b.prev_ang_vel, 0.f), f4(b.inertia, 0.f), z, z, z, z, z, z, z};
         float4* dst[] = {w.pos, w.rot, w.scale, w.vel, w.angvel, w.pvel, w.pangvel, w.inertia,
-                         w.last_pos, w.last_rot, w.inert_pos, w.inert_rot, w.diff0, w.diff1};
-        for (int a = 0; a < 14; ++a)
+                         w.last_pos, w.last_rot, w.inert_pos, w.inert_rot, w.diff0, w.diff1, w.drag};
+        for (int a = 0; a < 15; ++a)
             HIP_TRY(hipMemcpyAsync(dst[a] + i, &f4s[a], sizeof(float4), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(w.flags + i, &b.flags, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
         const float4 white = make_float4(1.f, 1.f, 1.f, 0.f);
@@ -2337,6 +2338,32 @@ int avbd_upload_bodies(avbd_ctx* c, const avbd_body* in, int32_t n) {
     if ((rc = upload_vec(c, w.pos, pos)) || (rc = upload_vec(c, w.rot, rot)) || (rc = upload_vec(c, w.vel, vel)) ||
         (rc = upload_vec(c, w.angvel, av)) || (rc = upload_vec(c, w.pvel, pv)) || (rc = upload_vec(c, w.pangvel, pav)))
         return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int avbd_set_drag(avbd_ctx* c, const int32_t* bodies, const float* added_velocity, const int32_t* dragged,
+                  int32_t n) {
+    if (!c || (n > 0 && (!bodies || !added_velocity || !dragged)) || n < 0) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    if (n == 0) return 0;
+    HIP_TRY(hipSetDevice(c->device));
+    DevWorld& w = c->w;
+    // live index -> device slot
+    std::vector<int> slot;
+    slot.reserve(w.N);
+    for (int i = 0; i < w.N; ++i)
+        if (!c->body_dead[i]) slot.push_back(i);
+    std::vector<float4> drag;
+    int rc;
+    if ((rc = download_vec(c, drag, w.drag, w.N))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; ++k) {
+        if (bodies[k] < 0 || bodies[k] >= (int)slot.size()) return fail(AVBD_E_ARG, "body index out of range");
+        const float* v = added_velocity + 3 * k;
+        drag[slot[bodies[k]]] = make_float4(v[0], v[1], v[2], dragged[k] ? 1.f : 0.f);
+    }
+    if ((rc = upload_vec(c, w.drag, drag))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }

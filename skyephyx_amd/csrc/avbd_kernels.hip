@@ -1755,6 +1755,11 @@ AVBD_DEV void velocity_body(const DevWorld& w, int b, int s) {
             const float4 d0 = w.diff0[b], d1 = w.diff1[b];
             const float dt = sc.dt;
             v4v = make_float4(d0.x / dt, d0.y / dt, d0.z / dt, v4v.w);
+            const float4 dr = w.drag[b];
+            if (dr.w != 0.f) {  // a dragged body: its added velocity, once (solver.cpp:506-510)
+                v4v = make_float4(v4v.x + dr.x, v4v.y + dr.y, v4v.z + dr.z, v4v.w);
+                w.drag[b] = make_float4(0.f, 0.f, 0.f, dr.w);
+            }
             w.vel[b] = v4v;
             if (!(fl & AVBD_BODY_PARTICLE)) w.angvel[b] = make_float4(d0.w / dt, d1.x / dt, d1.y / dt, av4.w);
             const float4 p = w.pos[b];
@@ -1956,6 +1961,7 @@ __global__ void k_kill_bodies(DevWorld w, const int* list, int n) {
         w.angvel[b] = make_float4(0.f, 0.f, 0.f, av.w);
         w.pvel[b] = make_float4(0.f, 0.f, 0.f, 0.f);
         w.pangvel[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        w.drag[b] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 

@@ -104,6 +104,7 @@ struct DevWorld {
     float4* pvel;       // prev velocity
     float4* pangvel;    // prev angular velocity
     float4* inertia;    // body-frame diagonal
+    float4* drag;       // xyz: velocity added by the next velocity update, w: dragged (Mesh::isDragged)
     uint32_t* flags;
     int* scene;
     float4* last_pos;

@@ -681,8 +681,26 @@ void Solver::Step() {
     if (structureDirty_) {
         if (forcesStale_) SyncForces();
         Upload();
+        dragOnDevice_ = false;  // a fresh upload starts undragged
     } else if (state_changed) {
         check(avbd_upload_bodies(ctx_, bodies.data(), (int32_t)bodies.size()), "avbd_upload_bodies");
+    }
+    // Mesh::isDragged / addedDragVelocity (geometry.hpp:193-194): the velocity
+    // update adds a dragged body's added velocity once (solver.cpp:506-510)
+    bool dragging = false;
+    for (Mesh* m : bodyPtrs) dragging = dragging || m->isDragged;
+    if (dragging || dragOnDevice_) {
+        const int32_t n = (int32_t)bodyPtrs.size();
+        std::vector<int32_t> idx(n), flag(n);
+        std::vector<float> vel(3 * (size_t)n);
+        for (int32_t i = 0; i < n; ++i) {
+            const Mesh* m = bodyPtrs[i];
+            idx[i] = i;
+            flag[i] = m->isDragged ? 1 : 0;
+            for (int k = 0; k < 3; ++k) vel[3 * i + k] = m->isDragged ? m->addedDragVelocity[k] : 0.f;
+        }
+        check(avbd_set_drag(ctx_, idx.data(), vel.data(), flag.data(), n), "avbd_set_drag");
+        dragOnDevice_ = dragging;
     }
     check(avbd_set_option(ctx_, "debug_output", debugOutput ? 1 : 0), "avbd_set_option");
     if (debugOutput) {
@@ -692,6 +710,9 @@ void Solver::Step() {
         check(avbd_set_body_colours(ctx_, rgb.data(), (int32_t)solverBodies.size()), "avbd_set_body_colours");
     }
     check(avbd_step(ctx_, 1), "avbd_step");
+    if (dragging)  // the step's velocity update consumed the added velocities
+        for (Mesh* m : bodyPtrs)
+            if (m->isDragged && !m->isStatic) m->addedDragVelocity = Vector3f::Zero();
     lineData.clear();
     debugPointData.clear();
     if (debugOutput) {

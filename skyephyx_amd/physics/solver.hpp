@@ -160,6 +160,7 @@ private:
     int32_t nextEnergySerial_ = 0;
     std::vector<unsigned char> lastScene_;   // scene record of the last upload
     std::vector<unsigned char> lastBodies_;  // body records as last uploaded / read back
+    bool dragOnDevice_ = false;  // some body's dragged flag is set on the device
 
     void EnsureContext();
     void Upload();

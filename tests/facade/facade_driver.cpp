@@ -8,6 +8,8 @@
 //   facade_driver keep <scene> <snap> <before> <after>
 //                                           records before and after a refused
 //                                           LoadSnapshot (no GPU)
+//   facade_driver drag <scene> <before> <after> <body> <dragged> <out>
+//                                           records after a body is dragged
 //   facade_driver timings <scene> <steps> <explode_at>
 //                                           per step the C ABI's timings and the
 //                                           facade's SolverTimings
@@ -241,6 +243,26 @@ int main(int argc, char** argv) {
                             f.totalSubstepMs, s.averageStepTime, int(s.emergencyStop));
             }
             return 0;
+        }
+        if (mode == "drag" && argc == 8) {
+            // drag <scene> <steps before> <steps after> <body> <dragged> <out>:
+            // Mesh::isDragged and addedDragVelocity (2, 1.5, 0) of one body
+            // set between the steps; after each later step the added velocity
+            // reads 0 again (printed)
+            Solver s;
+            if (!build(s, argv[2])) return 2;
+            avbd_ctx* ctx = s.DeviceContext();
+            if (avbd_set_option(ctx, "lanes", 1) || avbd_set_option(ctx, "fem_lanes", 1)) return 3;
+            for (int k = 0, n = std::atoi(argv[3]); k < n; ++k) s.Step();
+            Mesh* m = s.bodyPtrs.at(std::atoi(argv[5]));
+            m->isDragged = std::atoi(argv[6]) != 0;
+            m->addedDragVelocity = Vector3f(2.f, 1.5f, 0.f);
+            for (int k = 0, n = std::atoi(argv[4]); k < n; ++k) {
+                s.Step();
+                std::printf("added %g %g %g dragged %d\n", m->addedDragVelocity[0], m->addedDragVelocity[1],
+                            m->addedDragVelocity[2], int(m->isDragged));
+            }
+            return write_records(s, argv[7]);
         }
         if (mode == "keep" && argc == 6) {
             // keep <scene> <bad snapshot> <before out> <after out>: a populated

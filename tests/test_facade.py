@@ -420,3 +420,33 @@ def test_solver_timings_skip_the_stopping_step(driver):
     assert [e for _, _, e in fac] == [0] * 8 + [1] * 4
     for k in range(8, 12):
         assert fac[k][0] == fac[7][0] and fac[k][1] == fac[7][1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dragged", [1, 0])
+def test_drag_through_the_facade(driver, tmp_path, dragged):
+    """Mesh::isDragged / addedDragVelocity set on a facade body between steps
+    (the reference's mouse drag, geometry.hpp:193-194, solver.cpp:506-510):
+    the facade's steps equal the C ABI driven with avbd_set_drag, the added
+    velocity is consumed by one step (reads 0 afterwards) when the body is
+    dragged and left alone when it is not."""
+    from skyephyx_amd.capi import Context
+
+    before, after, body = 8, 4, 54
+    out = str(tmp_path / "d.bin")
+    r = subprocess.run([driver, "drag", "pyramid", str(before), str(after), str(body), str(dragged), out],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    added = [line.split()[1:4] for line in r.stdout.splitlines() if line.startswith("added")]
+    assert len(added) == after
+    assert all(a == (["0", "0", "0"] if dragged else ["2", "1.5", "0"]) for a in added), added
+    fac = read_records(out)
+    subprocess.run([driver, "records", "pyramid", str(tmp_path / "w.bin")], check=True)
+    with Context(lanes=1, fem_lanes=1) as ctx:
+        ctx.upload(read_records(str(tmp_path / "w.bin")))
+        ctx.step(before)
+        ctx.set_drag([body], (2.0, 1.5, 0.0), dragged)
+        for _ in range(after):
+            ctx.step(1)
+        ref = ctx.download()
+    _same(fac.bodies, ref.bodies, ("pos", "rot", "vel", "ang_vel", "prev_vel", "prev_ang_vel"), "bodies")

@@ -237,3 +237,43 @@ def test_spare_slots_change_nothing_and_take_added_bodies():
         res = summary(sp.download().extract_scene(0), ref.save())
         assert res["pairs_equal"] and res["n_equal"] and res["ids_equal"], res
         assert res["pos_rel"] <= POS_TOL and res["rot"] <= POS_TOL, res
+
+
+@pytest.mark.parametrize("lanes", [(1, 1), (0, 0)], ids=["seq", "tuned"])
+def test_drag_matches_oracle(lanes):
+    """Mesh::isDragged / addedDragVelocity (solver.cpp:506-510) set on the
+    resident world: a dragged box of a settled pile and one of a joint chain
+    get their added velocity once in the next velocity update, the flag stays
+    set, an undragged body ignores its added velocity.  Each step against the
+    oracle given the same drag state; also after a removal renumbered the
+    bodies (the drag state moves with its body)."""
+    o = O.Oracle(World.concat([scenes.box_stack_3d(levels=3, drop=0.02)]), fp64=False, order=O.NATURAL)
+    o.step(15)
+    state = o.save()
+    n = len(state.bodies)
+    with Context(lanes=lanes[0], fem_lanes=lanes[1]) as ctx:
+        ctx.upload(state)
+        ref = O.Oracle(state, fp64=False, order=O.COLOUR_MAJOR)
+        drag = [(n - 1, (1.5, 2.0, 0.0), 1), (n - 3, (0.0, 0.0, -3.0), 1), (2, (9.0, 0.0, 0.0), 0)]
+        for i, v, d in drag:
+            ctx.set_drag([i], v, d)
+            ref.set_drag([i], v, d)
+        for step in range(3):
+            ctx.step(1)
+            ref.step(1)
+            g, r = ctx.download(), ref.save()
+            np.testing.assert_allclose(g.bodies["vel"], r.bodies["vel"], rtol=0, atol=2e-3)
+            np.testing.assert_allclose(g.bodies["pos"], r.bodies["pos"], rtol=0, atol=1e-4)
+            assert len(g.manifolds) == len(r.manifolds), step
+        # the dragged box left with about its drag added
+        assert g.bodies["vel"][n - 1][0] > 0.5
+        # a removal renumbers: body n-1 swaps into slot 0, keeps its (spent) drag flag
+        ctx.remove_bodies([0])
+        ref.remove_bodies([0])
+        ctx.set_drag([0], (0.0, 4.0, 0.0), 1)
+        ref.set_drag([0], (0.0, 4.0, 0.0), 1)
+        ctx.step(1)
+        ref.step(1)
+        g, r = ctx.download(), ref.save()
+        np.testing.assert_allclose(g.bodies["vel"], r.bodies["vel"], rtol=0, atol=2e-3)
+        np.testing.assert_allclose(g.bodies["pos"], r.bodies["pos"], rtol=0, atol=1e-4)

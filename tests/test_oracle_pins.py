@@ -839,3 +839,27 @@ def test_fracture_resets_the_manifolds_of_both_linked_bodies():
         assert np.all(m["contacts"]["lambda"][:n] == 0) and np.all(m["contacts"]["penalty"][:n] == 0), k
     m = by_body[3]
     assert np.any(m["contacts"]["lambda"][: m["n_contacts"]] != 0)
+
+
+# ----------------------------------------------------------- drag (solver.cpp:506-510)
+def test_drag_adds_its_velocity_once():
+    """A dragged body's added velocity joins the velocity the step computes
+    from its displacement, once: a free-falling box's first step gives
+    v = g dt + v_drag (first step: no predicted acceleration, so x = y),
+    the next step g dt more and no drag; an undragged body with an added
+    velocity set keeps it unused."""
+    b = WorldBuilder()
+    b.add_body(MODEL_CUBE, 1.0, 0.5, (0, 5, 0), (1, 1, 1))
+    b.add_body(MODEL_CUBE, 1.0, 0.5, (5, 5, 0), (1, 1, 1))
+    o = O.Oracle(b.build(), fp64=True)
+    o.set_drag([0], (2.0, 0.5, -1.0), True)
+    o.set_drag([1], (7.0, 0.0, 0.0), False)
+    o.step(1)
+    v1 = o.save().bodies["vel"].astype(np.float64)
+    g_dt = -9.81 / 60.0
+    np.testing.assert_allclose(v1[0], [2.0, 0.5 + g_dt, -1.0], atol=1e-5)
+    np.testing.assert_allclose(v1[1], [0.0, g_dt, 0.0], atol=1e-5)
+    o.step(1)
+    v2 = o.save().bodies["vel"].astype(np.float64)
+    # the second step moves by v1 dt (+ g dt^2): its velocity is v1 + g dt, no drag again
+    np.testing.assert_allclose(v2[0], v1[0] + [0.0, g_dt, 0.0], atol=1e-4)
