@@ -471,7 +471,8 @@ def main():
             "n_manifolds": int(len(state.manifolds)),
             "lanes_per_body": args.lanes or "autotuned",
             "lanes_per_fem_vertex": args.fem_lanes or "autotuned",
-            "tuned_lanes": {f"{'fem' if k[0] else 'forces'}/2^{k[1]}": v for k, v in sorted(tune.items())},
+            "tuned_lanes_workgroup": {f"{'fem' if k[0] else 'forces'}/2^{k[1]}": list(v)
+                                      for k, v in sorted(tune.items())},
             "colours_last_step": int(timings["n_colours"]),
         },
         "roofline": roof,

@@ -448,11 +448,14 @@ int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
  *                    from scratch.  The colouring is the same either way. */
 int avbd_set_option(avbd_ctx* ctx, const char* name, int64_t value);
 
-/* Autotuner table (no reference counterpart): the lane width chosen per
- * (kernel class, log2 of bodies per launch) as JSON {"key": width, ...}.
- * Writes at most len bytes (NUL-terminated) and returns the full length. */
+/* Autotuner table (no reference counterpart): the launch shape chosen per
+ * (kernel class, log2 of bodies per launch) of the primal sweep, lanes per
+ * body and workgroup size, as JSON {"key": [lanes, workgroup], ...}; keys
+ * still being tuned are left out.  Writes at most len bytes (NUL-terminated)
+ * and returns the full length. */
 int avbd_tune_table(avbd_ctx* ctx, char* out, int32_t len);
-/* Seed the autotuner with a table written by avbd_tune_table. */
+/* Seed the autotuner with a table written by avbd_tune_table (a bare
+ * width, {"key": width}, takes workgroup size 256). */
 int avbd_tune_load(avbd_ctx* ctx, const char* json);
 
 /* Accumulated launch count and device time (ms) of one kernel class since

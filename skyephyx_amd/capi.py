@@ -291,19 +291,21 @@ class Context:
         return int(n.value), float(ms.value)
 
     def tune_table(self) -> dict:
-        """Autotuned lane widths: {(kernel class, log2 bodies per launch): lanes}."""
+        """Autotuned launch shapes: {(kernel class, log2 bodies per launch):
+        (lanes, workgroup size)}."""
         import json
 
         n = self.L.avbd_tune_table(self.h, None, 0)
         buf = C.create_string_buffer(n + 1)
         self.L.avbd_tune_table(self.h, buf, n + 1)
         raw = json.loads(buf.value.decode() or "{}")
-        return {(int(k) // 64, int(k) % 64): v for k, v in raw.items()}
+        return {(int(k) // 64, int(k) % 64): tuple(v) if isinstance(v, list) else (v, 256) for k, v in raw.items()}
 
     def tune_load(self, table: dict):
         import json
 
-        raw = {str(cls * 64 + band): v for (cls, band), v in table.items()}
+        raw = {str(cls * 64 + band): list(v) if isinstance(v, (tuple, list)) else v
+               for (cls, band), v in table.items()}
         _check(self.L.avbd_tune_load(self.h, json.dumps(raw).encode()))
 
     def colours(self, n: int) -> np.ndarray:

@@ -353,8 +353,8 @@ int build_incidence(avbd_ctx* c) {
 // Blocks of a bucket launch: enough for n bodies at G lanes, rounded up to a
 // power of two (the kernel strides over the bucket, whose size it reads from
 // device memory), so that a step graph keyed on it survives small changes.
-int bucket_grid(const avbd_ctx* c, int n, int G) {
-    const int g = grid_for((long long)n * G, 256, 16384);
+int bucket_grid(const avbd_ctx* c, int n, int G, int block = LaneTuner::kDefaultBlock) {
+    const int g = grid_for((long long)n * G, block, 16384);
     // In a world of several scenes at least 64 blocks (the surplus ones exit
     // at once): the launch shape of small buckets then does not change with
     // their exact size, so a served tick's step graph repeats although its
@@ -365,24 +365,25 @@ int bucket_grid(const avbd_ctx* c, int n, int G) {
 }
 
 template <bool FEM>
-void launch_primal_g(avbd_ctx* c, int G, int grid, int bucket, int iter, int poststab) {
+void launch_primal_g(avbd_ctx* c, int G, int grid, int block, int bucket, int iter, int poststab) {
     const DevWorld& w = c->w;
     switch (G) {
-        case 1: hipLaunchKernelGGL((k_primal<1, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
-        case 2: hipLaunchKernelGGL((k_primal<2, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
-        case 4: hipLaunchKernelGGL((k_primal<4, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
-        case 8: hipLaunchKernelGGL((k_primal<8, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
-        case 16: hipLaunchKernelGGL((k_primal<16, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
-        default: hipLaunchKernelGGL((k_primal<32, FEM>), dim3(grid), dim3(256), 0, c->stream, w, bucket, iter, poststab); break;
+        case 1: hipLaunchKernelGGL((k_primal<1, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+        case 2: hipLaunchKernelGGL((k_primal<2, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+        case 4: hipLaunchKernelGGL((k_primal<4, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+        case 8: hipLaunchKernelGGL((k_primal<8, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+        case 16: hipLaunchKernelGGL((k_primal<16, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+        default: hipLaunchKernelGGL((k_primal<32, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
     }
 }
 
 // Lane width of a bucket launch of n bodies (class 0 = bodies with forces,
 // 1 = FEM-only) without starting a tuning sample: the fixed width, the tuned
 // one, or 0 while the tuner still samples this size.
-int bucket_lanes(const avbd_ctx* c, bool fem, int n) {
+int bucket_lanes(const avbd_ctx* c, bool fem, int n, int* block) {
     const int fixed = fem ? c->fem_lanes : c->lanes;
-    return fixed > 0 ? fixed : c->tuner.peek(fem ? 1 : 0, n);
+    *block = LaneTuner::kDefaultBlock;
+    return fixed > 0 ? fixed : c->tuner.peek(fem ? 1 : 0, n, block);
 }
 
 // One bucket launch on c->stream with the lane width fixed by the user or
@@ -390,11 +391,11 @@ int bucket_lanes(const avbd_ctx* c, bool fem, int n) {
 template <bool FEM>
 void launch_bucket(avbd_ctx* c, int bucket, int n, int iter, int poststab) {
     int fixed = FEM ? c->fem_lanes : c->lanes;
-    int measure = -1;
-    int G = fixed > 0 ? fixed : c->tuner.choose(FEM ? 1 : 0, n, &measure);
+    int measure = -1, block = LaneTuner::kDefaultBlock;
+    int G = fixed > 0 ? fixed : c->tuner.choose(FEM ? 1 : 0, n, &measure, &block);
     const int mark = c->mark_begin(FEM ? avbd_ctx::K_FEM : avbd_ctx::K_FORCES);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].a, c->stream);
-    launch_primal_g<FEM>(c, G, bucket_grid(c, n, G), bucket, iter, poststab);
+    launch_primal_g<FEM>(c, G, bucket_grid(c, n, G, block), block, bucket, iter, poststab);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].b, c->stream);
     c->mark_end(mark);
 }
@@ -1326,9 +1327,11 @@ bool step_graph_key(const avbd_ctx* c, const int* off, int ncol, int max_iters, 
             key->push_back(0);
             continue;
         }
-        const int G = bucket_lanes(c, (k & 1) != 0, n);
+        int block = 0;
+        const int G = bucket_lanes(c, (k & 1) != 0, n, &block);
         if (G == 0) return false;
-        key->push_back(G * 65536 + bucket_grid(c, n, G));
+        key->push_back(G * 65536 + bucket_grid(c, n, G, block));
+        key->push_back(block);
     }
     return true;
 }

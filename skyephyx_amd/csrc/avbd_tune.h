@@ -1,14 +1,21 @@
 // avbd_tune.h -- autotuner for the launch shape of the primal sweep.
 //
-// The lanes per body (G) trade per-body latency (more lanes share a body's
-// incidence list) against redundant per-body work and reduction cost (every
-// lane of the group solves the block system).  Which wins depends on how
-// much parallel work a launch has: a 32^3 FEM block alone wants 16 lanes, the
-// 4096-scene mix wants 4.  The tuner keys on (kernel class, log2 of the
-// bodies in the launch), tries every candidate width a few times on live
-// launches (timed with HIP events on the stream that runs the kernel), and
-// then keeps the fastest.  The table can be written to and read from a small
-// JSON file so later runs start tuned.
+// Two parameters of a bucket launch are tuned.  The lanes per body (G) trade
+// per-body latency (more lanes share a body's incidence list) against
+// redundant per-body work and reduction cost (every lane of the group solves
+// the block system); which wins depends on how much parallel work a launch
+// has: a 32^3 FEM block alone wants 16 lanes, the 4096-scene mix wants 4.
+// The workgroup size (64, 128 or 256 threads) then sets how many bodies a
+// workgroup holds and how the launch spreads over the CUs.  Neither changes
+// a body's arithmetic (a lane group is always inside one wave), so the
+// choice never changes results.
+//
+// The tuner keys on (kernel class, log2 of the bodies in the launch).  A key
+// first samples every candidate width at 256 threads, a few times each, on
+// live launches (timed with HIP events on the stream that runs the kernel),
+// keeps the fastest, then samples the three workgroup sizes at that width
+// and keeps the fastest.  The table can be written to and read from a small
+// JSON object so later runs start tuned.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,6 +29,9 @@
 struct LaneTuner {
     static constexpr int kNumCands = 6;
     static constexpr int kCands[kNumCands] = {1, 2, 4, 8, 16, 32};
+    static constexpr int kNumBlocks = 3;
+    static constexpr int kBlocks[kNumBlocks] = {64, 128, 256};
+    static constexpr int kDefaultBlock = 256;
     static constexpr int kSamples = 2;
 
     struct Entry {
@@ -30,10 +40,15 @@ struct LaneTuner {
         unsigned mask = 0;  // candidates sampled (bit c: kCands[c]); set at first use
         int count[kNumCands] = {0};
         double us_per_body[kNumCands] = {0.0};
+        int block = 0;      // chosen workgroup size (after best), 0 while tuning
+        int bnext = 0;
+        int bcount[kNumBlocks] = {0};
+        double bus[kNumBlocks] = {0.0};
     };
     struct Pending {
         int key;
-        int cand;
+        int cand;   // width candidate (stage 1) or workgroup candidate (stage 2)
+        bool block; // stage 2
         int n;
         hipEvent_t a, b;
     };
@@ -64,20 +79,35 @@ struct LaneTuner {
         return pool[pool_used++];
     }
 
-    // Width to use for a launch of class cls over n bodies.  While the key is
-    // being tuned, *measure receives the index of a pending record whose
-    // events the caller records around the launch; else -1.
-    int choose(int cls, int n, int* measure) {
+    // Width and workgroup size for a launch of class cls over n bodies.
+    // While the key is being tuned, *measure receives the index of a pending
+    // record whose events the caller records around the launch; else -1.
+    int choose(int cls, int n, int* measure, int* block) {
         *measure = -1;
+        *block = kDefaultBlock;
         const int k = key(cls, n);
         Entry& e = table[k];
-        if (!enabled || e.best) return e.best ? e.best : 8;
+        if (!enabled) {
+            if (e.best && e.block) *block = e.block;
+            return e.best ? e.best : 8;
+        }
+        if (e.best && e.block) {
+            *block = e.block;
+            return e.best;
+        }
+        if (e.best) {  // stage 2: workgroup sizes at the chosen width
+            const int bi = e.bnext % kNumBlocks;
+            e.bnext = bi + 1;
+            pending.push_back(Pending{k, bi, true, n, event(), event()});
+            *measure = int(pending.size()) - 1;
+            *block = kBlocks[bi];
+            return e.best;
+        }
         if (!e.mask) e.mask = seed_mask(k);
         int ci = e.next % kNumCands;
         while (!((e.mask >> ci) & 1u)) ci = (ci + 1) % kNumCands;
         e.next = ci + 1;
-        Pending p{key(cls, n), ci, n, event(), event()};
-        pending.push_back(p);
+        pending.push_back(Pending{k, ci, false, n, event(), event()});
         *measure = int(pending.size()) - 1;
         return kCands[ci];
     }
@@ -99,13 +129,15 @@ struct LaneTuner {
         return (1u << kNumCands) - 1;
     }
 
-    // The width choose() would return without sampling: 0 while the key is
-    // still being tuned.
-    int peek(int cls, int n) const {
+    // The width choose() would return without sampling (and its workgroup
+    // size in *block): 0 while the key is still being tuned.
+    int peek(int cls, int n, int* block) const {
         auto it = table.find(key(cls, n));
         const int best = it == table.end() ? 0 : it->second.best;
+        const int blk = it == table.end() ? 0 : it->second.block;
+        *block = blk ? blk : kDefaultBlock;
         if (!enabled) return best ? best : 8;
-        return best;
+        return blk ? best : 0;
     }
 
     // After the stream(s) have been synchronised.
@@ -114,6 +146,20 @@ struct LaneTuner {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) continue;
             Entry& e = table[p.key];
+            if (p.block) {
+                if (e.block) continue;
+                e.bcount[p.cand]++;
+                e.bus[p.cand] += 1e3 * ms / (p.n > 0 ? p.n : 1);
+                bool done = true;
+                for (int c = 0; c < kNumBlocks; ++c) done = done && e.bcount[c] >= kSamples;
+                if (done) {
+                    int bi = 0;
+                    for (int c = 1; c < kNumBlocks; ++c)
+                        if (e.bus[c] / e.bcount[c] < e.bus[bi] / e.bcount[bi]) bi = c;
+                    e.block = kBlocks[bi];
+                }
+                continue;
+            }
             if (e.best) continue;
             e.count[p.cand]++;
             e.us_per_body[p.cand] += 1e3 * ms / (p.n > 0 ? p.n : 1);
@@ -132,20 +178,24 @@ struct LaneTuner {
         pool_used = 0;
     }
 
+    // {"key": [width, workgroup], ...} for every fully tuned key.
     std::string to_json() const {
         std::string s = "{";
         bool first = true;
         for (const auto& kv : table) {
-            if (!kv.second.best) continue;
+            if (!kv.second.best || !kv.second.block) continue;
             char buf[96];
-            std::snprintf(buf, sizeof(buf), "%s\"%d\": %d", first ? "" : ", ", kv.first, kv.second.best);
+            std::snprintf(buf, sizeof(buf), "%s\"%d\": [%d, %d]", first ? "" : ", ", kv.first, kv.second.best,
+                          kv.second.block);
             s += buf;
             first = false;
         }
         return s + "}";
     }
 
-    // Parses what to_json writes: {"key": width, ...}.
+    // Parses what to_json writes; a bare width ("key": width, the earlier
+    // format) takes the default workgroup size.  Values outside the
+    // candidate sets are ignored.
     void from_json(const std::string& s) {
         size_t i = 0;
         while ((i = s.find('"', i)) != std::string::npos) {
@@ -154,11 +204,27 @@ struct LaneTuner {
             int k = std::atoi(s.substr(i + 1, j - i - 1).c_str());
             size_t c = s.find(':', j);
             if (c == std::string::npos) break;
-            int v = std::atoi(s.c_str() + c + 1);
-            bool ok = false;
+            size_t v0 = s.find_first_not_of(" \t\n", c + 1);
+            if (v0 == std::string::npos) break;
+            int v = 0, blk = kDefaultBlock;
+            if (s[v0] == '[') {
+                v = std::atoi(s.c_str() + v0 + 1);
+                size_t comma = s.find(',', v0);
+                size_t close = s.find(']', v0);
+                if (comma != std::string::npos && close != std::string::npos && comma < close)
+                    blk = std::atoi(s.c_str() + comma + 1);
+                i = close == std::string::npos ? v0 + 1 : close + 1;
+            } else {
+                v = std::atoi(s.c_str() + v0);
+                i = v0 + 1;
+            }
+            bool ok = false, okb = false;
             for (int q = 0; q < kNumCands; ++q) ok = ok || kCands[q] == v;
-            if (ok) table[k].best = v;
-            i = c + 1;
+            for (int q = 0; q < kNumBlocks; ++q) okb = okb || kBlocks[q] == blk;
+            if (ok && okb) {
+                table[k].best = v;
+                table[k].block = blk;
+            }
         }
     }
 };
