@@ -117,6 +117,9 @@ struct avbd_ctx {
     // a scene whose constraint graph changed a little updates its last
     // colouring from the changed bodies instead of colouring from scratch
     bool colour_updates = true;
+    // FEM-only sweeps read each vertex's neighbour positions from LDS (its
+    // neighbour list) instead of gathering four positions per element
+    bool use_nb = true;
     int spare_bodies = 0;  // dead slots per scene added at upload (avbd_add_bodies fills them)
     // Debug/render output (option debug_output): counters, offsets and the
     // contact points come from alloc (an upload frees them; set up again on
@@ -506,6 +509,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     const int S = d.S, N = d.N, J = d.J, SP = d.SP, T = d.T, TR = d.TR, Mcap = d.Mcap;
     DevWorld& w = c->w;
     w.S = S; w.N = N; w.J = J; w.SP = SP; w.T = T; w.TR = TR; w.Mcap = Mcap;
+    w.use_nb = c->use_nb ? 1 : 0;
     int rc = 0;
 #define A(p, n) if ((rc = c->alloc(p, n))) return rc
     A(d_sc, S);
@@ -537,6 +541,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     w.e_ent = *e_ent;
     A(&w.ev_a, (size_t)4 * T + 3 * TR + 1); A(&w.ev_b, (size_t)4 * T + 3 * TR + 1);
     A(&w.ev_c, (size_t)4 * T + 3 * TR + 1); A(&w.ev_v, (size_t)4 * T + 3 * TR + 1);
+    A(&w.nb_id, 2 * ((size_t)4 * T + 3 * TR) + 1); A(&w.nb_n, N); A(&w.ev_l, (size_t)4 * T + 3 * TR + 1);
     A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
     A(&w.ccount, N); A(&w.cseed, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1);
     w.pair_cap = (int)d.pair_cap;
@@ -620,6 +625,7 @@ int upload_energies(avbd_ctx* c, int N, const avbd_tet* tets, int T, const avbd_
     // records cross once, not once per vertex)
     if (eoff[N] > 0) {
         hipLaunchKernelGGL(k_element_stream, dim3(grid_for(eoff[N])), dim3(256), 0, c->stream, w, eoff[N]);
+        hipLaunchKernelGGL(k_nb_build, dim3(grid_for(N)), dim3(256), 0, c->stream, w, 0, N);
         HIP_TRY(hipGetLastError());
     } else {
         // no elements: the one padding entry of each stream array is zero
@@ -1127,6 +1133,9 @@ int remove_bodies(avbd_ctx* c, const std::vector<char>& aff, std::vector<int>& s
             (rc = upload_vec(c, w.ev_b + E0, nb)) || (rc = upload_vec(c, w.ev_c + E0, nc)) ||
             (rc = upload_vec(c, w.ev_v + E0, nv)))
             return rc;
+        // the segment's neighbour lists from its rebuilt streams (new indices)
+        hipLaunchKernelGGL(k_nb_build, dim3(grid_for(g.e - g.a)), dim3(256), 0, st, w, g.a, g.e);
+        HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(st));
     }
     // device renumbering of the moved bodies, then of the forces' and
@@ -1616,6 +1625,12 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
     }
     if (n == "colour_updates") {
         c->colour_updates = value != 0;
+        return 0;
+    }
+    if (n == "fem_neighbour_lists") {
+        c->use_nb = value != 0;
+        c->w.use_nb = c->use_nb ? 1 : 0;
+        c->drop_graphs();  // captured launches carry the world by value
         return 0;
     }
     if (n == "step_graphs") {

@@ -40,6 +40,7 @@ __global__ void k_remap_forces(DevWorld w, BodyRemap rm);
 __global__ void k_remap_manifolds(DevWorld w, BodyRemap rm);
 __global__ void k_kill_bodies(DevWorld w, const int* list, int n);
 __global__ void k_element_stream(DevWorld w, int n);
+__global__ void k_nb_build(DevWorld w, int b0, int b1);
 __global__ void k_remap_elements(DevWorld w, BodyRemap rm);
 __global__ void k_dbg_count(DevWorld w, int* lc, int* pc);
 __global__ void k_live_flag(DevWorld w, int* flag);

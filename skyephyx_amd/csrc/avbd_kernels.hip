@@ -1393,8 +1393,26 @@ AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int k, int projMode, v
 // performs on that block; the rotation increment is exactly zero.
 // One FEM-only particle: its G lanes (a group within one wave) share the
 // particle's energy list.
+// Positions of an element from the vertex's neighbour slots in LDS: slot
+// byte 0xFF is the vertex itself (or a triangle's unused slot 3).
+AVBD_DEV void elem_pos_lds(const float4* nbp, uint32_t l, v3 own, ElemPos& p) {
+    const uint32_t i0 = l & 0xFFu, i1 = (l >> 8) & 0xFFu, i2 = (l >> 16) & 0xFFu, i3 = l >> 24;
+    p.p0 = i0 == 0xFFu ? own : f3(nbp[i0]);
+    p.p1 = i1 == 0xFFu ? own : f3(nbp[i1]);
+    p.p2 = i2 == 0xFFu ? own : f3(nbp[i2]);
+    p.p3 = i3 == 0xFFu ? own : f3(nbp[i3]);
+}
+
+// Order LDS writes of a wave's lanes before other lanes of the same wave
+// read them (a lane group never straddles a wave).
+AVBD_DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int G>
-AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, int poststab) {
+AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, int poststab, float4* nbp = nullptr) {
     const int s = w.scene[b];
     const avbd_scene& sc = w.sc[s];
     if (w.st[s * ST_WORDS + ST_EMERGENCY]) return;
@@ -1416,6 +1434,54 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
     // 0) so that none sits under a branch, which would make every wait
     // drain all loads in flight.
     const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
+    // With an LDS slot (bucket sweeps, G >= 4): the group loads the vertex's
+    // distinct neighbours once and each element reads its positions from
+    // there (about one gather per element instead of four).  The positions
+    // are the same values the gathers would read: neighbours share an
+    // element with the vertex, so none has its colour and none moves in
+    // this launch.
+    const int nn = nbp ? w.nb_n[b] : -1;
+    if (nn >= 0) {
+        const int* list = w.nb_id + 2 * (size_t)e0;
+        for (int j = lane; j < nn; j += G) nbp[j] = w.pos[list[j]];
+        wave_lds_sync();
+        int k = e0 + lane;
+        if (k < e1) {
+            const int last = e1 - 1;
+            ElemIn in0;
+            in0.e = w.e_ent[k];
+            in0.a = w.ev_a[k];
+            in0.b = w.ev_b[k];
+            in0.c = w.ev_c[k];
+            uint32_t l0 = w.ev_l[k];
+            bool h0 = true;
+            while (h0) {
+                const bool h1 = k + G < e1;
+                const int k1 = min(k + G, last);
+                ElemIn in1;
+                in1.e = w.e_ent[k1];
+                in1.a = w.ev_a[k1];
+                in1.b = w.ev_b[k1];
+                in1.c = w.ev_c[k1];
+                const uint32_t l1 = w.ev_l[k1];
+                ElemPos p0;
+                elem_pos_lds(nbp, l0, p, p0);
+                v3 g;
+                float H[6];
+                elem_terms(in0, p0, sc.projection_mode, g, H);
+                r[0] = r[0] + g.x;
+                r[1] = r[1] + g.y;
+                r[2] = r[2] + g.z;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) L[i] = L[i] + H[i];
+                in0 = in1;
+                l0 = l1;
+                h0 = h1;
+                k += G;
+            }
+        }
+        wave_lds_sync();  // the slot is reused by the group's next vertex
+    } else {
     int k = e0 + lane;
     if (k < e1) {
         const int last = e1 - 1;
@@ -1452,6 +1518,7 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
             k += G;
         }
     }
+    }
     if (G > 1) {
 #pragma unroll
         for (int off = 1; off < G; off <<= 1) {
@@ -1482,10 +1549,15 @@ template <int G>
 AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int poststab) {
     const int lane = threadIdx.x % G;
     const int gpb = blockDim.x / G;
+    // neighbour position slots: NB_MAX per lane group (32 KiB at G = 4 and
+    // 256 threads); narrower groups would need too much LDS and gather
+    constexpr int kGroups = G >= 4 ? 256 / G : 1;
+    __shared__ float4 nb_lds[kGroups * NB_MAX];
+    float4* slot = (G >= 4 && w.use_nb) ? nb_lds + (threadIdx.x / G) * NB_MAX : nullptr;
     for (int base = blockIdx.x * gpb; base < hi - lo; base += gridDim.x * gpb) {
         const int item = base + threadIdx.x / G;
         if (item >= hi - lo) continue;  // lane groups never straddle a wave: the group leaves together
-        primal_fem_vertex<G>(w, w.col_list[lo + item], lane, iter, poststab);
+        primal_fem_vertex<G>(w, w.col_list[lo + item], lane, iter, poststab, slot);
     }
 }
 
@@ -1983,6 +2055,46 @@ __global__ void k_element_stream(DevWorld w, int n) {
             w.ev_c[k] = w.t_d[(size_t)idx * 3 + 2];
             w.ev_v[k] = w.t_v[idx];
         }
+    }
+}
+
+// Neighbour lists of the vertices [b0, b1) from their element streams (run
+// after the stream is built or rebuilt): one thread per vertex walks its
+// entries in order, keeps the distinct other vertices in first-appearance
+// order at nb_id[2 e_off[b] ...] (room for two per entry, at most NB_MAX),
+// and writes each entry's slot indices into ev_l.  A vertex whose
+// neighbours do not fit gets nb_n = -1 and is swept by element ids.
+__global__ void k_nb_build(DevWorld w, int b0, int b1) {
+    GRID_STRIDE(t, b1 - b0) {
+        const int b = b0 + t;
+        const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
+        const int cap = min(2 * (e1 - e0), NB_MAX);
+        int* list = w.nb_id + 2 * (size_t)e0;
+        int n = 0;
+        bool fits = true;
+        for (int k = e0; k < e1 && fits; ++k) {
+            const uint32_t e = w.e_ent[k];
+            const int4 v = w.ev_v[k];
+            const int local = (int)EENT_LOCAL(e);
+            const int slots = EENT_TRI(e) ? 3 : 4;
+            const int ids[4] = {v.x, v.y, v.z, v.w};
+            uint32_t packed = 0xFFFFFFFFu;
+            for (int q = 0; q < slots; ++q) {
+                if (q == local) continue;
+                int j = 0;
+                while (j < n && list[j] != ids[q]) ++j;
+                if (j == n) {
+                    if (n == cap) {
+                        fits = false;
+                        break;
+                    }
+                    list[n++] = ids[q];
+                }
+                packed = (packed & ~(0xFFu << (8 * q))) | ((uint32_t)j << (8 * q));
+            }
+            w.ev_l[k] = packed;
+        }
+        w.nb_n[b] = fits ? n : -1;
     }
 }
 

@@ -20,6 +20,8 @@
 #define EENT_TRI(e) ((e) >> 31)
 #define EENT_LOCAL(e) (((e) >> 29) & 3u)
 #define MAKE_EENT(tri, local, idx) (((uint32_t)(tri) << 31) | ((uint32_t)(local) << 29) | (uint32_t)(idx))
+// most neighbours a FEM vertex keeps in its list (and in LDS during a sweep)
+#define NB_MAX 32
 
 // force state flags (device); bit 0 = AVBD_FORCE_DISABLED
 #define FORCE_DEAD 2u
@@ -160,6 +162,16 @@ struct DevWorld {
     float4* ev_b;
     float4* ev_c;
     int4* ev_v;
+    // Neighbour lists of the FEM vertices: vertex b's distinct element
+    // neighbours (first appearance in its element list) at
+    // nb_id[2 e_off[b] ...], nb_n[b] of them (-1: more than fit, the
+    // vertex gathers by element ids instead); ev_l[k] packs, per slot of
+    // entry k's element, the slot vertex's index in that list (byte s; 0xFF
+    // for the vertex itself and a triangle's unused slot 3).
+    int* nb_id;         // [2 (4T + 3TR) + 1]
+    int* nb_n;          // [N]
+    uint32_t* ev_l;     // [4T + 3TR + 1]
+    int use_nb;         // FEM-only sweeps read neighbour positions from LDS (option fem_neighbour_lists)
     // colouring
     int* colour;        // [N]
     int* col_cnt;       // [AVBD_BUCKETS + 1]
