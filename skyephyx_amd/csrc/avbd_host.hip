@@ -156,6 +156,16 @@ struct avbd_ctx {
     int4* items = nullptr;  // broad-phase work items (scene, i-tile, j-tile)
     int n_items = 0;
     int n_tiles = 0;
+    // The same on the host, grouped by scene (item_off, tile_off: per-scene
+    // ranges), and the buffers a step over a subset of the scenes fills with
+    // the running scenes' items and tiles (pinned staging, device copy).
+    std::vector<int4> h_items;
+    std::vector<int> h_item_off, h_tile_off;
+    int4* sub_items = nullptr;
+    int* sub_tiles = nullptr;
+    int4* pin_items = nullptr;
+    int* pin_tiles = nullptr;
+    std::vector<int> run_ids;  // the running scenes of a subset step, ascending
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
     int lanes = 0;      // 0 = chosen by the tuner
@@ -178,6 +188,9 @@ struct avbd_ctx {
     hipEvent_t ev[8]{};
     avbd_timings timings{};
     int last_ncol = 0;
+    // set by avbd_step_scenes when some scenes sit the step out: the dual
+    // and stick passes then walk a list of the running scenes' manifolds
+    bool subset = false;
     // Per-launch timing (option kernel_timing): HIP events recorded on the
     // stream the work runs on, around each dual launch (K_DUAL) and each
     // primal bucket launch (K_FORCES, K_FEM).  Two events per launch and no
@@ -291,6 +304,12 @@ struct avbd_ctx {
         mb[0] = mb[1] = ManifoldBuf{};
         items = nullptr;
         n_items = n_tiles = 0;
+        sub_items = nullptr;
+        sub_tiles = nullptr;
+        if (pin_items) hipHostFree(pin_items);
+        if (pin_tiles) hipHostFree(pin_tiles);
+        pin_items = nullptr;
+        pin_tiles = nullptr;
         cub_tmp = nullptr;
         have_world = false;
         if (dbg.lines) hipFree(dbg.lines);
@@ -331,9 +350,26 @@ int download_vec(avbd_ctx* c, std::vector<T>& v, const T* src, size_t n) {
 
 float4 f4(const float* p, float w) { return make_float4(p[0], p[1], p[2], w); }
 
+// Exclusive prefix sum of n ints (in != out).  One tile: a single launch;
+// more: tile sums (held in the sort/scan temporary buffer), their scan, the
+// tiles.  A few launches of no more than n/16 threads each, where a
+// decoupled-lookback scan over a world-sized array spends most of its time
+// in the chained look-back between thousands of partitions.
 int scan_ints(avbd_ctx* c, const int* in, int* out, int n) {
-    size_t bytes = c->cub_bytes;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, in, out, n, c->stream));
+    if (n <= 0) return 0;
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) % 16)
+        return fail(AVBD_E_ARG, "scan buffers must be 16-byte aligned");
+    const int tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles == 1) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, c->stream, in, out, n, (const int*)nullptr);
+        return 0;
+    }
+    if ((size_t)(tiles + 1) * sizeof(int) > c->cub_bytes) return fail(AVBD_E_CAPACITY, "scan temporary too small");
+    int* sums = reinterpret_cast<int*>(c->cub_tmp);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(256), 0, c->stream, in, n, sums);
+    hipLaunchKernelGGL(k_scan_flags_1wg, dim3(1), dim3(1024), 0, c->stream, (const int*)sums, sums, tiles,
+                       sums + tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, c->stream, in, out, n, (const int*)sums);
     return 0;
 }
 
@@ -403,13 +439,38 @@ void launch_bucket(avbd_ctx* c, int bucket, int n, int iter, int poststab) {
     c->mark_end(mark);
 }
 
+// Both buckets of a colour in one k_primal_pair launch, when both are small
+// (together at most kPairBlocks blocks, well within one pass of the chip),
+// their shapes are settled (fixed or tuned widths, 256-thread blocks) and
+// an instantiation exists: forces at 4 lanes, FEM at 4 to 32.  Returns
+// false when the colour takes the two-launch path.
+constexpr int kPairBlocks = 512;
+bool launch_pair(avbd_ctx* c, int colour, int n_gen, int n_fem, int iter, int poststab) {
+    if (c->ktiming) return false;  // per-launch timing keeps the two classes apart
+    int br = 0, bf = 0;
+    const int gr = bucket_lanes(c, false, n_gen, &br), gf = bucket_lanes(c, true, n_fem, &bf);
+    if (gr != 4 || gf < 4 || br != LaneTuner::kDefaultBlock || bf != LaneTuner::kDefaultBlock) return false;
+    const int rb = bucket_grid(c, n_gen, gr), fb = bucket_grid(c, n_fem, gf);
+    if (rb + fb > kPairBlocks) return false;
+    const DevWorld& w = c->w;
+    switch (gf) {
+        case 4: hipLaunchKernelGGL((k_primal_pair<4, 4>), dim3(rb + fb), dim3(256), 0, c->stream, w, colour, iter, poststab, rb); break;
+        case 8: hipLaunchKernelGGL((k_primal_pair<4, 8>), dim3(rb + fb), dim3(256), 0, c->stream, w, colour, iter, poststab, rb); break;
+        case 16: hipLaunchKernelGGL((k_primal_pair<4, 16>), dim3(rb + fb), dim3(256), 0, c->stream, w, colour, iter, poststab, rb); break;
+        default: hipLaunchKernelGGL((k_primal_pair<4, 32>), dim3(rb + fb), dim3(256), 0, c->stream, w, colour, iter, poststab, rb); break;
+    }
+    return true;
+}
+
 // One colour of a sweep: the bodies with forces and the FEM-only bodies, two
 // instantiations that run concurrently (the FEM bucket on the side stream,
 // forked from and joined back into the main stream with events, so the next
-// colour still sees both).  Empty buckets are not launched.
+// colour still sees both), or one paired launch for small buckets.  Empty
+// buckets are not launched.
 int launch_colour(avbd_ctx* c, const int* off, int colour, int iter, int poststab) {
     const int n_gen = off[2 * colour + 1] - off[2 * colour], n_fem = off[2 * colour + 2] - off[2 * colour + 1];
     if (n_gen == 0 && n_fem == 0) return 0;
+    if (n_gen > 0 && n_fem > 0 && launch_pair(c, colour, n_gen, n_fem, iter, poststab)) return 0;
     if (n_gen > 0 && n_fem > 0) {
         HIP_TRY(hipEventRecord(c->fork, c->stream));
         HIP_TRY(hipStreamWaitEvent(c->side, c->fork, 0));
@@ -556,6 +617,8 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     c->n_tiles = d.n_tiles;
     c->n_items = d.n_items;
     A(&c->items, std::max(d.n_items, 1));
+    A(&c->sub_items, std::max(d.n_items, 1));
+    A(&c->sub_tiles, std::max(d.n_tiles, 1));
     A(&w.alive, Mcap + 1); A(&w.alive_off, Mcap + 1);
     A(&w.fracture_bodies, (size_t)2 * (J + SP) + 2);
     A(&w.g, G_WORDS);
@@ -649,7 +712,7 @@ int upload_energies(avbd_ctx* c, int N, const avbd_tet* tets, int T, const avbd_
 int upload_tiles(avbd_ctx* c, const std::vector<int>& off) {
     DevWorld& w = c->w;
     const int S = (int)off.size() - 1;
-    std::vector<int> toff(S + 1, 0), tscene;
+    std::vector<int> toff(S + 1, 0), tscene, ioff(S + 1, 0);
     std::vector<int4> items;
     for (int s = 0; s < S; ++s) {
         const int tiles = (int)broad_tiles(off[s + 1] - off[s]);
@@ -658,6 +721,7 @@ int upload_tiles(avbd_ctx* c, const std::vector<int>& off) {
             tscene.push_back(s);
             for (int tj = ti; tj < tiles; ++tj) items.push_back(make_int4(s, ti, tj, 0));
         }
+        ioff[s + 1] = (int)items.size();
     }
     int rc;
     if ((rc = upload_vec(c, const_cast<int*>(w.tile_off), toff)) ||
@@ -665,6 +729,15 @@ int upload_tiles(avbd_ctx* c, const std::vector<int>& off) {
         return rc;
     c->n_tiles = (int)tscene.size();
     c->n_items = (int)items.size();
+    if (c->pin_items) hipHostFree(c->pin_items);
+    if (c->pin_tiles) hipHostFree(c->pin_tiles);
+    c->pin_items = nullptr;
+    c->pin_tiles = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&c->pin_items, std::max<size_t>(items.size(), 1) * sizeof(int4), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&c->pin_tiles, std::max<size_t>(tscene.size(), 1) * sizeof(int), hipHostMallocDefault));
+    c->h_items = std::move(items);
+    c->h_item_off = std::move(ioff);
+    c->h_tile_off = std::move(toff);
     return 0;
 }
 
@@ -731,6 +804,9 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
         hipcub::DoubleBuffer<unsigned long long> kb(nullptr, nullptr);
         HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b3, kb, (int)dims.pair_cap, 0, pair_key_bits(N), c->stream));
         dims.scan_bytes = std::max(dims.scan_bytes, std::max(b1, std::max(b2, b3)));
+        // the tile sums of scan_ints
+        const size_t tile_sums = ((size_t)std::max(N + 1, dims.Mcap + 1) / SCAN_TILE + 2) * sizeof(int);
+        dims.scan_bytes = std::max(dims.scan_bytes, tile_sums);
     }
     DevWorld& w = c->w;
     avbd_scene* d_sc;
@@ -1361,9 +1437,30 @@ int do_step(avbd_ctx* c) {
     // ---- 1. broad phase
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_NEW_PAIRS, 0);
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, w.g + G_CANDIDATES, 0);
-    if (c->n_items > 0) {
-        hipLaunchKernelGGL(k_tile_bounds, dim3(c->n_tiles), dim3(256), 0, st, w);
-        hipLaunchKernelGGL(k_broad, dim3(c->n_items), dim3(256), 0, st, w, (const int4*)c->items);
+    // With a subset of the scenes running, only their tiles and work items
+    // are launched (the others' would exit at once, but a world of
+    // thousands of scenes has hundreds of thousands of them).
+    int n_tiles = c->n_tiles, n_items = c->n_items;
+    const int* tiles = nullptr;
+    const int4* items = c->items;
+    if (c->subset && n_items > 0) {
+        n_tiles = n_items = 0;
+        for (int s : c->run_ids) {
+            for (int t = c->h_tile_off[s]; t < c->h_tile_off[s + 1]; ++t) c->pin_tiles[n_tiles++] = t;
+            const int i0 = c->h_item_off[s], i1 = c->h_item_off[s + 1];
+            std::copy(c->h_items.begin() + i0, c->h_items.begin() + i1, c->pin_items + n_items);
+            n_items += i1 - i0;
+        }
+        if (n_items > 0) {
+            HIP_TRY(hipMemcpyAsync(c->sub_tiles, c->pin_tiles, n_tiles * sizeof(int), hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(c->sub_items, c->pin_items, n_items * sizeof(int4), hipMemcpyHostToDevice, st));
+        }
+        tiles = c->sub_tiles;
+        items = c->sub_items;
+    }
+    if (n_items > 0) {
+        hipLaunchKernelGGL(k_tile_bounds, dim3(n_tiles), dim3(256), 0, st, w, tiles);
+        hipLaunchKernelGGL(k_broad, dim3(n_items), dim3(256), 0, st, w, items);
         hipLaunchKernelGGL(k_pair_filter, dim3(grid_for(N)), dim3(256), 0, st, w);
         if (c->small_pairs) {
             hipLaunchKernelGGL(k_pair_sort_small, dim3(1), dim3(1024), 0, st, w);
@@ -1887,6 +1984,7 @@ int avbd_step(avbd_ctx* c, int32_t n_steps) {
     for (int s = 0; s < c->w.S; ++s)
         if (!st[s * ST_WORDS + ST_ACTIVE]) { st[s * ST_WORDS + ST_ACTIVE] = 1; changed = true; }
     if (changed) HIP_TRY(hipMemcpy(c->w.st, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->subset = false;
     return step_impl(c, n_steps);
 }
 
@@ -1901,7 +1999,13 @@ int avbd_step_scenes(avbd_ctx* c, const int32_t* ids, int32_t n_ids, int32_t n_s
         st[ids[i] * ST_WORDS + ST_ACTIVE] = 1;
     }
     HIP_TRY(hipMemcpy(c->w.st, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
-    return step_impl(c, n_steps);
+    c->run_ids.clear();
+    for (int s = 0; s < c->w.S; ++s)
+        if (st[s * ST_WORDS + ST_ACTIVE]) c->run_ids.push_back(s);
+    c->subset = (int)c->run_ids.size() < c->w.S;
+    const int rc = step_impl(c, n_steps);
+    c->subset = false;
+    return rc;
 }
 
 int avbd_synchronize(avbd_ctx* c) {

@@ -5,7 +5,7 @@
 
 __global__ void k_fill_int(int* p, int n, int v);
 __global__ void k_set_word(int* p, int v);
-__global__ void k_tile_bounds(DevWorld w);
+__global__ void k_tile_bounds(DevWorld w, const int* tiles);
 __global__ void k_broad(DevWorld w, const int4* items);
 __global__ void k_pair_filter(DevWorld w);
 __global__ void k_pair_sort_small(DevWorld w);
@@ -26,6 +26,8 @@ __global__ void k_colour_fill(DevWorld w);
 __global__ void k_predict(DevWorld w);
 template <int G, bool FEM>
 __global__ void k_primal(DevWorld w, int bucket, int iter, int poststab);
+template <int GR, int GF>
+__global__ void k_primal_pair(DevWorld w, int colour, int iter, int poststab, int rigid_blocks);
 template <int G>
 __global__ void k_scene_step(DevWorld w, const int* scenes, int mode);
 __global__ void k_dual(DevWorld w, int iter);
@@ -44,6 +46,8 @@ __global__ void k_nb_build(DevWorld w, int b0, int b1);
 __global__ void k_remap_elements(DevWorld w, BodyRemap rm);
 __global__ void k_dbg_count(DevWorld w, int* lc, int* pc);
 __global__ void k_live_flag(DevWorld w, int* flag);
+__global__ void k_scan_reduce(const int* in, int n, int* sums);
+__global__ void k_scan_tiles(const int* in, int* out, int n, const int* sums);
 __global__ void k_scan_flags_1wg(const int* flag, int* off, int n, int* total);
 __global__ void k_instances(DevWorld w, const int* flag, const int* live_off, const float4* rgb, float4* out);
 __global__ void k_dbg_write(DevWorld w, DebugOut d, const int* lc, const int* pc);

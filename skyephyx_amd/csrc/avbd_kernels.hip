@@ -84,8 +84,10 @@ AVBD_DEV bool boxes_disjoint(float4 alo, float4 ahi, float4 blo, float4 bhi) {
     return ahi.x < blo.x || bhi.x < alo.x || ahi.y < blo.y || bhi.y < alo.y || ahi.z < blo.z || bhi.z < alo.z;
 }
 
-__global__ __launch_bounds__(256) void k_tile_bounds(DevWorld w) {
-    const int t = blockIdx.x;
+// tiles == nullptr: block b bounds tile b; else tile tiles[b] (the tiles of
+// the scenes that run).
+__global__ __launch_bounds__(256) void k_tile_bounds(DevWorld w, const int* tiles) {
+    const int t = tiles ? tiles[blockIdx.x] : (int)blockIdx.x;
     const int s = w.tile_scene[t];
     const int i = w.sc_off[s] + (t - w.tile_off[s]) * 256 + threadIdx.x;
     const float inf = __builtin_huge_valf();
@@ -1545,8 +1547,9 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
     }
 }
 
+// Blocks bid of nblocks stride over the bucket [lo, hi).
 template <int G>
-AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int poststab) {
+AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int poststab, int bid, int nblocks) {
     const int lane = threadIdx.x % G;
     const int gpb = blockDim.x / G;
     // neighbour position slots: NB_MAX per lane group (32 KiB at G = 4 and
@@ -1554,7 +1557,7 @@ AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int p
     constexpr int kGroups = G >= 4 ? 256 / G : 1;
     __shared__ float4 nb_lds[kGroups * NB_MAX];
     float4* slot = (G >= 4 && w.use_nb) ? nb_lds + (threadIdx.x / G) * NB_MAX : nullptr;
-    for (int base = blockIdx.x * gpb; base < hi - lo; base += gridDim.x * gpb) {
+    for (int base = bid * gpb; base < hi - lo; base += nblocks * gpb) {
         const int item = base + threadIdx.x / G;
         if (item >= hi - lo) continue;  // lane groups never straddle a wave: the group leaves together
         primal_fem_vertex<G>(w, w.col_list[lo + item], lane, iter, poststab, slot);
@@ -1646,20 +1649,43 @@ AVBD_DEV void primal_rigid(const DevWorld& w, int b, bool valid, int iter, int p
     }
 }
 
-template <int G, bool FEM>
-__global__ __launch_bounds__(256) void k_primal(DevWorld w, int bucket, int iter, int poststab) {
-    const int lo = w.col_off[bucket], hi = w.col_off[bucket + 1];
-    if constexpr (FEM) {
-        primal_fem_body<G>(w, lo, hi, iter, poststab);
-        return;
-    }
+template <int G>
+AVBD_DEV void primal_rigid_body(const DevWorld& w, int lo, int hi, int iter, int poststab, int bid, int nblocks) {
     const int groups_per_block = blockDim.x / G;
-    for (int base = blockIdx.x * groups_per_block; base < hi - lo; base += gridDim.x * groups_per_block) {
+    for (int base = bid * groups_per_block; base < hi - lo; base += nblocks * groups_per_block) {
         const int item = base + threadIdx.x / G;
         const bool valid = item < hi - lo;
         primal_rigid<G>(w, valid ? w.col_list[lo + item] : 0, valid, iter, poststab);
     }
 }
+
+template <int G, bool FEM>
+__global__ __launch_bounds__(256) void k_primal(DevWorld w, int bucket, int iter, int poststab) {
+    const int lo = w.col_off[bucket], hi = w.col_off[bucket + 1];
+    if constexpr (FEM)
+        primal_fem_body<G>(w, lo, hi, iter, poststab, blockIdx.x, gridDim.x);
+    else
+        primal_rigid_body<G>(w, lo, hi, iter, poststab, blockIdx.x, gridDim.x);
+}
+
+// Both buckets of a colour in one launch: blocks [0, rigid_blocks) take the
+// bodies with forces (GR lanes each), the rest the FEM-only particles (GF
+// lanes).  The two sets of bodies are disjoint and neither reads what the
+// other writes in this colour, as with the two concurrent launches it
+// replaces; for small buckets it saves the fork and join between streams.
+template <int GR, int GF>
+__global__ __launch_bounds__(256) void k_primal_pair(DevWorld w, int colour, int iter, int poststab, int rigid_blocks) {
+    const int bid = blockIdx.x;
+    if (bid < rigid_blocks)
+        primal_rigid_body<GR>(w, w.col_off[2 * colour], w.col_off[2 * colour + 1], iter, poststab, bid, rigid_blocks);
+    else
+        primal_fem_body<GF>(w, w.col_off[2 * colour + 1], w.col_off[2 * colour + 2], iter, poststab, bid - rigid_blocks,
+                            (int)gridDim.x - rigid_blocks);
+}
+template __global__ void k_primal_pair<4, 4>(DevWorld, int, int, int, int);
+template __global__ void k_primal_pair<4, 8>(DevWorld, int, int, int, int);
+template __global__ void k_primal_pair<4, 16>(DevWorld, int, int, int, int);
+template __global__ void k_primal_pair<4, 32>(DevWorld, int, int, int, int);
 
 #define AVBD_PRIMAL_INST(G)                                                        \
     template __global__ void k_primal<G, false>(DevWorld, int, int, int); \
@@ -2320,6 +2346,79 @@ __global__ __launch_bounds__(1024) void k_scan_flags_1wg(const int* flag, int* o
         __syncthreads();
     }
     if (threadIdx.x == 0) *total = carry_s;
+}
+
+// Exclusive scan of n ints in tiles of SCAN_TILE (reduce, then scan):
+// k_scan_reduce writes each tile's sum, k_scan_flags_1wg scans the sums in
+// place, k_scan_tiles scans each tile from its offset.  Integer sums, so the
+// result does not depend on the tiling.
+AVBD_DEV int block_sum256(int v) {
+    __shared__ int ws[4];
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int* in, int n, int* sums) {
+    const int base = blockIdx.x * SCAN_TILE;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_TILE / 256; ++k) {
+        const int i = base + k * 256 + (int)threadIdx.x;
+        if (i < n) v += in[i];
+    }
+    v = block_sum256(v);
+    if (threadIdx.x == 0) sums[blockIdx.x] = v;
+}
+
+// Thread t scans the 16 consecutive entries base + 16 t ...; the thread
+// totals are scanned across the block (waves with __shfl_up, wave totals in
+// LDS).  sums == nullptr: a single tile starting at 0.
+__global__ __launch_bounds__(256) void k_scan_tiles(const int* in, int* out, int n, const int* sums) {
+    static_assert(SCAN_TILE == 256 * 16, "16 entries per thread");
+    const int base = blockIdx.x * SCAN_TILE + 16 * (int)threadIdx.x;
+    int v[16];
+    if (base + 16 <= n) {
+        const int4* q = reinterpret_cast<const int4*>(in + base);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int4 x = q[k];
+            v[4 * k] = x.x; v[4 * k + 1] = x.y; v[4 * k + 2] = x.z; v[4 * k + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = base + k < n ? in[base + k] : 0;
+    }
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += v[k];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = t;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    __shared__ int ws[4];
+    if (lane == 63) ws[wave] = x;
+    __syncthreads();
+    int acc = (sums ? sums[blockIdx.x] : 0) + x - t;
+    for (int k = 0; k < wave; ++k) acc += ws[k];
+    int o[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        o[k] = acc;
+        acc += v[k];
+    }
+    if (base + 16 <= n) {
+        int4* q = reinterpret_cast<int4*>(out + base);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = make_int4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (base + k < n) out[base + k] = o[k];
+    }
 }
 
 __global__ void k_instances(DevWorld w, const int* flag, const int* live_off, const float4* rgb, float4* out) {

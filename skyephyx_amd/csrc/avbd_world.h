@@ -22,6 +22,8 @@
 #define MAKE_EENT(tri, local, idx) (((uint32_t)(tri) << 31) | ((uint32_t)(local) << 29) | (uint32_t)(idx))
 // most neighbours a FEM vertex keeps in its list (and in LDS during a sweep)
 #define NB_MAX 32
+// entries per workgroup of the tiled exclusive scan (256 threads x 16)
+#define SCAN_TILE 4096
 
 // force state flags (device); bit 0 = AVBD_FORCE_DISABLED
 #define FORCE_DEAD 2u
