@@ -287,6 +287,21 @@ def served(world, fractions, duration, budget=200e9):
     return out, st, sat, timings
 
 
+COPY_CALIBRATION_BYTES = 512 * 1024 * 1024
+
+
+def copy_calibration():
+    """One device copy of COPY_CALIBRATION_BYTES (read once, written once)."""
+    import torch
+
+    x = torch.ones(COPY_CALIBRATION_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {COPY_CALIBRATION_BYTES} bytes", file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,7 +334,14 @@ def main():
     ap.add_argument("--snapshot-dir", default=None,
                     help="after the timed steps, save the device world as a snapshot file in this folder and "
                          "load it into a fresh context (reports bytes and both rates; the file is removed)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 512 MiB device to device once and exit: the reference kernel of known traffic "
+                         "(512 MiB read, 512 MiB written) whose FETCH_SIZE / WRITE_SIZE counters scale the "
+                         "counters of the step kernels to HBM bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: only one GPU is supported in this build (asked for --gpus {args.gpus}); "
               "scenes are independent, so a multi-GPU run would shard whole scenes across GPUs", file=sys.stderr)
