@@ -518,7 +518,8 @@ def main():
             "hbm_budget_bytes": int(200e9),
             "resident_bytes": int(sst["resident_bytes"]),
             "resident_scenes": int(sst["resident_scenes"]),
-            "lanes": "pinned 4/4 (result independent of the mix)",
+            "lanes": "forces pinned at 4 lanes; FEM-only particles tuned 4-32 lanes with canonical lane sums "
+                     "(results independent of the mix)",
             "request": "U{1..8} steps of a uniformly drawn scene, Poisson arrivals",
             "saturation": sat,
             "points": points,

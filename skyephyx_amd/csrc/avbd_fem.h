@@ -11,11 +11,14 @@
 //
 // This is the compute-heavy part of the step (one SVD and one 3x3
 // eigensystem per tet visit), and nothing here feeds a discrete decision
-// that must match the CPU bit for bit, so it is built for throughput: FMA
-// contraction is allowed and the Jacobi rotation parameters and
-// normalisations use the hardware reciprocal / reciprocal-sqrt / sqrt
-// (v_rcp_f32, v_rsq_f32, v_sqrt_f32, ~1 ulp) instead of the correctly
-// rounded sequences.
+// that must match the CPU bit for bit, so it is built for throughput: a
+// multiply-add written in one expression may be fused and the Jacobi rotation
+// parameters and normalisations use the hardware reciprocal /
+// reciprocal-sqrt / sqrt (v_rcp_f32, v_rsq_f32, v_sqrt_f32, ~1 ulp) instead of
+// the correctly rounded sequences.  Contraction stays per expression
+// (contract(on), not fast): which operations fuse then follows from the
+// source alone, not from how a kernel around it is scheduled, so every
+// instantiation of the sweep (each lane width) rounds alike.
 #pragma once
 
 #include "avbd_math.h"
@@ -81,7 +84,7 @@ AVBD_DEV void swap_cols(m3& a, int p, int q) {
 
 // Jacobi rotation parameters that zero g_pq of a 2x2 symmetric block.
 AVBD_DEV void jacobi_cs(float app, float aqq, float apq, float& c, float& s) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     float theta = (aqq - app) * frcp(2.f * apq);
     float t = (theta >= 0.f ? 1.f : -1.f) * frcp(fabsf(theta) + fsqrt(theta * theta + 1.f));
     c = frsq(t * t + 1.f);
@@ -90,7 +93,7 @@ AVBD_DEV void jacobi_cs(float app, float aqq, float apq, float& c, float& s) {
 
 // Symmetric 3x3 eigen-decomposition, ascending eigenvalues, vectors in columns.
 AVBD_DEV void sym_eig3(const m3& A, v3& w, m3& V) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     m3 a = A;
     m3 v;
 #pragma unroll
@@ -157,7 +160,7 @@ AVBD_DEV void sym_eig2(float a00, float a01, float a11, float& w0, float& w1, fl
 // F V = U S by one-sided Jacobi; S descending; U, V proper rotations with the
 // sign moved into S(2) (math.cpp:22-32).
 AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     m3 A = F;
     m3 W;
 #pragma unroll
@@ -252,7 +255,7 @@ AVBD_DEV void add_outer(float Hv[6], float l, v3 w) {
 // respect to the vertex whose shape gradient is gradN.
 AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmInv, float V0, float mu, float lam,
                                       v3 gradN, int projMode, v3& g, float H[6]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     m3 Ds;
     v3 e1 = sub(p1, p0), e2 = sub(p2, p0), e3 = sub(p3, p0);
     Ds.m[0][0] = e1.x; Ds.m[1][0] = e1.y; Ds.m[2][0] = e1.z;
@@ -384,7 +387,7 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
 // 2-D shape gradient (gn0, gn1).
 AVBD_DEV void stvk_vertex_terms(v3 p0, v3 p1, v3 p2, const float Dm[4], float area, float mu, float lam, float gn0,
                                 float gn1, int projMode, v3& g, float H[6]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     v3 e0 = sub(p1, p0), e1 = sub(p2, p0);
     v3 f0 = add(mul(e0, Dm[0]), mul(e1, Dm[2]));
     v3 f1 = add(mul(e0, Dm[1]), mul(e1, Dm[3]));

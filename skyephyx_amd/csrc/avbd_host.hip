@@ -170,6 +170,10 @@ struct avbd_ctx {
     size_t cub_bytes = 0;
     int lanes = 0;      // 0 = chosen by the tuner
     int fem_lanes = 0;  // 0 = chosen by the tuner
+    // option fem_canonical_sums: FEM-only buckets sum their lanes in one
+    // fixed tree (k_primal_canon), so the FEM width (4 to 32, tuner class 2)
+    // can change with the launch size without changing results
+    bool fem_canon = false;
     LaneTuner tuner;
     bool have_world = false;
     std::vector<int> bucket_off = std::vector<int>(AVBD_BUCKETS + 1, 0);
@@ -403,9 +407,22 @@ int bucket_grid(const avbd_ctx* c, int n, int G, int block = LaneTuner::kDefault
     return std::min(p, 16384);
 }
 
+// Tuner class of a bucket launch: 0 bodies with forces, 1 FEM-only, 2
+// FEM-only with canonical lane sums (widths 4 to 32).
+int tuner_class(const avbd_ctx* c, bool fem) { return fem ? (c->fem_canon ? 2 : 1) : 0; }
+
 template <bool FEM>
 void launch_primal_g(avbd_ctx* c, int G, int grid, int block, int bucket, int iter, int poststab) {
     const DevWorld& w = c->w;
+    if (FEM && c->fem_canon) {
+        switch (G) {
+            case 4: hipLaunchKernelGGL(k_primal_canon<4>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+            case 8: hipLaunchKernelGGL(k_primal_canon<8>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+            case 16: hipLaunchKernelGGL(k_primal_canon<16>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+            default: hipLaunchKernelGGL(k_primal_canon<32>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+        }
+        return;
+    }
     switch (G) {
         case 1: hipLaunchKernelGGL((k_primal<1, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
         case 2: hipLaunchKernelGGL((k_primal<2, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
@@ -422,7 +439,7 @@ void launch_primal_g(avbd_ctx* c, int G, int grid, int block, int bucket, int it
 int bucket_lanes(const avbd_ctx* c, bool fem, int n, int* block) {
     const int fixed = fem ? c->fem_lanes : c->lanes;
     *block = LaneTuner::kDefaultBlock;
-    return fixed > 0 ? fixed : c->tuner.peek(fem ? 1 : 0, n, block);
+    return fixed > 0 ? fixed : c->tuner.peek(tuner_class(c, fem), n, block);
 }
 
 // One bucket launch on c->stream with the lane width fixed by the user or
@@ -431,7 +448,7 @@ template <bool FEM>
 void launch_bucket(avbd_ctx* c, int bucket, int n, int iter, int poststab) {
     int fixed = FEM ? c->fem_lanes : c->lanes;
     int measure = -1, block = LaneTuner::kDefaultBlock;
-    int G = fixed > 0 ? fixed : c->tuner.choose(FEM ? 1 : 0, n, &measure, &block);
+    int G = fixed > 0 ? fixed : c->tuner.choose(tuner_class(c, FEM), n, &measure, &block);
     const int mark = c->mark_begin(FEM ? avbd_ctx::K_FEM : avbd_ctx::K_FORCES);
     if (measure >= 0) (void)hipEventRecord(c->tuner.pending[measure].a, c->stream);
     launch_primal_g<FEM>(c, G, bucket_grid(c, n, G, block), block, bucket, iter, poststab);
@@ -453,11 +470,22 @@ bool launch_pair(avbd_ctx* c, int colour, int n_gen, int n_fem, int iter, int po
     const int rb = bucket_grid(c, n_gen, gr), fb = bucket_grid(c, n_fem, gf);
     if (rb + fb > kPairBlocks) return false;
     const DevWorld& w = c->w;
+    const dim3 grid(rb + fb), block(256);
+    hipStream_t st = c->stream;
+    if (c->fem_canon) {
+        switch (gf) {
+            case 4: hipLaunchKernelGGL((k_primal_pair<4, 4, true>), grid, block, 0, st, w, colour, iter, poststab, rb); break;
+            case 8: hipLaunchKernelGGL((k_primal_pair<4, 8, true>), grid, block, 0, st, w, colour, iter, poststab, rb); break;
+            case 16: hipLaunchKernelGGL((k_primal_pair<4, 16, true>), grid, block, 0, st, w, colour, iter, poststab, rb); break;
+            default: hipLaunchKernelGGL((k_primal_pair<4, 32, true>), grid, block, 0, st, w, colour, iter, poststab, rb); break;
+        }
+        return true;
+    }
     switch (gf) {
-        case 4: hipLaunchKernelGGL((k_primal_pair<4, 4>), dim3(rb + fb), dim3(256), 0, c->stream, w, colour, iter, poststab, rb); break;
-        case 8: hipLaunchKernelGGL((k_primal_pair<4, 8>), dim3(rb + fb), dim3(256), 0, c->stream, w, colour, iter, poststab, rb); break;
-        case 16: hipLaunchKernelGGL((k_primal_pair<4, 16>), dim3(rb + fb), dim3(256), 0, c->stream, w, colour, iter, poststab, rb); break;
-        default: hipLaunchKernelGGL((k_primal_pair<4, 32>), dim3(rb + fb), dim3(256), 0, c->stream, w, colour, iter, poststab, rb); break;
+        case 4: hipLaunchKernelGGL((k_primal_pair<4, 4, false>), grid, block, 0, st, w, colour, iter, poststab, rb); break;
+        case 8: hipLaunchKernelGGL((k_primal_pair<4, 8, false>), grid, block, 0, st, w, colour, iter, poststab, rb); break;
+        case 16: hipLaunchKernelGGL((k_primal_pair<4, 16, false>), grid, block, 0, st, w, colour, iter, poststab, rb); break;
+        default: hipLaunchKernelGGL((k_primal_pair<4, 32, false>), grid, block, 0, st, w, colour, iter, poststab, rb); break;
     }
     return true;
 }
@@ -1405,7 +1433,8 @@ int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_
 // grid).  False while a bucket's lane width is still being tuned.
 bool step_graph_key(const avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_post, int dgrid,
                     std::vector<int>* key) {
-    key->assign({c->cur, ncol, max_iters, any_post ? 1 : 0, dgrid, fused_lanes(c), (int)c->fusable_ids.size()});
+    key->assign({c->cur, ncol, max_iters, any_post ? 1 : 0, dgrid, fused_lanes(c), (int)c->fusable_ids.size(),
+                 c->fem_canon ? 1 : 0});
     for (int k = 0; k < 2 * ncol; ++k) {
         const int n = off[k + 1] - off[k];
         if (n == 0) {
@@ -1695,7 +1724,15 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
     if (n == "lanes" || n == "fem_lanes") {
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
             return fail(AVBD_E_ARG, n + " must be 0 (tuned), 1, 2, 4, 8, 16 or 32");
+        if (n == "fem_lanes" && c->fem_canon && value > 0 && value < 4)
+            return fail(AVBD_E_ARG, "fem_lanes must be 0 or 4 to 32 with fem_canonical_sums");
         (n == "lanes" ? c->lanes : c->fem_lanes) = int(value);
+        return 0;
+    }
+    if (n == "fem_canonical_sums") {
+        if (value != 0 && c->fem_lanes > 0 && c->fem_lanes < 4)
+            return fail(AVBD_E_ARG, "fem_canonical_sums needs fem_lanes 0 or 4 to 32");
+        c->fem_canon = value != 0;
         return 0;
     }
     if (n == "kernel_timing") {

@@ -26,8 +26,10 @@ __global__ void k_colour_fill(DevWorld w);
 __global__ void k_predict(DevWorld w);
 template <int G, bool FEM>
 __global__ void k_primal(DevWorld w, int bucket, int iter, int poststab);
-template <int GR, int GF>
+template <int GR, int GF, bool CANON>
 __global__ void k_primal_pair(DevWorld w, int colour, int iter, int poststab, int rigid_blocks);
+template <int G>
+__global__ void k_primal_canon(DevWorld w, int bucket, int iter, int poststab);
 template <int G>
 __global__ void k_scene_step(DevWorld w, const int* scenes, int mode);
 __global__ void k_dual(DevWorld w, int iter);

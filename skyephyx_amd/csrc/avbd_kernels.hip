@@ -1413,6 +1413,24 @@ AVBD_DEV void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The block solve of a FEM-only particle and its writes (the system is
+// blockdiag(A, 0): the rotation increment is exactly zero).
+AVBD_DEV void fem_vertex_solve(const DevWorld& w, int b, v3 p, float L[6], float r[3]) {
+    float dx[3];
+    ldlt3_solve(L, r, dx);
+    const qt q = fq(w.rot[b]);
+    const qt lr = fq(w.last_rot[b]);
+    v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
+    qt dq = quaternion_from_difference(v3{0.f, 0.f, 0.f}, -1.f);
+    qt nq = qnormalized(qmul(dq, q));
+    w.pos[b] = v4(np, 0.f);
+    w.rot[b] = q4(nq);
+    v3 dl = sub(np, f3(w.last_pos[b]));
+    v3 da = rotation_difference(nq, lr);
+    w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
+    w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
+}
+
 template <int G>
 AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, int poststab, float4* nbp = nullptr) {
     const int s = w.scene[b];
@@ -1530,20 +1548,144 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
             for (int i = 0; i < 3; ++i) r[i] = r[i] + __shfl_xor(r[i], off, G);
         }
     }
-    if (lane == 0) {
-        float dx[3];
-        ldlt3_solve(L, r, dx);
-        const qt q = fq(w.rot[b]);
-        const qt lr = fq(w.last_rot[b]);
-        v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
-        qt dq = quaternion_from_difference(v3{0.f, 0.f, 0.f}, -1.f);
-        qt nq = qnormalized(qmul(dq, q));
-        w.pos[b] = v4(np, 0.f);
-        w.rot[b] = q4(nq);
-        v3 dl = sub(np, f3(w.last_pos[b]));
-        v3 da = rotation_difference(nq, lr);
-        w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
-        w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
+    if (lane == 0) fem_vertex_solve(w, b, p, L, r);
+}
+
+// Canonical lane sums: the FEM-only update with a result that does not
+// depend on G.  The entries are shared out over 32 virtual lanes (virtual
+// lane v takes entries e0 + v, + 32, ... in order, lane 0 starting from the
+// inertia term) and the 32 partial systems are summed by one fixed tree: a
+// butterfly with offsets 16, 8, 4, 2, 1.  A real lane of a G-lane group
+// holds the virtual lanes lane + G t (t < T = 32 / G); the tree's first
+// log2 T levels pair virtual lanes of the same real lane, so the lane sums
+// them itself (leaves in bit-reversed order of t, partial sums kept as in
+// pairwise summation), and the last log2 G levels are the shuffles.  Every
+// G in {4, 8, 16, 32} performs the same additions in the same tree, so the
+// launch width can follow the launch size without changing any result.
+template <int T>
+AVBD_DEV int bitrev_leaf(int i) {
+    if constexpr (T == 8) return ((i & 1) << 2) | (i & 2) | ((i >> 2) & 1);
+    else if constexpr (T == 4) return ((i & 1) << 1) | ((i >> 1) & 1);
+    else return i;
+}
+
+struct FemSum {
+    float L[6];
+    float r[3];
+};
+AVBD_DEV void fem_sum_zero(FemSum& s) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) s.L[i] = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s.r[i] = 0.f;
+}
+AVBD_DEV void fem_sum_add(FemSum& acc, const FemSum& x) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) acc.L[i] = x.L[i] + acc.L[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) acc.r[i] = x.r[i] + acc.r[i];
+}
+
+template <int G>
+AVBD_DEV void primal_fem_vertex_canon(const DevWorld& w, int b, int lane, int iter, int poststab) {
+    static_assert(G >= 4 && G <= 32, "canonical sums need 4 to 32 lanes");
+    constexpr int T = 32 / G;
+    const int s = w.scene[b];
+    const avbd_scene& sc = w.sc[s];
+    if (w.st[s * ST_WORDS + ST_EMERGENCY]) return;
+    if (poststab ? (!sc.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) : iter >= sc.iterations) return;
+    const float dt2 = sc.dt * sc.dt;
+    const v3 p = f3(w.pos[b]);
+    const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
+    const int last = max(e1 - 1, 0);
+    // leaf ti's first entry is e0 + lane + G bitrev(ti); it is loaded one
+    // leaf ahead (clamped, unconditionally, as in primal_fem_vertex)
+    int k = e0 + lane;
+    ElemIn in0;
+    elem_load(w, min(k, last), in0);
+    ElemPos p0;
+    elem_gather(w, in0.v, p0);
+    FemSum cur, lv0, lv1, lv2;
+    fem_sum_zero(lv0);
+    fem_sum_zero(lv1);
+    fem_sum_zero(lv2);
+#pragma unroll 1
+    for (int ti = 0; ti < T; ++ti) {
+        const int kn = e0 + lane + G * bitrev_leaf<T>(min(ti + 1, T - 1));
+        ElemIn in1;
+        elem_load(w, min(kn, last), in1);
+        ElemPos p1;
+        elem_gather(w, in1.v, p1);
+        fem_sum_zero(cur);
+        if (lane + G * bitrev_leaf<T>(ti) == 0) {
+            const float ml = w.vel[b].w / dt2;
+            const v3 dl = sub(p, f3(w.inert_pos[b]));
+            cur.L[0] = ml; cur.L[2] = ml; cur.L[5] = ml;
+            cur.r[0] = ml * dl.x; cur.r[1] = ml * dl.y; cur.r[2] = ml * dl.z;
+        }
+        if (k < e1) {
+            v3 g;
+            float H[6];
+            elem_terms(in0, p0, sc.projection_mode, g, H);
+            cur.r[0] = cur.r[0] + g.x;
+            cur.r[1] = cur.r[1] + g.y;
+            cur.r[2] = cur.r[2] + g.z;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) cur.L[i] = cur.L[i] + H[i];
+            // more than 32 entries: the leaf's later ones, in order
+            for (int k2 = k + 32; k2 < e1; k2 += 32) {
+                ElemIn x;
+                elem_load(w, k2, x);
+                ElemPos q;
+                elem_gather(w, x.v, q);
+                elem_terms(x, q, sc.projection_mode, g, H);
+                cur.r[0] = cur.r[0] + g.x;
+                cur.r[1] = cur.r[1] + g.y;
+                cur.r[2] = cur.r[2] + g.z;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) cur.L[i] = cur.L[i] + H[i];
+            }
+        }
+        // pairwise summation of the leaves: level j holds the sum of the
+        // last 2^j leaves until its partner subtree completes
+        bool done = true;
+        if constexpr (T >= 2) {
+            if (!(ti & 1)) { lv0 = cur; done = false; }
+            else fem_sum_add(cur, lv0);
+        }
+        if constexpr (T >= 4) {
+            if (done && !(ti & 2)) { lv1 = cur; done = false; }
+            else if (done) fem_sum_add(cur, lv1);
+        }
+        if constexpr (T >= 8) {
+            if (done && !(ti & 4)) { lv2 = cur; done = false; }
+            else if (done) fem_sum_add(cur, lv2);
+        }
+        in0 = in1;
+        p0 = p1;
+        k = kn;
+    }
+    // cur is the lane's subtree (the loop ends on ti = T - 1, all bits set)
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) cur.L[i] = cur.L[i] + __shfl_xor(cur.L[i], off, G);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) cur.r[i] = cur.r[i] + __shfl_xor(cur.r[i], off, G);
+    }
+    if (lane == 0) fem_vertex_solve(w, b, p, cur.L, cur.r);
+}
+
+// Blocks bid of nblocks stride over the bucket [lo, hi) (canonical lane
+// sums: primal_fem_vertex_canon).
+template <int G>
+AVBD_DEV void primal_fem_body_canon(const DevWorld& w, int lo, int hi, int iter, int poststab, int bid, int nblocks) {
+    const int lane = threadIdx.x % G;
+    const int gpb = blockDim.x / G;
+    for (int base = bid * gpb; base < hi - lo; base += nblocks * gpb) {
+        const int item = base + threadIdx.x / G;
+        if (item >= hi - lo) continue;
+        primal_fem_vertex_canon<G>(w, w.col_list[lo + item], lane, iter, poststab);
     }
 }
 
@@ -1673,19 +1815,32 @@ __global__ __launch_bounds__(256) void k_primal(DevWorld w, int bucket, int iter
 // lanes).  The two sets of bodies are disjoint and neither reads what the
 // other writes in this colour, as with the two concurrent launches it
 // replaces; for small buckets it saves the fork and join between streams.
-template <int GR, int GF>
+template <int GR, int GF, bool CANON>
 __global__ __launch_bounds__(256) void k_primal_pair(DevWorld w, int colour, int iter, int poststab, int rigid_blocks) {
     const int bid = blockIdx.x;
+    const int lo = w.col_off[2 * colour + 1], hi = w.col_off[2 * colour + 2], nb = (int)gridDim.x - rigid_blocks;
     if (bid < rigid_blocks)
         primal_rigid_body<GR>(w, w.col_off[2 * colour], w.col_off[2 * colour + 1], iter, poststab, bid, rigid_blocks);
+    else if constexpr (CANON)
+        primal_fem_body_canon<GF>(w, lo, hi, iter, poststab, bid - rigid_blocks, nb);
     else
-        primal_fem_body<GF>(w, w.col_off[2 * colour + 1], w.col_off[2 * colour + 2], iter, poststab, bid - rigid_blocks,
-                            (int)gridDim.x - rigid_blocks);
+        primal_fem_body<GF>(w, lo, hi, iter, poststab, bid - rigid_blocks, nb);
 }
-template __global__ void k_primal_pair<4, 4>(DevWorld, int, int, int, int);
-template __global__ void k_primal_pair<4, 8>(DevWorld, int, int, int, int);
-template __global__ void k_primal_pair<4, 16>(DevWorld, int, int, int, int);
-template __global__ void k_primal_pair<4, 32>(DevWorld, int, int, int, int);
+
+// FEM-only bucket with canonical lane sums (result independent of G).
+template <int G>
+__global__ __launch_bounds__(256) void k_primal_canon(DevWorld w, int bucket, int iter, int poststab) {
+    primal_fem_body_canon<G>(w, w.col_off[bucket], w.col_off[bucket + 1], iter, poststab, blockIdx.x, gridDim.x);
+}
+
+#define AVBD_PAIR_INST(GF)                                                              \
+    template __global__ void k_primal_pair<4, GF, false>(DevWorld, int, int, int, int); \
+    template __global__ void k_primal_pair<4, GF, true>(DevWorld, int, int, int, int);  \
+    template __global__ void k_primal_canon<GF>(DevWorld, int, int, int);
+AVBD_PAIR_INST(4)
+AVBD_PAIR_INST(8)
+AVBD_PAIR_INST(16)
+AVBD_PAIR_INST(32)
 
 #define AVBD_PRIMAL_INST(G)                                                        \
     template __global__ void k_primal<G, false>(DevWorld, int, int, int); \

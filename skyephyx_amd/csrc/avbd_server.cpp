@@ -338,10 +338,13 @@ int avbd_server_create(int32_t device, int64_t hbm_budget_bytes, int32_t max_slo
         delete s;
         return rc;
     }
-    // Fixed lane widths: a body's accumulation order then does not depend on
-    // the size of the launch it lands in, so a scene's result is the same
-    // whichever scenes share its ticks.
-    if ((rc = avbd_set_option(s->ctx, "lanes", 4)) || (rc = avbd_set_option(s->ctx, "fem_lanes", 4))) {
+    // A body's accumulation order must not depend on the size of the launch
+    // it lands in, so that a scene's result is the same whichever scenes
+    // share its ticks: bodies with forces at a fixed 4 lanes, FEM-only
+    // particles with canonical lane sums (the same tree at every width), so
+    // their width is tuned to each tick's launch sizes.
+    if ((rc = avbd_set_option(s->ctx, "lanes", 4)) || (rc = avbd_set_option(s->ctx, "fem_canonical_sums", 1)) ||
+        (rc = avbd_set_option(s->ctx, "fem_lanes", 0))) {
         avbd_server_destroy(s);
         return rc;
     }

@@ -10,7 +10,10 @@
 // a body's arithmetic (a lane group is always inside one wave), so the
 // choice never changes results.
 //
-// The tuner keys on (kernel class, log2 of the bodies in the launch).  A key
+// The tuner keys on (kernel class, log2 of the bodies in the launch).  Class
+// 2 (FEM-only buckets with canonical lane sums) samples only the widths that
+// kernel has, 4 to 32; there the width is free to change with the launch
+// size, because results do not depend on it.  A key
 // first samples every candidate width at 256 threads, a few times each, on
 // live launches (timed with HIP events on the stream that runs the kernel),
 // keeps the fastest, then samples the three workgroup sizes at that width
@@ -112,11 +115,15 @@ struct LaneTuner {
         return kCands[ci];
     }
 
+    // Widths a class can take: every candidate, 4 to 32 for class 2.
+    static unsigned class_mask(int cls) { return cls == 2 ? 0x3Cu : (1u << kNumCands) - 1; }
+
     // Candidates for a key first seen: next to a tuned key of the same class
     // (launches within a factor of two in size) only that key's width and
     // its half and double, so a bucket that crosses a power of two while the
     // world runs costs a few sampled launches, not a sweep of every width.
     unsigned seed_mask(int k) const {
+        const unsigned all = class_mask(k / 64);
         for (int d : {-1, 1}) {
             auto it = table.find(k + d);
             if (it == table.end() || !it->second.best || (k + d) / 64 != k / 64) continue;
@@ -124,9 +131,10 @@ struct LaneTuner {
             unsigned m = 0;
             for (int c = 0; c < kNumCands; ++c)
                 if (kCands[c] == b || 2 * kCands[c] == b || kCands[c] == 2 * b) m |= 1u << c;
-            return m;
+            m &= all;
+            return m ? m : all;
         }
-        return (1u << kNumCands) - 1;
+        return all;
     }
 
     // The width choose() would return without sampling (and its workgroup
@@ -219,7 +227,7 @@ struct LaneTuner {
                 i = v0 + 1;
             }
             bool ok = false, okb = false;
-            for (int q = 0; q < kNumCands; ++q) ok = ok || kCands[q] == v;
+            for (int q = 0; q < kNumCands; ++q) ok = ok || (kCands[q] == v && ((class_mask(k / 64) >> q) & 1u));
             for (int q = 0; q < kNumBlocks; ++q) okb = okb || kBlocks[q] == blk;
             if (ok && okb) {
                 table[k].best = v;

@@ -182,7 +182,9 @@ def test_served_mix_4096_scenes_alone_equals_mixed(mix, capsys):
     1-4 steps for every scene arriving in two waves (continuous batching:
     the second wave joins while the first runs); sampled scenes after their
     requests are bit-identical to the scene stepped alone for the same
-    number of steps (the server's pinned lane widths, 4 and 4)."""
+    number of steps (the server's accumulation rules: 4 lanes per body with
+    forces, canonical FEM sums; the scene alone runs its FEM particles at 4
+    lanes, the server at whatever width its tuner picks per launch)."""
     say = _Progress(capsys, "served4096")
     assert footprint(mix) <= 200e9
     rng = np.random.default_rng(11)
@@ -207,7 +209,9 @@ def test_served_mix_4096_scenes_alone_equals_mixed(mix, capsys):
         say("served every request; sampled scenes downloaded")
     for s in sample:
         one = mix.extract_scene(s)
-        with Context(lanes=4, fem_lanes=4) as ctx:
+        with Context(lanes=4) as ctx:
+            ctx.set_option("fem_canonical_sums", 1)
+            ctx.set_option("fem_lanes", 4)
             ctx.upload(one)
             ctx.step(int(steps[s]))
             ref = ctx.download()

@@ -43,7 +43,12 @@ def small_mix():
 
 
 def alone(world, steps, lanes):
-    with Context(lanes=lanes[0], fem_lanes=lanes[1]) as ctx:
+    """The scene stepped on its own, with the server's accumulation rules:
+    lanes[0] lanes per body with forces, FEM-only particles with canonical
+    lane sums at lanes[1] lanes (0: tuned; the width does not change them)."""
+    with Context(lanes=lanes[0]) as ctx:
+        ctx.set_option("fem_canonical_sums", 1)
+        ctx.set_option("fem_lanes", lanes[1])
         ctx.upload(world)
         ctx.step(steps)
         return ctx.download()
@@ -136,11 +141,12 @@ def test_eviction_round_trip_bit_exact():
 
 
 def test_alone_close_to_mixed_autotuned():
-    """Autotuned lane widths follow the launch size, so the accumulation
-    order of a body differs between the mix and the scene alone: one step
-    from the same state agrees to the single-step parity bar.  (Over many
-    steps, contact stick/slip decisions in stacks amplify rounding, which is
-    why the server pins the widths by default.)"""
+    """An autotuned width for bodies with forces follows the launch size, so
+    their accumulation order differs between the mix and the scene alone:
+    one step from the same state agrees to the single-step parity bar.
+    (Over many steps, contact stick/slip decisions in stacks amplify
+    rounding, which is why the server pins that width by default; FEM-only
+    particles sum canonically at any width.)"""
     ws = small_mix()
     served, _ = _serve(ws, [1] * len(ws), 200e9, (0, 0), stagger=False, second=False)
     for k, (w, g) in enumerate(zip(ws, served)):
@@ -149,13 +155,15 @@ def test_alone_close_to_mixed_autotuned():
 
 
 def test_default_server_is_bit_exact_over_many_steps():
-    """The server's default (pinned) lane widths keep even the contact-rich
-    pyramid bit-identical to the scene alone over many steps."""
+    """The server's defaults (4 lanes per body with forces, canonical FEM
+    sums at a tuned width) keep even the contact-rich pyramid bit-identical
+    to the scene alone over many steps, the scene alone running its FEM
+    particles at another width (32 lanes)."""
     ws = small_mix()
     steps = [12] * len(ws)
     served, _ = _serve(ws, steps, 200e9, (None, None), second=False)
     for k, (w, g) in enumerate(zip(ws, served)):
-        assert_bit_equal(g, alone(w, 12, (4, 4)), f"scene {k}")
+        assert_bit_equal(g, alone(w, 12, (4, 32)), f"scene {k}")
 
 
 def test_poisson_requests_all_complete():
