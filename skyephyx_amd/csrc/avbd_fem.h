@@ -2,18 +2,21 @@
 // (neohookeanfem.cpp:40-110, math.cpp:13-230) and StVK membrane triangles
 // (stvkfem.cpp:38-86, math.cpp:237-356).
 //
-// The SVD is one-sided (Hestenes) Jacobi on F; the small symmetric
-// eigenproblems are cyclic Jacobi.  Any SVD with the reference's sign fix
-// yields the same projected vertex Hessian (it is invariant to the choice of
-// singular vectors), so these are pinned to tolerance, not to the bit.
-// Everything stays in registers: loops over matrix indices are unrolled and
-// run-time permutations are applied as predicated compile-time swaps.
+// The projected vertex Hessian is the unprojected closed form plus one
+// rank-one correction per Hessian mode whose eigenvalue falls below the
+// projection floor (analytic eigensystem of Smith et al., "Analytic
+// Eigensystems for Isotropic Distortion Energies", 2019).  Those corrections
+// need the singular vectors of F, taken from a closed-form eigen-decomposition
+// of F^T F (no iteration); the result does not depend on the choice of
+// singular vectors, so it is pinned to tolerance, not to the bit.
+// Everything stays in registers: loops over matrix indices are unrolled.
 //
-// This is the compute-heavy part of the step (one SVD and one 3x3
-// eigensystem per tet visit), and nothing here feeds a discrete decision
-// that must match the CPU bit for bit, so it is built for throughput: a
-// multiply-add written in one expression may be fused and the Jacobi rotation
-// parameters and normalisations use the hardware reciprocal /
+// This is the compute-heavy part of the step (per tet visit the
+// deformation gradient, its cofactor and, off the positive-definite case, a
+// 3x3 eigensystem), and nothing here feeds a discrete decision that must
+// match the CPU bit for bit, so it is built for throughput: a multiply-add
+// written in one expression may be fused and the rotation parameters and
+// normalisations use the hardware reciprocal /
 // reciprocal-sqrt / sqrt (v_rcp_f32, v_rsq_f32, v_sqrt_f32, ~1 ulp) instead of
 // the correctly rounded sequences.  Contraction stays per expression
 // (contract(on), not fast): which operations fuse then follows from the
@@ -34,14 +37,14 @@ AVBD_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // the contact code).
 namespace fc {
 AVBD_DEV float dot(v3 a, v3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
-AVBD_DEV float norm(v3 a) { return fsqrt(dot(a, a)); }
+AVBD_DEV float norm(v3 a) { return fsqrt(fc::dot(a, a)); }
 AVBD_DEV v3 lin2(float a, v3 u, float b, v3 v) {
     return v3{fmaf(b, v.x, a * u.x), fmaf(b, v.y, a * u.y), fmaf(b, v.z, a * u.z)};
 }
 AVBD_DEV v3 lin3(float a, v3 u, float b, v3 v, float c, v3 w) {
     return v3{fmaf(c, w.x, fmaf(b, v.x, a * u.x)), fmaf(c, w.y, fmaf(b, v.y, a * u.y)), fmaf(c, w.z, fmaf(b, v.z, a * u.z))};
 }
-AVBD_DEV v3 mv(const m3& a, v3 v) { return v3{dot(row(a, 0), v), dot(row(a, 1), v), dot(row(a, 2), v)}; }
+AVBD_DEV v3 mv(const m3& a, v3 v) { return v3{fc::dot(row(a, 0), v), fc::dot(row(a, 1), v), fc::dot(row(a, 2), v)}; }
 AVBD_DEV m3 mm(const m3& a, const m3& b) {
     m3 r;
 #pragma unroll
@@ -69,18 +72,9 @@ AVBD_DEV m3 cofactor(const m3& F) {
 AVBD_DEV float det_cof(const m3& F, const m3& cof) {
     return fmaf(F.m[0][2], cof.m[0][2], fmaf(F.m[0][1], cof.m[0][1], F.m[0][0] * cof.m[0][0]));
 }
-AVBD_DEV float det3(const m3& a) { return det_cof(a, cofactor(a)); }
+AVBD_DEV float det3(const m3& a) { return fc::det_cof(a, fc::cofactor(a)); }
 AVBD_DEV v3 cross(v3 a, v3 b) { return v3{det2(a.y, a.z, b.y, b.z), det2(a.z, a.x, b.z, b.x), det2(a.x, a.y, b.x, b.y)}; }
 }  // namespace fc
-
-AVBD_DEV void swap_cols(m3& a, int p, int q) {
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        float t = a.m[r][p];
-        a.m[r][p] = a.m[r][q];
-        a.m[r][q] = t;
-    }
-}
 
 // Jacobi rotation parameters that zero g_pq of a 2x2 symmetric block.
 AVBD_DEV void jacobi_cs(float app, float aqq, float apq, float& c, float& s) {
@@ -91,55 +85,81 @@ AVBD_DEV void jacobi_cs(float app, float aqq, float apq, float& c, float& s) {
     s = t * c;
 }
 
-// Symmetric 3x3 eigen-decomposition, ascending eigenvalues, vectors in columns.
-AVBD_DEV void sym_eig3(const m3& A, v3& w, m3& V) {
+// Eigenpairs of the symmetric 3x3 [a00 a01 a02; a01 a11 a12; a02 a12 a22]
+// after D. Eberly, "A Robust Eigensolver for 3x3 Symmetric Matrices"
+// (Geometric Tools, 2014).  The eigenvalue farther from the other two comes
+// from the trigonometric solution of the characteristic cubic (the largest
+// when det(A - qI) >= 0, else the smallest); its vector is the longest cross
+// product of two rows of A - e I, and the value is then refined to the
+// vector's Rayleigh quotient.  The other pair diagonalises the 2x2
+// restriction of A to the orthogonal complement with one Jacobi rotation.
+// No iteration and no sorting: the pairs come in no particular order and
+// the vectors are orthonormal to rounding, also for (near-)equal eigenvalues.
+struct SymEig3 {
+    float e[3];
+    v3 v[3];
+};
+AVBD_DEV v3 sym_mv(float a00, float a01, float a02, float a11, float a12, float a22, v3 x) {
+    return v3{fmaf(a02, x.z, fmaf(a01, x.y, a00 * x.x)), fmaf(a12, x.z, fmaf(a11, x.y, a01 * x.x)),
+              fmaf(a22, x.z, fmaf(a12, x.y, a02 * x.x))};
+}
+AVBD_DEV void sym_eig3_closed(float a00, float a01, float a02, float a11, float a12, float a22, SymEig3& r) {
 #pragma clang fp contract(on)
-    m3 a = A;
-    m3 v;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) v.m[i][j] = (i == j) ? 1.f : 0.f;
-    for (int sweep = 0; sweep < 16; ++sweep) {
-        float off = (fabsf(a.m[0][1]) + fabsf(a.m[0][2])) + fabsf(a.m[1][2]);
-        float diag = (fabsf(a.m[0][0]) + fabsf(a.m[1][1])) + fabsf(a.m[2][2]);
-        if (off <= AVBD_FLT_EPS * 0.25f * diag || off == 0.f) break;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int p = (k == 2) ? 1 : 0;
-            const int q = (k == 0) ? 1 : 2;
-            if (a.m[p][q] == 0.f) continue;
-            float c, s;
-            jacobi_cs(a.m[p][p], a.m[q][q], a.m[p][q], c, s);
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                float arp = a.m[r][p], arq = a.m[r][q];
-                a.m[r][p] = c * arp - s * arq;
-                a.m[r][q] = s * arp + c * arq;
-            }
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                float apr = a.m[p][r], aqr = a.m[q][r];
-                a.m[p][r] = c * apr - s * aqr;
-                a.m[q][r] = s * apr + c * aqr;
-            }
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                float vrp = v.m[r][p], vrq = v.m[r][q];
-                v.m[r][p] = c * vrp - s * vrq;
-                v.m[r][q] = s * vrp + c * vrq;
-            }
-        }
+    const float q = (a00 + a11 + a22) * (1.f / 3.f);
+    const float b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+    const float off = a01 * a01 + a02 * a02 + a12 * a12;
+    const float p2 = (b00 * b00 + b11 * b11 + b22 * b22 + 2.f * off) * (1.f / 6.f);
+    const float p = fsqrt(p2);
+    // det(B / p) / 2 with B = A - q I, formed from B / p (entries of order
+    // one whatever the spread p); a scalar matrix (p = 0) takes 0
+    const float ip = p > 1e-30f ? frcp(p) : 0.f;
+    const float n00 = b00 * ip, n11 = b11 * ip, n22 = b22 * ip, n01 = a01 * ip, n02 = a02 * ip, n12 = a12 * ip;
+    const float c00 = n11 * n22 - n12 * n12, c01 = n01 * n22 - n12 * n02, c02 = n01 * n12 - n11 * n02;
+    float hd = 0.5f * (n00 * c00 - n01 * c01 + n02 * c02);
+    hd = fminf(fmaxf(hd, -1.f), 1.f);
+    const float ang = acosf(hd) * (1.f / 3.f);
+    float es = q + 2.f * p * __cosf(hd >= 0.f ? ang : ang + 2.09439510f);
+    // rows of A - es I scaled by their largest entry: for a nearly scalar A
+    // they are tiny and their cross products' squared norms would fall
+    // below the normal range, where the hardware reciprocal square root
+    // reads them as zero
+    const float d0 = a00 - es, d1 = a11 - es, d2 = a22 - es;
+    const float mr = fmaxf(fmaxf(fmaxf(fabsf(d0), fabsf(d1)), fmaxf(fabsf(d2), fabsf(a01))), fmaxf(fabsf(a02), fabsf(a12)));
+    const float im = mr > 1e-30f ? frcp(mr) : 0.f;
+    const v3 r0{d0 * im, a01 * im, a02 * im}, r1{a01 * im, d1 * im, a12 * im}, r2{a02 * im, a12 * im, d2 * im};
+    v3 x = fc::cross(r0, r1);
+    float dx = fc::dot(x, x);
+    const v3 x02 = fc::cross(r0, r2), x12 = fc::cross(r1, r2);
+    const float d02 = fc::dot(x02, x02), d12 = fc::dot(x12, x12);
+    if (d02 > dx) { x = x02; dx = d02; }
+    if (d12 > dx) { x = x12; dx = d12; }
+    const v3 vs = dx > 1e-30f ? smul(frsq(dx), x) : v3{1.f, 0.f, 0.f};
+    es = fc::dot(vs, sym_mv(a00, a01, a02, a11, a12, a22, vs));
+    // orthonormal basis (U, W) of the plane orthogonal to vs
+    v3 U;
+    if (fabsf(vs.x) > fabsf(vs.y)) {
+        const float il = frsq(fmaf(vs.x, vs.x, vs.z * vs.z));
+        U = v3{-vs.z * il, 0.f, vs.x * il};
+    } else {
+        const float il = frsq(fmaf(vs.y, vs.y, vs.z * vs.z));
+        U = v3{0.f, vs.z * il, -vs.y * il};
     }
-    float d0 = a.m[0][0], d1 = a.m[1][1], d2 = a.m[2][2];
-    // stable ascending sort of (d0,d1,d2) with their columns
-    if (d1 < d0) { float t = d0; d0 = d1; d1 = t; swap_cols(v, 0, 1); }
-    if (d2 < d1) {
-        float t = d1; d1 = d2; d2 = t; swap_cols(v, 1, 2);
-        if (d1 < d0) { t = d0; d0 = d1; d1 = t; swap_cols(v, 0, 1); }
+    const v3 W = fc::cross(vs, U);
+    const v3 AU = sym_mv(a00, a01, a02, a11, a12, a22, U), AW = sym_mv(a00, a01, a02, a11, a12, a22, W);
+    const float m00 = fc::dot(U, AU), m01 = fc::dot(U, AW), m11 = fc::dot(W, AW);
+    // one Jacobi rotation zeroes m01: eigenvalues m00 - t m01, m11 + t m01
+    float t = 0.f;
+    if (m01 != 0.f) {
+        const float theta = (m11 - m00) * frcp(2.f * m01);
+        t = (theta >= 0.f ? 1.f : -1.f) * frcp(fabsf(theta) + fsqrt(fmaf(theta, theta, 1.f)));
     }
-    w = v3{d0, d1, d2};
-    V = v;
+    const float c = frsq(fmaf(t, t, 1.f)), s = t * c;
+    r.e[0] = es;
+    r.v[0] = vs;
+    r.e[1] = fmaf(-t, m01, m00);
+    r.v[1] = fc::lin2(c, U, -s, W);
+    r.e[2] = fmaf(t, m01, m11);
+    r.v[2] = fc::lin2(s, U, c, W);
 }
 
 // Symmetric 2x2: ascending eigenvalues w0 <= w1, vectors (c0, c1) in columns.
@@ -157,78 +177,6 @@ AVBD_DEV void sym_eig2(float a00, float a01, float a11, float& w0, float& w1, fl
     }
 }
 
-// F V = U S by one-sided Jacobi; S descending; U, V proper rotations with the
-// sign moved into S(2) (math.cpp:22-32).
-AVBD_DEV void svd3(const m3& F, m3& U, v3& S, m3& V) {
-#pragma clang fp contract(on)
-    m3 A = F;
-    m3 W;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) W.m[i][j] = (i == j) ? 1.f : 0.f;
-    for (int sweep = 0; sweep < 30; ++sweep) {
-        bool rotated = false;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int p = (k == 2) ? 1 : 0;
-            const int q = (k == 0) ? 1 : 2;
-            v3 ap = col(A, p), aq = col(A, q);
-            float alpha = fc::dot(ap, ap), beta = fc::dot(aq, aq), gamma = fc::dot(ap, aq);
-            if (gamma * gamma <= (4.f * AVBD_FLT_EPS * AVBD_FLT_EPS) * (alpha * beta) || gamma == 0.f) continue;
-            rotated = true;
-            float zeta = (beta - alpha) * frcp(2.f * gamma);
-            float t = (zeta >= 0.f ? 1.f : -1.f) * frcp(fabsf(zeta) + fsqrt(1.f + zeta * zeta));
-            float c = frsq(1.f + t * t);
-            float s = c * t;
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                float x = A.m[r][p], y = A.m[r][q];
-                A.m[r][p] = c * x - s * y;
-                A.m[r][q] = s * x + c * y;
-                float wx = W.m[r][p], wy = W.m[r][q];
-                W.m[r][p] = c * wx - s * wy;
-                W.m[r][q] = s * wx + c * wy;
-            }
-        }
-        if (!rotated) break;
-    }
-    float s0 = fc::norm(col(A, 0)), s1 = fc::norm(col(A, 1)), s2 = fc::norm(col(A, 2));
-    // stable descending sort
-    if (s1 > s0) { float t = s0; s0 = s1; s1 = t; swap_cols(A, 0, 1); swap_cols(W, 0, 1); }
-    if (s2 > s1) {
-        float t = s1; s1 = s2; s2 = t; swap_cols(A, 1, 2); swap_cols(W, 1, 2);
-        if (s1 > s0) { t = s0; s0 = s1; s1 = t; swap_cols(A, 0, 1); swap_cols(W, 0, 1); }
-    }
-    const float tiny = 1.17549435e-38f * 1e6f;
-    v3 u0 = col(A, 0), u1 = col(A, 1), u2 = col(A, 2);
-    if (s0 > tiny) u0 = mul(u0, frcp(s0)); else u0 = v3{1.f, 0.f, 0.f};
-    if (s1 > tiny) u1 = mul(u1, frcp(s1));
-    else {
-        v3 e = fabsf(u0.x) < 0.9f ? v3{1.f, 0.f, 0.f} : v3{0.f, 1.f, 0.f};
-        u1 = fc::cross(u0, e);
-        u1 = mul(u1, frsq(fc::dot(u1, u1)));
-    }
-    if (s2 > tiny) u2 = mul(u2, frcp(s2)); else u2 = fc::cross(u0, u1);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        U.m[r][0] = comp(u0, r);
-        U.m[r][1] = comp(u1, r);
-        U.m[r][2] = comp(u2, r);
-    }
-    V = W;
-    if (fc::det3(U) < 0.f) {
-#pragma unroll
-        for (int r = 0; r < 3; ++r) U.m[r][2] = -U.m[r][2];
-        s2 = -s2;
-    }
-    if (fc::det3(V) < 0.f) {
-#pragma unroll
-        for (int r = 0; r < 3; ++r) V.m[r][2] = -V.m[r][2];
-        s2 = -s2;
-    }
-    S = v3{s0, s1, s2};
-}
 
 // ProjectEigenvalues (math.cpp:137-171) with rho = 0 (the reference never
 // updates it, solver.cpp:451-455) and threshold 0.01, epsilon 1e-6.
@@ -289,98 +237,114 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
     // mu |gradN|^2 I + c1 (cof gradN)(cof gradN)^T, c1 = diagExtra / J^2:
     // the d2J term vanishes on a single vertex (dF = dx gradN^T is rank one
     // and det(F + t dx gradN^T) is affine in t).
-    if (mu > 1e-6f && dPhi <= 0.f) {
-        bool pd = dPhi == 0.f;
+    const float dg = mu * fc::dot(gradN, gradN);
+    const float c1 = diagExtra * invJ * invJ;
+    const v3 cc = smul(c1, cg);
+    float Hv[6] = {fmaf(cc.x, cg.x, dg), cc.y * cg.x, fmaf(cc.y, cg.y, dg), cc.z * cg.x, cc.z * cg.y, fmaf(cc.z, cg.z, dg)};
+    const v3 f0 = col(F, 0), f1 = col(F, 1), f2 = col(F, 2);
+    const float C00 = fc::dot(f0, f0), C11 = fc::dot(f1, f1), C22 = fc::dot(f2, f2);
+    const float C01 = fc::dot(f0, f1), C02 = fc::dot(f0, f2), C12 = fc::dot(f1, f2);
+    const bool soft_ok = mu > 1e-6f && dPhi <= 0.f;  // flip and scaling modes all >= mu
+    bool pd = false;
+    if (soft_ok) {
+        pd = dPhi == 0.f;
         if (!pd) {
             const float T = (mu - 1e-6f) * frcp(-dPhi);
             const float T2 = T * T;
-            const v3 f0 = col(F, 0), f1 = col(F, 1), f2 = col(F, 2);
-            const float m00 = T2 - fc::dot(f0, f0), m11 = T2 - fc::dot(f1, f1), m22 = T2 - fc::dot(f2, f2);
-            const float m01 = -fc::dot(f0, f1), m02 = -fc::dot(f0, f2), m12 = -fc::dot(f1, f2);
+            const float m00 = T2 - C00, m11 = T2 - C11, m22 = T2 - C22;
+            const float m01 = -C01, m02 = -C02, m12 = -C12;
             const float d01 = m00 * m11 - m01 * m01;
             const float dM = m00 * (m11 * m22 - m12 * m12) - m01 * (m01 * m22 - m12 * m02) + m02 * (m01 * m12 - m11 * m02);
             pd = m00 > 0.f && d01 > 0.f && dM > 0.f;
         }
-        if (pd) {
-            const float dg = mu * fc::dot(gradN, gradN);
-            const float c1 = diagExtra * invJ * invJ;
-            const v3 cc = smul(c1, cg);
-            H[0] = V0 * fmaf(cc.x, cg.x, dg);
-            H[1] = V0 * (cc.y * cg.x);
-            H[2] = V0 * fmaf(cc.y, cg.y, dg);
-            H[3] = V0 * (cc.z * cg.x);
-            H[4] = V0 * (cc.z * cg.y);
-            H[5] = V0 * fmaf(cc.z, cg.z, dg);
-            return;
-        }
     }
-    m3 U, V;
-    v3 S;
-    svd3(F, U, S, V);
-    const float s0 = S.x, s1 = S.y, s2 = S.z;
-    float ev3 = mu + dPhi * s2, ev4 = mu + dPhi * s1, ev5 = mu + dPhi * s0;
-    float ev6 = mu - dPhi * s2, ev7 = mu - dPhi * s1, ev8 = mu - dPhi * s0;
-    m3 a;
-    a.m[0][0] = mu + diagExtra * frcp(s0 * s0);
-    a.m[1][1] = mu + diagExtra * frcp(s1 * s1);
-    a.m[2][2] = mu + diagExtra * frcp(s2 * s2);
-    const float lJ = lam * invJ;
-    a.m[0][1] = a.m[1][0] = lJ * s2;
-    a.m[0][2] = a.m[2][0] = lJ * s1;
-    a.m[1][2] = a.m[2][1] = lJ * s0;
-    float ev[9] = {0.f, 0.f, 0.f, ev3, ev4, ev5, ev6, ev7, ev8};
+    if (!pd) {
+        // The projection (math.cpp:137-171 on the 9 modes of math.cpp:86-130)
+        // changes only the modes whose eigenvalue ev lies below the floor, so
+        // the projected block is the unprojected one above plus
+        // (p(ev) - ev) (Q gradN)(Q gradN)^T over those modes, Q = U X V^T the
+        // mode's unit eigenmatrix (F = U S V^T).  With the eigenpairs
+        // (s_k^2, v_k) of F^T F: w_k = v_k . gradN and
+        // u_k = F v_k / s_k = (F + cof F) v_k / (s_k + s_i s_j)
+        // (cof F v_k = s_i s_j u_k; the sum stays accurate for a small s_k).
+        // Mode (i, j, k a permutation): twist ev = mu + dPhi s_k,
+        // Q gradN = (w_j u_i - w_i u_j) / sqrt 2; flip ev = mu - dPhi s_k,
+        // (w_j u_i + w_i u_j) / sqrt 2; scaling: eigenpairs (alpha, z) of the
+        // 3x3 a below, Q gradN = sum_k z_k w_k u_k.  Any orthonormal set of
+        // eigenvectors gives the same block (signs cancel in the outer
+        // products; a repeated eigenvalue's modes share one correction).
+        SymEig3 E;
+        sym_eig3_closed(C00, C01, C02, C11, C12, C22, E);
+        float s[3];
 #pragma unroll
-    for (int i = 3; i < 9; ++i) ev[i] = project_ev(ev[i], projMode);
-    v3 u0 = col(U, 0), u1 = col(U, 1), u2 = col(U, 2);
-    float vg0 = fc::dot(col(V, 0), gradN), vg1 = fc::dot(col(V, 1), gradN), vg2 = fc::dot(col(V, 2), gradN);
-    float Hv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    // Scaling modes: with a = Va diag(aw) Va^T and the columns vg_j u_j of
-    // Ut, their part of the vertex block is Ut Va proj(aw) Va^T Ut^T.  Both
-    // projections (clamp and absolute, floor 1e-6) leave every eigenvalue
-    // >= 1e-6 unchanged, so when a - 1e-6 I is positive definite (Sylvester's
-    // criterion; the usual case) that part is Ut a Ut^T and the eigensystem
-    // is not needed.  Otherwise it is projected through Jacobi as in the
-    // reference.
-    const float e6 = 1e-6f;
-    const float b00 = a.m[0][0] - e6, b11 = a.m[1][1] - e6, b22 = a.m[2][2] - e6;
-    const float m01 = b00 * b11 - a.m[0][1] * a.m[0][1];
-    const float dB = b00 * (b11 * b22 - a.m[1][2] * a.m[1][2]) - a.m[0][1] * (a.m[0][1] * b22 - a.m[1][2] * a.m[0][2]) +
-                     a.m[0][2] * (a.m[0][1] * a.m[1][2] - b11 * a.m[0][2]);
-    if (b00 > 0.f && m01 > 0.f && dB > 0.f) {
-        const v3 t0 = smul(vg0, u0), t1 = smul(vg1, u1), t2 = smul(vg2, u2);
-        // rows of Ut a: r_j = sum_k a_jk t_k; block = sum_j t_j r_j^T
-        const v3 r0 = fc::lin3(a.m[0][0], t0, a.m[0][1], t1, a.m[0][2], t2);
-        const v3 r1 = fc::lin3(a.m[1][0], t0, a.m[1][1], t1, a.m[1][2], t2);
-        const v3 r2 = fc::lin3(a.m[2][0], t0, a.m[2][1], t1, a.m[2][2], t2);
-        Hv[0] = (t0.x * r0.x + t1.x * r1.x) + t2.x * r2.x;
-        Hv[1] = (t0.y * r0.x + t1.y * r1.x) + t2.y * r2.x;
-        Hv[2] = (t0.y * r0.y + t1.y * r1.y) + t2.y * r2.y;
-        Hv[3] = (t0.z * r0.x + t1.z * r1.x) + t2.z * r2.x;
-        Hv[4] = (t0.z * r0.y + t1.z * r1.y) + t2.z * r2.y;
-        Hv[5] = (t0.z * r0.z + t1.z * r1.z) + t2.z * r2.z;
-    } else {
-        v3 aw;
-        m3 av;
-        sym_eig3(a, aw, av);
-        const float pw[3] = {project_ev(aw.x, projMode), project_ev(aw.y, projMode), project_ev(aw.z, projMode)};
+        for (int k = 0; k < 3; ++k) s[k] = fsqrt(fmaxf(E.e[k], 0.f));
+        {
+            // the smallest singular value from J / (the other two): its square
+            // can be lost to the rounding of F^T F
+            const float p12 = s[1] * s[2], p02 = s[0] * s[2], p01 = s[0] * s[1];
+            const bool k0 = E.e[0] <= E.e[1] && E.e[0] <= E.e[2];
+            const bool k1 = !k0 && E.e[1] <= E.e[2];
+            if (k0) { if (p12 > 0.f) s[0] = J * frcp(p12); }
+            else if (k1) { if (p02 > 0.f) s[1] = J * frcp(p02); }
+            else if (p01 > 0.f) s[2] = J * frcp(p01);
+        }
+        m3 G;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) G.m[r][c] = F.m[r][c] + cof.m[r][c];
+        v3 u[3];
+        float w[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            v3 c = col(av, k);
-            v3 w = fc::lin3(c.x * vg0, u0, c.y * vg1, u1, c.z * vg2, u2);
-            add_outer(Hv, pw[k], w);
+            const int i = (k + 1) % 3, j = (k + 2) % 3;
+            u[k] = smul(frcp(fmaxf(fmaf(s[i], s[j], s[k]), 1e-30f)), fc::mv(G, E.v[k]));
+            w[k] = fc::dot(E.v[k], gradN);
+        }
+        // twist modes (the ones that fall below the floor in the usual case)
+        float cf[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int i = (k + 1) % 3, j = (k + 2) % 3;
+            const float et = fmaf(dPhi, s[k], mu), ef = fmaf(-dPhi, s[k], mu);
+            const float ct = project_ev(et, projMode) - et;
+            cf[k] = project_ev(ef, projMode) - ef;
+            add_outer(Hv, 0.5f * ct, fc::lin2(w[j], u[i], -w[i], u[j]));
+        }
+        if (cf[0] > 0.f || cf[1] > 0.f || cf[2] > 0.f) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int i = (k + 1) % 3, j = (k + 2) % 3;
+                add_outer(Hv, 0.5f * cf[k], fc::lin2(w[j], u[i], w[i], u[j]));
+            }
+        }
+        if (!soft_ok) {
+            // scaling block a = diag(mu + diagExtra / s_k^2) + lam / J (s_k off the
+            // diagonal in row i, column j): projected only when a - 1e-6 I is
+            // not positive definite (Sylvester's criterion); with mu > 1e-6
+            // and dPhi <= 0 it is >= mu I
+            const float lJ = lam * invJ;
+            const float a00 = mu + diagExtra * frcp(s[0] * s[0]), a11 = mu + diagExtra * frcp(s[1] * s[1]),
+                        a22 = mu + diagExtra * frcp(s[2] * s[2]);
+            const float a01 = lJ * s[2], a02 = lJ * s[1], a12 = lJ * s[0];
+            const float e6 = 1e-6f;
+            const float b00 = a00 - e6, b11 = a11 - e6, b22 = a22 - e6;
+            const float m01 = b00 * b11 - a01 * a01;
+            const float dB = b00 * (b11 * b22 - a12 * a12) - a01 * (a01 * b22 - a12 * a02) + a02 * (a01 * a12 - b11 * a02);
+            if (!(b00 > 0.f && m01 > 0.f && dB > 0.f)) {
+                SymEig3 A;
+                sym_eig3_closed(a00, a01, a02, a11, a12, a22, A);
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const float ca = project_ev(A.e[m], projMode) - A.e[m];
+                    const v3 z = A.v[m];
+                    add_outer(Hv, ca, fc::lin3(z.x * w[0], u[0], z.y * w[1], u[1], z.z * w[2], u[2]));
+                }
+            }
         }
     }
-    // twist (ev3-5) and flip (ev6-8) modes: w = (vg_j u_i -+ vg_i u_j) / sqrt 2
-    const float is2 = 0.70710678118f;
-    const float a0 = is2 * vg0, a1 = is2 * vg1, a2 = is2 * vg2;
-    add_outer(Hv, ev[3], fc::lin2(a1, u0, -a0, u1));
-    add_outer(Hv, ev[4], fc::lin2(a2, u0, -a0, u2));
-    add_outer(Hv, ev[5], fc::lin2(a2, u1, -a1, u2));
-    add_outer(Hv, ev[6], fc::lin2(a1, u0, a0, u1));
-    add_outer(Hv, ev[7], fc::lin2(a2, u0, a0, u2));
-    add_outer(Hv, ev[8], fc::lin2(a2, u1, a1, u2));
 #pragma unroll
-    for (int i = 0; i < 6; ++i) H[i] = Hv[i] * V0;
+    for (int i = 0; i < 6; ++i) H[i] = V0 * Hv[i];
 }
 
 // StVK membrane triangle: gradient and projected Hessian for the vertex with
