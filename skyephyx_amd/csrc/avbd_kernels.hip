@@ -1415,35 +1415,64 @@ AVBD_DEV void wave_lds_sync() {
 
 // The block solve of a FEM-only particle and its writes (the system is
 // blockdiag(A, 0): the rotation increment is exactly zero).
-AVBD_DEV void fem_vertex_solve(const DevWorld& w, int b, v3 p, float L[6], float r[3]) {
+// rot, last_rot and last_pos are the particle's, loaded by the caller.
+AVBD_DEV void fem_vertex_solve(const DevWorld& w, int b, v3 p, float L[6], float r[3], float4 rot, float4 last_rot,
+                               float4 last_pos) {
     float dx[3];
     ldlt3_solve(L, r, dx);
-    const qt q = fq(w.rot[b]);
-    const qt lr = fq(w.last_rot[b]);
+    const qt q = fq(rot);
+    const qt lr = fq(last_rot);
     v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
     qt dq = quaternion_from_difference(v3{0.f, 0.f, 0.f}, -1.f);
     qt nq = qnormalized(qmul(dq, q));
     w.pos[b] = v4(np, 0.f);
     w.rot[b] = q4(nq);
-    v3 dl = sub(np, f3(w.last_pos[b]));
+    v3 dl = sub(np, f3(last_pos));
     v3 da = rotation_difference(nq, lr);
     w.diff0[b] = make_float4(dl.x, dl.y, dl.z, da.x);
     w.diff1[b] = make_float4(da.y, da.z, 0.f, 0.f);
 }
+AVBD_DEV void fem_vertex_solve(const DevWorld& w, int b, v3 p, float L[6], float r[3]) {
+    fem_vertex_solve(w, b, p, L, r, w.rot[b], w.last_rot[b], w.last_pos[b]);
+}
 
 template <int G>
 AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, int poststab, float4* nbp = nullptr) {
+    // What the sweep needs of b is loaded at once, before the scene's run
+    // test, so a particle waits on few dependent round trips before its
+    // first element: its own records, then its scene and its neighbour ids,
+    // then the neighbours' positions.
     const int s = w.scene[b];
+    const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
+    const int nn = nbp ? w.nb_n[b] : -1;
+    const float4 p4 = w.pos[b], vel = w.vel[b], ipos = w.inert_pos[b];
+    // the neighbour ids of this lane's slots (clamped into the list; a slot
+    // past the list's end later gathers the particle itself, unused)
+    constexpr int PER = G >= 4 ? NB_MAX / G : 1;  // narrower groups have no slot
+    int nid[PER];
+    if constexpr (G >= 4) {
+        if (nn >= 0) {
+            const int* list = w.nb_id + 2 * (size_t)e0;
+            const int jmax = max(nn - 1, 0);
+#pragma unroll
+            for (int t = 0; t < PER; ++t) nid[t] = list[min(lane + t * G, jmax)];
+        }
+    }
     const avbd_scene& sc = w.sc[s];
-    if (w.st[s * ST_WORDS + ST_EMERGENCY]) return;
-    if (poststab ? (!sc.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) : iter >= sc.iterations) return;
-    const float dt2 = sc.dt * sc.dt;
+    // the scene's run test from words loaded together (no load waits on
+    // the outcome of another)
+    const int st_em = w.st[s * ST_WORDS + ST_EMERGENCY], st_oob = w.st[s * ST_WORDS + ST_OOB];
+    const int post = sc.post_stabilization, iters = sc.iterations;
+    const float dt = sc.dt;
+    const bool run = (st_em == 0) & (poststab ? (post != 0) & (st_oob == 0) : iter < iters);
+    if (!run) return;
+    const float dt2 = dt * dt;
     float L[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float r[3] = {0.f, 0.f, 0.f};
-    const v3 p = f3(w.pos[b]);
+    const v3 p = f3(p4);
     if (lane == 0) {
-        const float ml = w.vel[b].w / dt2;
-        const v3 dl = sub(p, f3(w.inert_pos[b]));
+        const float ml = vel.w / dt2;
+        const v3 dl = sub(p, f3(ipos));
         L[0] = ml; L[2] = ml; L[5] = ml;
         r[0] = ml * dl.x; r[1] = ml * dl.y; r[2] = ml * dl.z;
     }
@@ -1453,27 +1482,35 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
     // unconditional (indices clamped to the list; a triangle's slot 3 id is
     // 0) so that none sits under a branch, which would make every wait
     // drain all loads in flight.
-    const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
     // With an LDS slot (bucket sweeps, G >= 4): the group loads the vertex's
     // distinct neighbours once and each element reads its positions from
     // there (about one gather per element instead of four).  The positions
     // are the same values the gathers would read: neighbours share an
     // element with the vertex, so none has its colour and none moves in
-    // this launch.
-    const int nn = nbp ? w.nb_n[b] : -1;
-    if (nn >= 0) {
-        const int* list = w.nb_id + 2 * (size_t)e0;
-        for (int j = lane; j < nn; j += G) nbp[j] = w.pos[list[j]];
-        wave_lds_sync();
+    // this launch.  All of a lane's gathers are issued before the first
+    // LDS write.
+    if (G >= 4 && nn >= 0) {
+        // the lane's first stream record (clamped into the padded stream)
+        // travels with the gathers
         int k = e0 + lane;
+        const int last = max(e1 - 1, e0);
+        const int kc = min(k, last);
+        ElemIn in0;
+        in0.e = w.e_ent[kc];
+        in0.a = w.ev_a[kc];
+        in0.b = w.ev_b[kc];
+        in0.c = w.ev_c[kc];
+        uint32_t l0 = w.ev_l[kc];
+        // every slot of the group's NB_MAX is written (those past the list
+        // hold the particle's own position and are never read): with no
+        // branch around them the gathers all issue before the first wait
+        float4 nv[PER];
+#pragma unroll
+        for (int t = 0; t < PER; ++t) nv[t] = w.pos[lane + t * G < nn ? nid[t] : b];
+#pragma unroll
+        for (int t = 0; t < PER; ++t) nbp[lane + t * G] = nv[t];
+        wave_lds_sync();
         if (k < e1) {
-            const int last = e1 - 1;
-            ElemIn in0;
-            in0.e = w.e_ent[k];
-            in0.a = w.ev_a[k];
-            in0.b = w.ev_b[k];
-            in0.c = w.ev_c[k];
-            uint32_t l0 = w.ev_l[k];
             bool h0 = true;
             while (h0) {
                 const bool h1 = k + G < e1;
