@@ -60,7 +60,7 @@
 extern "C" {
 #endif
 
-#define AVBD_ABI_VERSION 1
+#define AVBD_ABI_VERSION 2
 
 /* status codes */
 #define AVBD_OK 0
@@ -75,6 +75,7 @@ extern "C" {
 /* body flags */
 #define AVBD_BODY_STATIC 1u
 #define AVBD_BODY_PARTICLE 2u
+#define AVBD_BODY_SPHERE 4u  /* Mesh::modelType == ModelType_Sphere (geometry.hpp:17-22) */
 
 /* force flags */
 #define AVBD_FORCE_DISABLED 1u
@@ -83,6 +84,26 @@ extern "C" {
 #define AVBD_PROJ_CLAMP 0
 #define AVBD_PROJ_ABSOLUTE 1
 #define AVBD_PROJ_ADAPTIVE 2
+
+/* Scene options: the two features the reference declares but leaves
+ * unfinished, off by default (the reference's behaviour).
+ * AVBD_SCENE_TRUST_REGION: with projection_mode ADAPTIVE, trust_region_rho is
+ *   updated after every primal sweep (the empty step 5.2 of solver.cpp:451-455)
+ *   to the trust-region ratio of the sweep's FEM-only block updates:
+ *   sum (E(x) - E(x - dx)) / sum (r . dx) / 2 over the particles whose block
+ *   has energies and no forces, E = the particle's inertia term plus the
+ *   elastic energy of its elements; ADAPTIVE then projects by CLAMP when
+ *   |rho - 1| <= 0.01, else by ABSOLUTE (math.cpp:152-160).  A sweep whose
+ *   predicted decrease is below 3e-5 of the summed magnitude of its terms (a
+ *   nearly converged solve, where the ratio is rounding noise) keeps rho.
+ *   Without the option rho keeps its value (0 in the reference, so ADAPTIVE
+ *   acts as ABSOLUTE).
+ * AVBD_SCENE_SPHERE_COLLISION: bodies flagged AVBD_BODY_SPHERE collide as
+ *   spheres of radius scale[0] / 2 (the radius their inertia uses,
+ *   solver.cpp:84-87); the reference collides every pair as boxes
+ *   (manifold.cpp:57). */
+#define AVBD_SCENE_TRUST_REGION 1u
+#define AVBD_SCENE_SPHERE_COLLISION 2u
 
 /* Per-scene solver parameters and the scene's sticky status word.
  * Mirrors Solver's public tunables (solver.hpp:69-86); gravity and world
@@ -102,6 +123,8 @@ typedef struct avbd_scene {
     float bounds_max[3];
     int32_t emergency_stop;   /* state: nonzero once CheckExplosion fired */
     int32_t n_removed;        /* state: bodies removed out of bounds (cumulative) */
+    uint32_t options;         /* AVBD_SCENE_* */
+    float trust_region_rho;   /* state: trustRegionRho (solver.hpp:80) */
 } avbd_scene;
 
 /* One rigid body or particle (Mesh, geometry.hpp:148-256).
@@ -303,6 +326,10 @@ int avbd_snapshot_load(avbd_ctx* ctx, const char* path, int64_t manifold_capacit
 
 /* Per-scene status words: emergency_stop and n_removed for `n` scenes. */
 int avbd_get_status(avbd_ctx* ctx, int32_t* emergency_stop, int32_t* n_removed, int32_t n);
+
+/* Solver::trustRegionRho (solver.hpp:80) of `n` scenes, as the last primal
+ * sweep left it (it changes only under AVBD_SCENE_TRUST_REGION). */
+int avbd_get_trust_region_rho(avbd_ctx* ctx, float* rho, int32_t n);
 
 /* Debug/render output of a step (option "debug_output"): the line and point
  * data Solver::Step collects for the renderer (Solver::lineData /

@@ -45,6 +45,8 @@ struct Body {
     R radius = R(0);
     bool isStatic = false;
     bool isParticle = false;
+    bool isSphere = false;        // ModelType_Sphere (geometry.hpp:17-22)
+    bool collidesAsSphere = false;  // isSphere under the scene's sphere collision option
     bool isDragged = false;       // Mesh::isDragged (geometry.hpp:193)
     V3<R> addedDragVelocity;      // Mesh::addedDragVelocity (geometry.hpp:194)
     V3<R> lastPos, inertialPos;
@@ -376,6 +378,113 @@ CollisionResult<R> collide_boxes(const V3<R>& posA, const Q<R>& qA, const V3<R>&
 }
 
 // ---------------------------------------------------------------- forces
+// Sphere contacts (scene option AVBD_SCENE_SPHERE_COLLISION).  The reference
+// builds a sphere's mass and inertia (solver.cpp:84-87) but collides every
+// pair as boxes (manifold.cpp:57).  A sphere's radius is scale.x / 2, the
+// radius of its inertia.  One contact per pair in CollisionBoxBox's
+// conventions: normal from A to B, penetration > 0 when overlapping,
+// (A's point) - (B's point) = penetration * normal, rA / rB the unscaled
+// body-local points, position their midpoint; feature id kind 3 with the
+// box region of the contact in bits 2-7 (0-26: the 3x3x3 region of the
+// clamped centre outside the box; 27-32: 27 + 2 axis + (side > 0), the face
+// a centre inside the box leaves through).
+static constexpr uint32_t kSphereContact = 3u;
+
+template <class R>
+CollisionResult<R> collide_spheres(const V3<R>& pA, const Q<R>& qA, R radA, const V3<R>& pB, const Q<R>& qB, R radB) {
+    CollisionResult<R> res;
+    const V3<R> d = pB - pA;
+    const R dist2 = dot(d, d);
+    const R rs = radA + radB;
+    if (!(dist2 < rs * rs)) return res;
+    const R dist = std::sqrt(dist2);
+    const V3<R> n = dist > R(0) ? d / dist : V3<R>(R(0), R(1), R(0));
+    const V3<R> offA = n * radA, offB = n * (-radB);
+    Contact<R>& c = res.c[0];
+    c.position = R(0.5) * ((pA + offA) + (pB + offB));
+    c.normal = n;
+    c.penetration = rs - dist;
+    c.rA = mtv(rotmat(qA), offA);
+    c.rB = mtv(rotmat(qB), offB);
+    c.id = kSphereContact;
+    res.n = 1;
+    return res;
+}
+
+// sphere (ps, qs, radius) against the box (pb, qb, scale); sphereIsA: the
+// sphere is the pair's body A
+template <class R>
+CollisionResult<R> collide_sphere_box(const V3<R>& ps, const Q<R>& qs, R rad, const V3<R>& pb, const Q<R>& qb,
+                                      const V3<R>& scale, bool sphereIsA) {
+    CollisionResult<R> res;
+    const M3<R> rb = rotmat(qb);
+    const V3<R> c = mtv(rb, ps - pb);  // sphere centre in the box frame
+    const V3<R> h = scale * R(0.5);
+    auto clampc = [](R x, R hi) { return x < -hi ? -hi : (hi < x ? hi : x); };
+    const V3<R> q(clampc(c.x, h.x), clampc(c.y, h.y), clampc(c.z, h.z));
+    V3<R> nl, onBox;
+    R pen;
+    uint32_t region;
+    if (q.x != c.x || q.y != c.y || q.z != c.z) {
+        const V3<R> d = c - q;
+        const R dist2 = dot(d, d);
+        if (!(dist2 < rad * rad)) return res;
+        const R dist = std::sqrt(dist2);
+        nl = d / dist;
+        pen = rad - dist;
+        onBox = q;
+        auto reg = [](R qi, R hi) { return qi == -hi ? 0u : (qi == hi ? 2u : 1u); };
+        region = reg(q.x, h.x) + 3u * reg(q.y, h.y) + 9u * reg(q.z, h.z);
+    } else {
+        // centre inside the box: out through the face of least depth (first axis on ties)
+        int ax = 0;
+        R best = h.x - std::abs(c.x);
+        const R dy = h.y - std::abs(c.y), dz = h.z - std::abs(c.z);
+        if (dy < best) { best = dy; ax = 1; }
+        if (dz < best) { best = dz; ax = 2; }
+        const R sg = c[ax] >= R(0) ? R(1) : R(-1);
+        nl = V3<R>(R(0), R(0), R(0));
+        nl[ax] = sg;
+        pen = rad + best;
+        onBox = c;
+        onBox[ax] = sg * h[ax];
+        region = 27u + 2u * uint32_t(ax) + (sg > R(0) ? 1u : 0u);
+    }
+    const V3<R> nW = mv(rb, nl);  // box -> sphere
+    const V3<R> boxW = mv(rb, onBox) + pb;
+    const V3<R> offS = nW * (-rad);  // sphere centre -> its contact point
+    Contact<R>& k = res.c[0];
+    k.position = R(0.5) * (boxW + (ps + offS));
+    k.penetration = pen;
+    const V3<R> rS = mtv(rotmat(qs), offS);
+    if (sphereIsA) {
+        k.normal = -nW;
+        k.rA = rS;
+        k.rB = onBox;
+    } else {
+        k.normal = nW;
+        k.rA = onBox;
+        k.rB = rS;
+    }
+    k.id = kSphereContact | (region << 2);
+    res.n = 1;
+    return res;
+}
+
+template <class R>
+CollisionResult<R> collide_bodies(const Body<R>* A, const Body<R>* B) {
+    const R half = R(0.5);
+    if (A->collidesAsSphere && B->collidesAsSphere)
+        return collide_spheres(A->pos, A->rot, A->scale.x * half, B->pos, B->rot, B->scale.x * half);
+    if (A->collidesAsSphere) return collide_sphere_box(A->pos, A->rot, A->scale.x * half, B->pos, B->rot, B->scale, true);
+    if (B->collidesAsSphere) return collide_sphere_box(B->pos, B->rot, B->scale.x * half, A->pos, A->rot, A->scale, false);
+    return collide_boxes(A->pos, A->rot, A->scale, B->pos, B->rot, B->scale);
+}
+
+// A sweep updates the trust-region ratio only when its predicted decrease
+// exceeds this fraction of the magnitude its sums cancel (FP32 ratios then
+// stay within ~3e-4 of FP64 ones, well inside the 0.01 threshold).
+static constexpr double kTrustNoise = 3e-5;
 static constexpr float kCollisionMargin = 0.0005f;  // force.hpp:149
 static constexpr float kStickThreshold = 0.01f;     // force.hpp:150
 
@@ -419,7 +528,7 @@ struct Manifold : Force<R> {
         R oldK[24], oldL[24];
         for (int i = 0; i < oldN; ++i) { oldC[i] = contacts[i]; oldI[i] = infos[i]; }
         for (int i = 0; i < oldN * 3; ++i) { oldK[i] = this->rows[i].penalty; oldL[i] = this->rows[i].lambda; }
-        CollisionResult<R> col = collide_boxes(A->pos, A->rot, A->scale, B->pos, B->rot, B->scale);
+        CollisionResult<R> col = collide_bodies(A, B);
         n = col.n;
         if (n == 0) return false;
         for (int i = 0; i < n; ++i) contacts[i] = col.c[i];
@@ -640,7 +749,10 @@ struct Energy {
             e.erase(std::remove(e.begin(), e.end(), this), e.end());
         }
     }
-    virtual void compute_terms(Body<R>* b, int projectionMode) = 0;
+    virtual void compute_terms(Body<R>* b, int projectionMode, R rho) = 0;
+    // E(vertex b at x - dx) - E(x), the other vertices fixed (the elastic
+    // part of the trust-region ratio; not in the reference)
+    virtual R energy_change(Body<R>* b, const V3<R>& dx) const = 0;
     void attach() {
         for (Body<R>* b : linked) b->energies.push_back(this);
     }
@@ -673,7 +785,7 @@ void project_eigenvalues(R* ev, int count, int mode, R rho = R(0), R threshold =
 // Per-vertex Neo-Hookean gradient (3) and projected Hessian (3x3) of one tet,
 // neohookeanfem.cpp:40-110 with NeoHookeanMath (math.cpp:13-230).
 template <class R>
-void neohookean_vertex_terms(const V3<R> p[4], const M3<R>& DmInv, R V0, R mu, R lam, int vtx, int projMode,
+void neohookean_vertex_terms(const V3<R> p[4], const M3<R>& DmInv, R V0, R mu, R lam, int vtx, int projMode, R rho,
                              V3<R>& g, M3<R>& H) {
     M3<R> Ds;
     Ds.set_col(0, p[1] - p[0]);
@@ -723,7 +835,7 @@ void neohookean_vertex_terms(const V3<R> p[4], const M3<R>& DmInv, R V0, R mu, R
     M3<R> av;
     sym_eig3(a, aw, av);
     ev[0] = aw[0]; ev[1] = aw[1]; ev[2] = aw[2];
-    project_eigenvalues(ev, 9, projMode);
+    project_eigenvalues(ev, 9, projMode, rho);
     // ReconstructVertexHessian (math.cpp:174-230)
     V3<R> u[3] = {U.col(0), U.col(1), U.col(2)};
     R vg[3] = {dot(V.col(0), gradN), dot(V.col(1), gradN), dot(V.col(2), gradN)};
@@ -780,7 +892,7 @@ void svd32(const V3<R>& f0, const V3<R>& f1, V3<R> U[3], R S[2], R V[2][2]) {
 // Per-vertex StVK membrane terms of one triangle (stvkfem.cpp:38-86, STVKMath
 // math.cpp:237-356).
 template <class R>
-void stvk_vertex_terms(const V3<R> p[3], const R DmInv[2][2], R area, R mu, R lam, int vtx, int projMode, V3<R>& g,
+void stvk_vertex_terms(const V3<R> p[3], const R DmInv[2][2], R area, R mu, R lam, int vtx, int projMode, R rho, V3<R>& g,
                        M3<R>& H) {
     V3<R> e0 = p[1] - p[0], e1 = p[2] - p[0];
     // F = Ds * DmInv (3x2); columns f0, f1
@@ -822,7 +934,7 @@ void stvk_vertex_terms(const V3<R> p[3], const R DmInv[2][2], R area, R mu, R la
     R w[2], E[2][2];
     sym_eig2(A00, A01, A11, w, E);
     ev[0] = w[0]; ev[1] = w[1];
-    project_eigenvalues(ev, 4, projMode);
+    project_eigenvalues(ev, 4, projMode, rho);
     R vg[2] = {V[0][0] * gn0 + V[1][0] * gn1, V[0][1] * gn0 + V[1][1] * gn1};
     M3<R> Hv;
     auto add_outer = [&](R l, const V3<R>& w3) {
@@ -839,6 +951,92 @@ void stvk_vertex_terms(const V3<R> p[3], const R DmInv[2][2], R area, R mu, R la
     H = mscale(Hv, area);
 }
 
+// Energy change of a Neo-Hookean tet when its vertex vtx moves from p[vtx]
+// to p[vtx] - dx, the others fixed (the elastic part of the trust-region
+// ratio of AVBD_SCENE_TRUST_REGION; the reference has no energy values).
+// dF = -dx gN^T is rank one, so J' = J - dx . (cof F gN) exactly and
+// log J' - log J = log1p(dJ / J): the change is formed term by term,
+// V0 [mu (F:dF + |dF|^2 / 2) - mu dlog + lam dlog (log J + dlog / 2)], not as
+// the difference of two large energies.  The energy is the one whose
+// gradient neohookean_vertex_terms uses: mu/2 (tr F^T F - 3) - mu log J +
+// lam/2 log^2 J, and for J <= 1e-6 the penalty (mu + lam)(1e-6 - J)^2 / 2
+// (neohookeanfem.cpp:86-110); a move across that branch takes the two
+// energies' difference.
+template <class R>
+R neohookean_energy_change(const V3<R> p[4], const M3<R>& DmInv, R V0, R mu, R lam, int vtx, const V3<R>& dx) {
+    M3<R> Ds;
+    Ds.set_col(0, p[1] - p[0]);
+    Ds.set_col(1, p[2] - p[0]);
+    Ds.set_col(2, p[3] - p[0]);
+    const M3<R> F = mm(Ds, DmInv);
+    V3<R> gN[4];
+    gN[1] = DmInv.row(0);
+    gN[2] = DmInv.row(1);
+    gN[3] = DmInv.row(2);
+    gN[0] = (-gN[1] - gN[2]) - gN[3];
+    const V3<R> g = gN[vtx];
+    const R J = det3(F);
+    const V3<R> cg = mv(cofactor(F), g);
+    const R dJ = -dot(dx, cg);
+    const R Jn = J + dJ;
+    const R FdF = -dot(dx, mv(F, g));
+    const R dF2 = dot(dx, dx) * dot(g, g);
+    const R floorJ = R(1e-6f);
+    if (J > floorJ && Jn > floorJ) {
+        const R logJ = std::log(J);
+        const R dl = std::log1p(dJ / J);
+        return V0 * ((mu * (FdF + R(0.5) * dF2) - mu * dl) + (lam * dl) * (logJ + R(0.5) * dl));
+    }
+    R trC = R(0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) trC = trC + F.m[i][j] * F.m[i][j];
+    auto psi = [&](R tc, R Jv) {
+        if (Jv <= floorJ) return R(0.5) * (mu + lam) * ((floorJ - Jv) * (floorJ - Jv));
+        const R l = std::log(Jv);
+        return (R(0.5) * mu * (tc - R(3)) - mu * l) + R(0.5) * lam * (l * l);
+    };
+    return V0 * (psi((trC + R(2) * FdF) + dF2, Jn) - psi(trC, J));
+}
+
+// The same for a StVK membrane triangle (energy mu E:E + lam/2 tr(E)^2,
+// E = (F^T F - I) / 2, the energy of the gradient stvk_vertex_terms uses, and
+// (mu + lam) |F|^2 / 2 when the area collapses, stvkfem.cpp:76-86):
+// dE = (F^T dF + dF^T F + dF^T dF) / 2 with dF = -dx gn^T.
+template <class R>
+R stvk_energy_change(const V3<R> p[3], const R DmInv[2][2], R area, R mu, R lam, int vtx, const V3<R>& dx) {
+    const V3<R> e0 = p[1] - p[0], e1 = p[2] - p[0];
+    const V3<R> f0 = e0 * DmInv[0][0] + e1 * DmInv[1][0];
+    const V3<R> f1 = e0 * DmInv[0][1] + e1 * DmInv[1][1];
+    R gN[3][2];
+    gN[1][0] = DmInv[0][0]; gN[1][1] = DmInv[0][1];
+    gN[2][0] = DmInv[1][0]; gN[2][1] = DmInv[1][1];
+    gN[0][0] = -gN[1][0] - gN[2][0];
+    gN[0][1] = -gN[1][1] - gN[2][1];
+    const R gn0 = gN[vtx][0], gn1 = gN[vtx][1];
+    // the moved vertex's edges
+    V3<R> q[3] = {p[0], p[1], p[2]};
+    q[vtx] = q[vtx] - dx;
+    const V3<R> n0 = q[1] - q[0], n1 = q[2] - q[0];
+    const R lim = (R(2) * area) * R(0.01f);
+    const bool degOld = norm(cross(e0, e1)) < lim, degNew = norm(cross(n0, n1)) < lim;
+    const R a0 = dot(f0, dx), a1 = dot(f1, dx), d2 = dot(dx, dx);
+    const R E00 = R(0.5) * (dot(f0, f0) - R(1)), E01 = R(0.5) * dot(f0, f1), E11 = R(0.5) * (dot(f1, f1) - R(1));
+    const R dE00 = R(0.5) * (d2 * (gn0 * gn0)) - a0 * gn0;
+    const R dE01 = R(0.5) * ((d2 * (gn0 * gn1) - a0 * gn1) - a1 * gn0);
+    const R dE11 = R(0.5) * (d2 * (gn1 * gn1)) - a1 * gn1;
+    if (!degOld && !degNew) {
+        const R tr = E00 + E11, dtr = dE00 + dE11;
+        const R dEE = R(2) * ((E00 * dE00 + R(2) * (E01 * dE01)) + E11 * dE11) + ((dE00 * dE00 + R(2) * (dE01 * dE01)) + dE11 * dE11);
+        return area * (mu * dEE + R(0.5) * lam * (R(2) * tr * dtr + dtr * dtr));
+    }
+    auto psi = [&](bool deg, R a, R b, R c) {  // E00, E01, E11
+        if (deg) return R(0.5) * (mu + lam) * ((R(2) * a + R(1)) + (R(2) * c + R(1)));  // |F|^2 = 2 (E00 + E11) + 2
+        const R t = a + c;
+        return mu * ((a * a + R(2) * (b * b)) + c * c) + R(0.5) * lam * (t * t);
+    };
+    return area * (psi(degNew, E00 + dE00, E01 + dE01, E11 + dE11) - psi(degOld, E00, E01, E11));
+}
+
 template <class R>
 struct NeoHookeanTet : Energy<R> {
     Body<R>* v[4];
@@ -849,7 +1047,7 @@ struct NeoHookeanTet : Energy<R> {
         this->linked = {a, b, c, d};
         this->attach();
     }
-    void compute_terms(Body<R>* b, int projMode) override {
+    void compute_terms(Body<R>* b, int projMode, R rho) override {
         V3<R> p[4] = {v[0]->pos, v[1]->pos, v[2]->pos, v[3]->pos};
         int k = -1;
         for (int i = 0; i < 4; ++i)
@@ -859,9 +1057,15 @@ struct NeoHookeanTet : Energy<R> {
         if (k < 0) return;
         V3<R> g;
         M3<R> H;
-        neohookean_vertex_terms(p, DmInv, V0, mu, lam, k, projMode, g, H);
+        neohookean_vertex_terms(p, DmInv, V0, mu, lam, k, projMode, rho, g, H);
         this->grad.set(g, V3<R>());
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) this->hess.m[i][j] = H.m[i][j];
+    }
+    R energy_change(Body<R>* b, const V3<R>& dx) const override {
+        V3<R> p[4] = {v[0]->pos, v[1]->pos, v[2]->pos, v[3]->pos};
+        for (int i = 0; i < 4; ++i)
+            if (v[i] == b) return neohookean_energy_change(p, DmInv, V0, mu, lam, i, dx);
+        return R(0);
     }
 };
 
@@ -875,12 +1079,17 @@ struct StvkTri : Energy<R> {
         this->linked = {a, b, c};
         this->attach();
     }
-    void compute_terms(Body<R>* b, int projMode) override {
+    R energy_change(Body<R>* b, const V3<R>& dx) const override {
+        V3<R> p[3] = {v[0]->pos, v[1]->pos, v[2]->pos};
+        int k = (b == v[0]) ? 0 : ((b == v[1]) ? 1 : 2);
+        return stvk_energy_change(p, DmInv, area, mu, lam, k, dx);
+    }
+    void compute_terms(Body<R>* b, int projMode, R rho) override {
         V3<R> p[3] = {v[0]->pos, v[1]->pos, v[2]->pos};
         int k = (b == v[0]) ? 0 : ((b == v[1]) ? 1 : 2);  // stvkfem.cpp:50-53
         V3<R> g;
         M3<R> H;
-        stvk_vertex_terms(p, DmInv, area, mu, lam, k, projMode, g, H);
+        stvk_vertex_terms(p, DmInv, area, mu, lam, k, projMode, rho, g, H);
         this->grad = V6<R>();
         this->hess = M6<R>();
         this->grad.set(g, V3<R>());
@@ -908,6 +1117,8 @@ struct World {
     V3<R> boundsMin{R(-50), R(-50), R(-50)}, boundsMax{R(50), R(50), R(50)};
     bool emergencyStop = false;
     int nRemoved = 0;
+    R trustRegionRho = R(0);  // solver.hpp:80 (updated only under AVBD_SCENE_TRUST_REGION)
+    uint32_t options = 0;     // AVBD_SCENE_*
     int64_t nextSerial = 0;
     VisitOrder order = VisitOrder::Natural;
     std::vector<int> colour;  // per body, -1 static (last computed)
@@ -1054,8 +1265,9 @@ struct World {
         return false;
     }
 
-    // One primal block update of one body (solver.cpp:392-448 / 540-595).
-    void primal_body(Body<R>* b, R alphaC) {
+    // One primal block update of one body (solver.cpp:392-448 / 540-595);
+    // rhsOut / dxOut receive the block's right-hand side and solution.
+    void primal_body(Body<R>* b, R alphaC, V6<R>* rhsOut = nullptr, V6<R>* dxOut = nullptr) {
         const R dt2 = dt * dt;
         M6<R> lhs;
         for (int i = 0; i < 3; ++i) lhs.m[i][i] = b->mass / dt2;
@@ -1090,7 +1302,7 @@ struct World {
             }
         }
         for (Energy<R>* e : b->energies) {
-            e->compute_terms(b, projectionMode);
+            e->compute_terms(b, projectionMode, trustRegionRho);
             for (int k = 0; k < 6; ++k) rhs[k] = rhs[k] + e->grad[k];
             for (int k = 0; k < 6; ++k)
                 for (int l = 0; l < 6; ++l) lhs.m[k][l] = lhs.m[k][l] + e->hess.m[k][l];
@@ -1098,6 +1310,8 @@ struct World {
         LDLT6<R> ldlt;
         ldlt.factor(lhs);
         V6<R> dx = ldlt.solve(rhs);
+        if (rhsOut) *rhsOut = rhs;
+        if (dxOut) *dxOut = dx;
         b->pos = b->pos - V3<R>(dx[0], dx[1], dx[2]);
         Q<R> dq = quaternion_from_difference(V3<R>(dx[3], dx[4], dx[5]), R(-1));
         b->rot = qnormalized(qmul(dq, b->rot));
@@ -1208,11 +1422,44 @@ struct World {
         // 5. iterations
         t0 = Clock::now();
         std::vector<Body<R>*> visit = visit_list();
+        const bool trustUpdate = projectionMode == AVBD_PROJ_ADAPTIVE && (options & AVBD_SCENE_TRUST_REGION);
         for (int it = 0; it < iterations; ++it) {
+            // trust-region ratio of the sweep (step 5.2, solver.cpp:451-455,
+            // empty in the reference): over the FEM-only block updates (a
+            // body with energies and no forces), the actual decrease of its
+            // inertia term plus its elements' energy against the decrease
+            // r . dx / 2 its block model predicts, summed in double.  Both
+            // sums cancel terms of size `scale` (the summed magnitudes of the
+            // per-term changes), so a sweep whose predicted decrease is within
+            // kTrustNoise of that scale (a nearly converged solve, where the
+            // ratio is rounding noise) leaves rho as it was.
+            double actual = 0.0, predicted = 0.0, scale = 0.0;
             for (Body<R>* b : visit) {
                 if (b->isStatic) continue;
-                primal_body(b, alpha);
+                if (trustUpdate && b->forces.empty() && !b->energies.empty()) {
+                    V6<R> rhs, dx;
+                    const V3<R> xOld = b->pos;
+                    const V3<R> y = xOld - b->inertialPos;
+                    primal_body(b, alpha, &rhs, &dx);
+                    const V3<R> xNew = b->pos;
+                    const V3<R> d(dx[0], dx[1], dx[2]);
+                    b->pos = xOld;  // the energies' change from the old position
+                    R dE = (R(0.5) * (b->mass / (dt * dt))) * (dot(d, d) - R(2) * dot(d, y));
+                    R dS = std::abs(dE);
+                    for (Energy<R>* e : b->energies) {
+                        const R de = e->energy_change(b, d);
+                        dE = dE + de;
+                        dS = dS + std::abs(de);
+                    }
+                    b->pos = xNew;
+                    actual -= double(dE);
+                    scale += double(dS);
+                    predicted += 0.5 * ((double(rhs[0]) * double(dx[0]) + double(rhs[1]) * double(dx[1])) + double(rhs[2]) * double(dx[2]));
+                } else {
+                    primal_body(b, alpha);
+                }
             }
+            if (trustUpdate && predicted > kTrustNoise * scale) trustRegionRho = R(actual / predicted);
             // dual (solver.cpp:457-489)
             std::vector<Force<R>*> fl;
             for (auto& f : forces) fl.push_back(f.get());

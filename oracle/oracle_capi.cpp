@@ -77,6 +77,9 @@ int load(World<R>& W, const avbd_world* w, int scene) {
     W.boundsMax = v3<R>(sc.bounds_max);
     W.emergencyStop = sc.emergency_stop != 0;
     W.nRemoved = sc.n_removed;
+    W.options = sc.options;
+    W.trustRegionRho = R(sc.trust_region_rho);
+    const bool sphereCollision = (sc.options & AVBD_SCENE_SPHERE_COLLISION) != 0;
     std::vector<int> local(w->n_bodies, -1);
     for (int i = 0; i < w->n_bodies; ++i) {
         const avbd_body& b = w->bodies[i];
@@ -97,6 +100,8 @@ int load(World<R>& W, const avbd_world* w, int scene) {
         B->radius = R(b.radius);
         B->isStatic = (b.flags & AVBD_BODY_STATIC) != 0;
         B->isParticle = (b.flags & AVBD_BODY_PARTICLE) != 0;
+        B->isSphere = (b.flags & AVBD_BODY_SPHERE) != 0;
+        B->collidesAsSphere = B->isSphere && sphereCollision;
         W.bodies.push_back(std::move(B));
     }
     auto body = [&](int g) -> Body<R>* {
@@ -248,6 +253,8 @@ int save(const World<R>& W, avbd_world* out) {
     put3(sc.bounds_max, W.boundsMax);
     sc.emergency_stop = W.emergencyStop ? 1 : 0;
     sc.n_removed = W.nRemoved;
+    sc.options = W.options;
+    sc.trust_region_rho = float(W.trustRegionRho);
     for (size_t i = 0; i < W.bodies.size(); ++i) {
         const Body<R>& B = *W.bodies[i];
         avbd_body& b = out->bodies[i];
@@ -263,7 +270,8 @@ int save(const World<R>& W, avbd_world* out) {
         put3(b.inertia, B.inertia);
         b.friction = float(B.friction);
         b.radius = float(B.radius);
-        b.flags = (B.isStatic ? AVBD_BODY_STATIC : 0u) | (B.isParticle ? AVBD_BODY_PARTICLE : 0u);
+        b.flags = (B.isStatic ? AVBD_BODY_STATIC : 0u) | (B.isParticle ? AVBD_BODY_PARTICLE : 0u) |
+                  (B.isSphere ? AVBD_BODY_SPHERE : 0u);
         b.scene = 0;
     }
     std::vector<const Force<R>*> fs;
@@ -383,6 +391,42 @@ int box_collide(const float* a, const float* b, float* out, uint32_t* ids) {
     return r.n;
 }
 
+// collide_bodies on two bodies given as (pos 3, rot 4, scale 3) with their
+// sphere flags (collide_boxes when neither is a sphere)
+template <class R>
+int body_collide(const float* a, const float* b, int sphereA, int sphereB, float* out, uint32_t* ids) {
+    Body<R> A, B;
+    A.pos = v3<R>(a); A.rot = q4<R>(a + 3); A.scale = v3<R>(a + 7); A.isSphere = A.collidesAsSphere = sphereA != 0;
+    B.pos = v3<R>(b); B.rot = q4<R>(b + 3); B.scale = v3<R>(b + 7); B.isSphere = B.collidesAsSphere = sphereB != 0;
+    CollisionResult<R> r = collide_bodies(&A, &B);
+    for (int i = 0; i < r.n; ++i) {
+        float* o = out + i * 13;
+        put3(o, r.c[i].position);
+        put3(o + 3, r.c[i].normal);
+        o[6] = float(r.c[i].penetration);
+        put3(o + 7, r.c[i].rA);
+        put3(o + 10, r.c[i].rB);
+        ids[i] = r.c[i].id;
+    }
+    return r.n;
+}
+
+template <class R>
+static double nh_energy_change(const double* p, const double* dm, double V0, double mu, double lam, int vtx, const double* dx) {
+    V3<R> P[4];
+    for (int i = 0; i < 4; ++i) P[i] = V3<R>(R(p[i * 3]), R(p[i * 3 + 1]), R(p[i * 3 + 2]));
+    M3<R> D;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) D.m[i][j] = R(dm[i * 3 + j]);
+    return double(neohookean_energy_change(P, D, R(V0), R(mu), R(lam), vtx, V3<R>(R(dx[0]), R(dx[1]), R(dx[2]))));
+}
+template <class R>
+static double stvk_energy_change_(const double* p, const double* dm, double area, double mu, double lam, int vtx, const double* dx) {
+    V3<R> P[3];
+    for (int i = 0; i < 3; ++i) P[i] = V3<R>(R(p[i * 3]), R(p[i * 3 + 1]), R(p[i * 3 + 2]));
+    R D[2][2] = {{R(dm[0]), R(dm[1])}, {R(dm[2]), R(dm[3])}};
+    return double(stvk_energy_change(P, D, R(area), R(mu), R(lam), vtx, V3<R>(R(dx[0]), R(dx[1]), R(dx[2]))));
+}
+
 template <class R>
 static void nh_terms(const double* p, const double* dm, double V0, double mu, double lam, int vtx, int proj, double* g,
                      double* H) {
@@ -391,7 +435,7 @@ static void nh_terms(const double* p, const double* dm, double V0, double mu, do
     M3<R> D;
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) D.m[i][j] = R(dm[i * 3 + j]);
     V3<R> gg; M3<R> HH;
-    neohookean_vertex_terms(P, D, R(V0), R(mu), R(lam), vtx, proj, gg, HH);
+    neohookean_vertex_terms(P, D, R(V0), R(mu), R(lam), vtx, proj, R(0), gg, HH);
     for (int i = 0; i < 3; ++i) { g[i] = double(gg[i]); for (int j = 0; j < 3; ++j) H[i * 3 + j] = double(HH.m[i][j]); }
 }
 template <class R>
@@ -401,7 +445,7 @@ static void stvk_terms(const double* p, const double* dm, double area, double mu
     for (int i = 0; i < 3; ++i) P[i] = V3<R>(R(p[i * 3]), R(p[i * 3 + 1]), R(p[i * 3 + 2]));
     R D[2][2] = {{R(dm[0]), R(dm[1])}, {R(dm[2]), R(dm[3])}};
     V3<R> gg; M3<R> HH;
-    stvk_vertex_terms(P, D, R(area), R(mu), R(lam), vtx, proj, gg, HH);
+    stvk_vertex_terms(P, D, R(area), R(mu), R(lam), vtx, proj, R(0), gg, HH);
     for (int i = 0; i < 3; ++i) { g[i] = double(gg[i]); for (int j = 0; j < 3; ++j) H[i * 3 + j] = double(HH.m[i][j]); }
 }
 }  // namespace
@@ -482,6 +526,22 @@ int orc_timings(void* hp, double* out) {
 // records of 13 floats {position[3], normal[3], penetration, rA[3], rB[3]}.
 int orc_collide(const float* a, const float* b, int fp64, float* out, uint32_t* ids) {
     return fp64 ? box_collide<double>(a, b, out, ids) : box_collide<float>(a, b, out, ids);
+}
+
+// Contacts of two bodies, either possibly a sphere (AVBD_SCENE_SPHERE_COLLISION).
+int orc_collide_bodies(const float* a, const float* b, int sphereA, int sphereB, int fp64, float* out, uint32_t* ids) {
+    return fp64 ? body_collide<double>(a, b, sphereA, sphereB, out, ids) : body_collide<float>(a, b, sphereA, sphereB, out, ids);
+}
+
+// E(x - dx) - E(x) of one element for a move of its vertex vtx.
+double orc_nh_energy_change(const double* p, const double* dm, double V0, double mu, double lam, int vtx,
+                            const double* dx, int fp64) {
+    return fp64 ? nh_energy_change<double>(p, dm, V0, mu, lam, vtx, dx) : nh_energy_change<float>(p, dm, V0, mu, lam, vtx, dx);
+}
+double orc_stvk_energy_change(const double* p, const double* dm, double area, double mu, double lam, int vtx,
+                              const double* dx, int fp64) {
+    return fp64 ? stvk_energy_change_<double>(p, dm, area, mu, lam, vtx, dx)
+                : stvk_energy_change_<float>(p, dm, area, mu, lam, vtx, dx);
 }
 
 // LDLT solve of a symmetric 6x6 (row-major, lower triangle read).

@@ -53,6 +53,11 @@ def lib():
         dp = C.POINTER(C.c_double)
         fp = C.POINTER(C.c_float)
         L.orc_collide.argtypes = [fp, fp, C.c_int, fp, C.POINTER(C.c_uint32)]
+        L.orc_collide_bodies.argtypes = [fp, fp, C.c_int, C.c_int, C.c_int, fp, C.POINTER(C.c_uint32)]
+        L.orc_nh_energy_change.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, C.c_int]
+        L.orc_nh_energy_change.restype = C.c_double
+        L.orc_stvk_energy_change.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, C.c_int]
+        L.orc_stvk_energy_change.restype = C.c_double
         L.orc_ldlt_solve.argtypes = [dp, dp, dp, C.c_int]
         L.orc_nh_terms.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_stvk_terms.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
@@ -149,6 +154,34 @@ def collide(box_a, box_b, fp64=False):
     ids = np.zeros(8, np.uint32)
     n = lib().orc_collide(_fptr(a), _fptr(b), 1 if fp64 else 0, _fptr(out), ids.ctypes.data_as(C.POINTER(C.c_uint32)))
     return n, out[:n].copy(), ids[:n].copy()
+
+
+def collide_bodies(body_a, body_b, sphere_a=False, sphere_b=False, fp64=False):
+    """Contacts of two bodies (pos[3], rot[4], scale[3]), each possibly a
+    sphere of radius scale[0] / 2 -> (n, contacts (n,13), ids (n,))."""
+    a = np.asarray(body_a, np.float32).reshape(10)
+    b = np.asarray(body_b, np.float32).reshape(10)
+    out = np.zeros((8, 13), np.float32)
+    ids = np.zeros(8, np.uint32)
+    n = lib().orc_collide_bodies(_fptr(a), _fptr(b), int(sphere_a), int(sphere_b), 1 if fp64 else 0, _fptr(out),
+                                 ids.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return n, out[:n].copy(), ids[:n].copy()
+
+
+def neohookean_energy_change(p, dm_inv, V0, mu, lam, vtx, dx, fp64=True):
+    """E(vertex vtx at p[vtx] - dx) - E(p) of one Neo-Hookean tet."""
+    p = np.ascontiguousarray(p, np.float64).reshape(12)
+    dm = np.ascontiguousarray(dm_inv, np.float64).reshape(9)
+    d = np.ascontiguousarray(dx, np.float64).reshape(3)
+    return lib().orc_nh_energy_change(_dptr(p), _dptr(dm), V0, mu, lam, vtx, _dptr(d), 1 if fp64 else 0)
+
+
+def stvk_energy_change(p, dm_inv, area, mu, lam, vtx, dx, fp64=True):
+    """E(vertex vtx at p[vtx] - dx) - E(p) of one StVK triangle."""
+    p = np.ascontiguousarray(p, np.float64).reshape(9)
+    dm = np.ascontiguousarray(dm_inv, np.float64).reshape(4)
+    d = np.ascontiguousarray(dx, np.float64).reshape(3)
+    return lib().orc_stvk_energy_change(_dptr(p), _dptr(dm), area, mu, lam, vtx, _dptr(d), 1 if fp64 else 0)
 
 
 def ldlt_solve(A, b, fp64=True):

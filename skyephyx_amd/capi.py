@@ -49,6 +49,7 @@ def lib():
         L.avbd_get_counts.argtypes = [vp, C.POINTER(AvbdWorld)]
         L.avbd_download.argtypes = [vp, C.POINTER(AvbdWorld)]
         L.avbd_get_status.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32]
+        L.avbd_get_trust_region_rho.argtypes = [vp, C.POINTER(C.c_float), C.c_int32]
         L.avbd_get_timings.argtypes = [vp, C.POINTER(AvbdTimings)]
         L.avbd_last_removed.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
                                          C.POINTER(C.c_int32)]
@@ -206,6 +207,12 @@ class Context:
         _check(self.L.avbd_get_status(self.h, em.ctypes.data_as(C.POINTER(C.c_int32)),
                                       rem.ctypes.data_as(C.POINTER(C.c_int32)), n_scenes))
         return em, rem
+
+    def trust_region_rho(self, n_scenes: int):
+        """Solver::trustRegionRho of each scene (avbd_get_trust_region_rho)."""
+        rho = np.zeros(n_scenes, np.float32)
+        _check(self.L.avbd_get_trust_region_rho(self.h, rho.ctypes.data_as(C.POINTER(C.c_float)), n_scenes))
+        return rho
 
     def remove_bodies(self, bodies):
         """Solver::RemoveBody of live bodies (indices before the call), in order."""

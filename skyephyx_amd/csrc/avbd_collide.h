@@ -236,3 +236,101 @@ AVBD_DEV void collide_boxes(v3 posA, qt qA, v3 scaleA, v3 posB, qt qB, v3 scaleB
         edge_edge_contact(posA, scaleA, rotA, posB, scaleB, rotB, normal, (bestAxis - 6) / 3, (bestAxis - 6) % 3, res);
     }
 }
+
+// Sphere contacts (scene option AVBD_SCENE_SPHERE_COLLISION; the reference
+// builds a sphere's mass and inertia, solver.cpp:84-87, but collides every
+// pair as boxes, manifold.cpp:57).  A sphere's radius is scale.x / 2, its
+// inertia's radius.  One contact per pair in collide_boxes' conventions:
+// normal from A to B, penetration > 0 when overlapping, (A's point) - (B's
+// point) = penetration * normal, rA / rB unscaled body-local points,
+// position their midpoint; feature id kind 3 with the box region in bits 2-7
+// (0-26: the 3x3x3 region of a centre outside the box, clamped onto it;
+// 27 + 2 axis + (side > 0): the face a centre inside the box leaves by).
+// Same operation order as the oracle, so contacts and ids match it exactly.
+#define CONTACT_SPHERE 3u
+
+AVBD_DEV void collide_spheres(v3 pA, qt qA, float radA, v3 pB, qt qB, float radB, DevCollision& res) {
+    res.n = 0;
+    const v3 d = sub(pB, pA);
+    const float dist2 = dot(d, d);
+    const float rs = radA + radB;
+    if (!(dist2 < rs * rs)) return;
+    const float dist = sqrtf(dist2);
+    const v3 n = dist > 0.f ? vdiv(d, dist) : v3{0.f, 1.f, 0.f};
+    const v3 offA = mul(n, radA), offB = mul(n, -radB);
+    DevContact& c = res.c[0];
+    c.position = smul(0.5f, add(add(pA, offA), add(pB, offB)));
+    c.normal = n;
+    c.penetration = rs - dist;
+    c.rA = mtv(rotmat(qA), offA);
+    c.rB = mtv(rotmat(qB), offB);
+    c.id = CONTACT_SPHERE;
+    res.n = 1;
+}
+
+AVBD_DEV float clamp_box(float x, float hi) { return x < -hi ? -hi : (hi < x ? hi : x); }
+AVBD_DEV uint32_t box_region(float qi, float hi) { return qi == -hi ? 0u : (qi == hi ? 2u : 1u); }
+
+// sphere (ps, qs, radius) against the box (pb, qb, scale); sphereIsA: the
+// sphere is the pair's body A
+AVBD_DEV void collide_sphere_box(v3 ps, qt qs, float rad, v3 pb, qt qb, v3 scale, bool sphereIsA, DevCollision& res) {
+    res.n = 0;
+    const m3 rb = rotmat(qb);
+    const v3 c = mtv(rb, sub(ps, pb));
+    const v3 h = mul(scale, 0.5f);
+    const v3 q{clamp_box(c.x, h.x), clamp_box(c.y, h.y), clamp_box(c.z, h.z)};
+    v3 nl, onBox;
+    float pen;
+    uint32_t region;
+    if (q.x != c.x || q.y != c.y || q.z != c.z) {
+        const v3 d = sub(c, q);
+        const float dist2 = dot(d, d);
+        if (!(dist2 < rad * rad)) return;
+        const float dist = sqrtf(dist2);
+        nl = vdiv(d, dist);
+        pen = rad - dist;
+        onBox = q;
+        region = box_region(q.x, h.x) + 3u * box_region(q.y, h.y) + 9u * box_region(q.z, h.z);
+    } else {
+        // centre inside the box: out through the face of least depth (first axis on ties)
+        int ax = 0;
+        float best = h.x - fabsf(c.x);
+        const float dy = h.y - fabsf(c.y), dz = h.z - fabsf(c.z);
+        if (dy < best) { best = dy; ax = 1; }
+        if (dz < best) { best = dz; ax = 2; }
+        const float ca = ax == 0 ? c.x : (ax == 1 ? c.y : c.z);
+        const float ha = ax == 0 ? h.x : (ax == 1 ? h.y : h.z);
+        const float sg = ca >= 0.f ? 1.f : -1.f;
+        nl = v3{ax == 0 ? sg : 0.f, ax == 1 ? sg : 0.f, ax == 2 ? sg : 0.f};
+        pen = rad + best;
+        const float face = sg * ha;
+        onBox = v3{ax == 0 ? face : c.x, ax == 1 ? face : c.y, ax == 2 ? face : c.z};
+        region = 27u + 2u * (uint32_t)ax + (sg > 0.f ? 1u : 0u);
+    }
+    const v3 nW = mv(rb, nl);  // box -> sphere
+    const v3 boxW = add(mv(rb, onBox), pb);
+    const v3 offS = mul(nW, -rad);
+    DevContact& k = res.c[0];
+    k.position = smul(0.5f, add(boxW, add(ps, offS)));
+    k.penetration = pen;
+    const v3 rS = mtv(rotmat(qs), offS);
+    if (sphereIsA) {
+        k.normal = neg(nW);
+        k.rA = rS;
+        k.rB = onBox;
+    } else {
+        k.normal = nW;
+        k.rA = onBox;
+        k.rB = rS;
+    }
+    k.id = CONTACT_SPHERE | (region << 2);
+    res.n = 1;
+}
+
+// The pair's contacts: spheres as spheres when flagged, boxes otherwise.
+AVBD_DEV void collide_bodies(v3 pA, qt qA, v3 sA, bool sphA, v3 pB, qt qB, v3 sB, bool sphB, DevCollision& res) {
+    if (sphA && sphB) collide_spheres(pA, qA, sA.x * 0.5f, pB, qB, sB.x * 0.5f, res);
+    else if (sphA) collide_sphere_box(pA, qA, sA.x * 0.5f, pB, qB, sB, true, res);
+    else if (sphB) collide_sphere_box(pB, qB, sB.x * 0.5f, pA, qA, sA, false, res);
+    else collide_boxes(pA, qA, sA, pB, qB, sB, res);
+}

@@ -410,3 +410,82 @@ AVBD_DEV void stvk_vertex_terms(v3 p0, v3 p1, v3 p2, const float Dm[4], float ar
 #pragma unroll
     for (int i = 0; i < 6; ++i) H[i] = Hv[i] * area;
 }
+
+// E(x - dx) - E(x) of a Neo-Hookean tet for a move of the vertex with shape
+// gradient gradN, the other vertices fixed: the elastic part of the
+// trust-region ratio (scene option AVBD_SCENE_TRUST_REGION; the reference
+// keeps no energy values).  dF = -dx gradN^T is rank one, so
+// J' = J - dx . (cof F gradN) exactly and log J' - log J = log1p(dJ / J); the
+// change is formed term by term, V0 [mu (F:dF + |dF|^2 / 2) - mu dlog +
+// lam dlog (log J + dlog / 2)], not as a difference of two large energies.
+// The energy is the one whose gradient neohookean_vertex_terms uses
+// (mu/2 (tr F^T F - 3) - mu log J + lam/2 log^2 J; for J <= 1e-6 the
+// penalty (mu + lam)(1e-6 - J)^2 / 2 of neohookeanfem.cpp:86-110); a move
+// across that branch takes the two energies' difference.
+AVBD_DEV float neohookean_energy_change(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmInv, float V0, float mu, float lam,
+                                        v3 gradN, v3 dx) {
+#pragma clang fp contract(on)
+    m3 Ds;
+    const v3 e1 = sub(p1, p0), e2 = sub(p2, p0), e3 = sub(p3, p0);
+    Ds.m[0][0] = e1.x; Ds.m[1][0] = e1.y; Ds.m[2][0] = e1.z;
+    Ds.m[0][1] = e2.x; Ds.m[1][1] = e2.y; Ds.m[2][1] = e2.z;
+    Ds.m[0][2] = e3.x; Ds.m[1][2] = e3.y; Ds.m[2][2] = e3.z;
+    const m3 F = fc::mm(Ds, DmInv);
+    const m3 cof = fc::cofactor(F);
+    const float J = fc::det_cof(F, cof);
+    const float dJ = -fc::dot(dx, fc::mv(cof, gradN));
+    const float Jn = J + dJ;
+    const float FdF = -fc::dot(dx, fc::mv(F, gradN));
+    const float dF2 = fc::dot(dx, dx) * fc::dot(gradN, gradN);
+    const float fl = 1e-6f;
+    if (J > fl && Jn > fl) {
+        const float logJ = logf(J);
+        const float dl = log1pf(dJ / J);
+        return V0 * ((mu * (FdF + 0.5f * dF2) - mu * dl) + (lam * dl) * (logJ + 0.5f * dl));
+    }
+    float trC = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) trC = trC + F.m[i][j] * F.m[i][j];
+    auto psi = [&](float tc, float Jv) {
+        if (Jv <= fl) return 0.5f * (mu + lam) * ((fl - Jv) * (fl - Jv));
+        const float l = logf(Jv);
+        return (0.5f * mu * (tc - 3.f) - mu * l) + 0.5f * lam * (l * l);
+    };
+    return V0 * (psi((trC + 2.f * FdF) + dF2, Jn) - psi(trC, J));
+}
+
+// The same for a StVK membrane triangle (energy mu E:E + lam/2 tr(E)^2,
+// E = (F^T F - I) / 2, the one stvk_vertex_terms differentiates, and
+// (mu + lam)|F|^2 / 2 when the area collapses, stvkfem.cpp:76-86): with
+// dF = -dx gn^T, dE = (F^T dF + dF^T F + dF^T dF) / 2.  local: the moved
+// vertex's slot (0-2).
+AVBD_DEV float stvk_energy_change(v3 p0, v3 p1, v3 p2, const float Dm[4], float area, float mu, float lam, int local,
+                                  float gn0, float gn1, v3 dx) {
+#pragma clang fp contract(on)
+    const v3 e0 = sub(p1, p0), e1 = sub(p2, p0);
+    const v3 f0 = add(mul(e0, Dm[0]), mul(e1, Dm[2]));
+    const v3 f1 = add(mul(e0, Dm[1]), mul(e1, Dm[3]));
+    const v3 q0 = local == 0 ? sub(p0, dx) : p0, q1 = local == 1 ? sub(p1, dx) : p1, q2 = local == 2 ? sub(p2, dx) : p2;
+    const v3 n0 = sub(q1, q0), n1 = sub(q2, q0);
+    const float lim = (2.f * area) * 0.01f;
+    const bool degOld = norm(cross(e0, e1)) < lim, degNew = norm(cross(n0, n1)) < lim;
+    const float a0 = fc::dot(f0, dx), a1 = fc::dot(f1, dx), d2 = fc::dot(dx, dx);
+    const float E00 = 0.5f * (fc::dot(f0, f0) - 1.f), E01 = 0.5f * fc::dot(f0, f1), E11 = 0.5f * (fc::dot(f1, f1) - 1.f);
+    const float dE00 = 0.5f * (d2 * (gn0 * gn0)) - a0 * gn0;
+    const float dE01 = 0.5f * ((d2 * (gn0 * gn1) - a0 * gn1) - a1 * gn0);
+    const float dE11 = 0.5f * (d2 * (gn1 * gn1)) - a1 * gn1;
+    if (!degOld && !degNew) {
+        const float tr = E00 + E11, dtr = dE00 + dE11;
+        const float dEE = 2.f * ((E00 * dE00 + 2.f * (E01 * dE01)) + E11 * dE11) +
+                          ((dE00 * dE00 + 2.f * (dE01 * dE01)) + dE11 * dE11);
+        return area * (mu * dEE + 0.5f * lam * (2.f * tr * dtr + dtr * dtr));
+    }
+    auto psi = [&](bool deg, float a, float b, float c) {
+        if (deg) return 0.5f * (mu + lam) * ((2.f * a + 1.f) + (2.f * c + 1.f));
+        const float t = a + c;
+        return mu * ((a * a + 2.f * (b * b)) + c * c) + 0.5f * lam * (t * t);
+    };
+    return area * (psi(degNew, E00 + dE00, E01 + dE01, E11 + dE11) - psi(degOld, E00, E01, E11));
+}

@@ -146,6 +146,8 @@ struct avbd_ctx {
     int max_scene_bodies = 0;
     std::vector<char> scene_breakable;  // per scene: a joint or spring with a finite fracture
     std::vector<char> fused_on;         // per scene: ST_FUSED as last written
+    std::vector<char> scene_trust;      // per scene: ADAPTIVE with AVBD_SCENE_TRUST_REGION
+    bool any_trust = false;
     std::vector<int> fusable_ids;
     int* fused_list = nullptr;          // device copy of fusable_ids
     std::vector<void*> allocs;
@@ -530,7 +532,10 @@ int sync_fused(avbd_ctx* c) {
     const int S = w.S;
     const bool on = c->use_fused && fused_lanes(c) > 0;
     std::vector<char> want(S, 0);
-    for (int s = 0; s < S; ++s) want[s] = on && !c->scene_breakable[s] && c->scene_bodies[s] <= c->fused_max;
+    // scenes updating a trust-region ratio need its kernel between sweeps
+    for (int s = 0; s < S; ++s)
+        want[s] = on && !c->scene_breakable[s] && !(s < (int)c->scene_trust.size() && c->scene_trust[s]) &&
+                  c->scene_bodies[s] <= c->fused_max;
     if (want == c->fused_on) return 0;
     std::vector<int> st;
     int rc;
@@ -603,6 +608,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
 #define A(p, n) if ((rc = c->alloc(p, n))) return rc
     A(d_sc, S);
     A(&w.st, (size_t)S * ST_WORDS);
+    A(&w.rho, S); A(&w.rho_acc, (size_t)3 * S);
     A(&c->fused_list, S);
     A(d_off, S + 1);
     w.sc = *d_sc;
@@ -874,6 +880,20 @@ int do_upload(avbd_ctx* c, const avbd_world* in, int64_t mcap_req) {
     if ((rc = upload_vec(c, d_sc, c->scenes))) return rc;
     if ((rc = upload_vec(c, w.st, st))) return rc;
     if ((rc = upload_vec(c, d_off, off))) return rc;
+    {
+        // trust-region state: rho per scene, the sweep's two sums cleared
+        std::vector<float> rho(S);
+        c->scene_trust.assign(S, 0);
+        c->any_trust = false;
+        for (int s = 0; s < S; ++s) {
+            rho[s] = in->scenes[s].trust_region_rho;
+            c->scene_trust[s] = in->scenes[s].projection_mode == AVBD_PROJ_ADAPTIVE &&
+                                (in->scenes[s].options & AVBD_SCENE_TRUST_REGION) != 0;
+            c->any_trust = c->any_trust || c->scene_trust[s];
+        }
+        if ((rc = upload_vec(c, w.rho, rho)) || (rc = upload_vec(c, w.rho_acc, std::vector<double>((size_t)3 * S, 0.0))))
+            return rc;
+    }
     {
         std::vector<float4> pos(N), rot(N), scale(N), vel(N), av(N), pv(N), pav(N), inert(N);
         std::vector<uint32_t> flags(N);
@@ -1340,6 +1360,8 @@ int launch_iterations(avbd_ctx* c, const int* off, int ncol, int max_iters, int 
     hipStream_t st = c->stream;
     for (int it = 0; it < max_iters; ++it) {
         for (int col = 0; col < ncol; ++col) launch_colour(c, off, col, it, 0);
+        // step 5.2 (solver.cpp:451-455): the sweep's trust-region ratio
+        if (c->any_trust) hipLaunchKernelGGL(k_rho_update, dim3(grid_for(w.S)), dim3(256), 0, st, w);
         const int mark = c->mark_begin(avbd_ctx::K_DUAL);
         hipLaunchKernelGGL(k_dual, dim3(dgrid), dim3(256), 0, st, w, it);
         c->mark_end(mark);
@@ -2113,11 +2135,14 @@ int avbd_download(avbd_ctx* c, avbd_world* out) {
     if ((rc = download_vec(c, tv, w.t_v, w.T)) || (rc = download_vec(c, td, w.t_d, (size_t)w.T * 3)) ||
         (rc = download_vec(c, rv, w.r_v, w.TR)) || (rc = download_vec(c, rd, w.r_d, (size_t)w.TR * 2)))
         return rc;
+    std::vector<float> rho;
+    if ((rc = download_vec(c, rho, w.rho, w.S))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (int s = 0; s < w.S; ++s) {
         out->scenes[s] = c->scenes[s];
         out->scenes[s].emergency_stop = st[s * ST_WORDS + ST_EMERGENCY];
         out->scenes[s].n_removed = st[s * ST_WORDS + ST_REMOVED];
+        out->scenes[s].trust_region_rho = rho[s];
     }
     auto put3 = [](float* d, float4 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; };
     const std::vector<int> live = live_index(c);
@@ -2224,6 +2249,14 @@ int avbd_get_status(avbd_ctx* c, int32_t* emergency, int32_t* removed, int32_t n
         if (emergency) emergency[s] = st[s * ST_WORDS + ST_EMERGENCY];
         if (removed) removed[s] = st[s * ST_WORDS + ST_REMOVED];
     }
+    return 0;
+}
+
+int avbd_get_trust_region_rho(avbd_ctx* c, float* rho, int32_t n) {
+    if (!c || (!rho && n > 0)) return fail(AVBD_E_ARG, "null argument");
+    if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
+    const int k = std::max(0, std::min(n, c->w.S));
+    HIP_TRY(hipMemcpy(rho, c->w.rho, (size_t)k * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }
 

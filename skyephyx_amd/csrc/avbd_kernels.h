@@ -33,6 +33,7 @@ __global__ void k_primal_canon(DevWorld w, int bucket, int iter, int poststab);
 template <int G>
 __global__ void k_scene_step(DevWorld w, const int* scenes, int mode);
 __global__ void k_dual(DevWorld w, int iter);
+__global__ void k_rho_update(DevWorld w);
 __global__ void k_fracture_reset(DevWorld w);
 __global__ void k_velocity(DevWorld w);
 __global__ void k_stick(DevWorld w);

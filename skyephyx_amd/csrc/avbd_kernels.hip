@@ -28,6 +28,13 @@ AVBD_DEV bool scene_runs(const DevWorld& w, int s) {
     return st[ST_ACTIVE] != 0 && st[ST_EMERGENCY] == 0;
 }
 AVBD_DEV bool is_static(const DevWorld& w, int b) { return (w.flags[b] & AVBD_BODY_STATIC) != 0; }
+// The projection a scene's energies use in this sweep: ADAPTIVE resolves by
+// the scene's trust-region ratio, CLAMP when |rho - 1| <= 0.01, else
+// ABSOLUTE (ProjectEigenvalues, math.cpp:152-160).
+AVBD_DEV int proj_mode(const DevWorld& w, int s, int mode) {
+    if (mode != AVBD_PROJ_ADAPTIVE) return mode;
+    return fabsf(w.rho[s] - 1.f) > 0.01f ? AVBD_PROJ_ABSOLUTE : AVBD_PROJ_CLAMP;
+}
 // The scene's phases 4-7 run in one workgroup of k_scene_step instead of the
 // per-colour bucket launches.
 AVBD_DEV bool scene_fused(const DevWorld& w, int s) { return w.st[s * ST_WORDS + ST_FUSED] != 0; }
@@ -314,7 +321,9 @@ __global__ void k_manifold_init(DevWorld w) {
         const v3 pA = f3(w.pos[a]), pB = f3(w.pos[b]);
         const qt qA = fq(w.rot[a]), qB = fq(w.rot[b]);
         DevCollision col;
-        collide_boxes(pA, qA, f3(w.scale[a]), pB, qB, f3(w.scale[b]), col);
+        const bool spheres = (sc.options & AVBD_SCENE_SPHERE_COLLISION) != 0;
+        collide_bodies(pA, qA, f3(w.scale[a]), spheres && (w.flags[a] & AVBD_BODY_SPHERE), pB, qB, f3(w.scale[b]),
+                       spheres && (w.flags[b] & AVBD_BODY_SPHERE), col);
         float4* hot = w.m.hot + (size_t)m * 8 * CONTACT_HOT_F4;
         float4* cold = w.m.cold + (size_t)m * 8 * CONTACT_COLD_F4;
         const int oldN = w.m.n[m];
@@ -1378,6 +1387,28 @@ AVBD_DEV void energy_terms(const DevWorld& w, uint32_t e, int k, int projMode, v
     elem_terms(x, p, projMode, g, H);
 }
 
+// E(x - dx) - E(x) of an element for the move of the entry's vertex.
+AVBD_DEV float elem_energy_change(const ElemIn& x, const ElemPos& p, v3 dx) {
+    const int local = EENT_LOCAL(x.e);
+    if (!EENT_TRI(x.e)) {
+        const float4 d0 = x.a, d1 = x.b, d2 = x.c;
+        m3 Dm;
+        Dm.m[0][0] = d0.x; Dm.m[0][1] = d0.y; Dm.m[0][2] = d0.z;
+        Dm.m[1][0] = d0.w; Dm.m[1][1] = d1.x; Dm.m[1][2] = d1.y;
+        Dm.m[2][0] = d1.z; Dm.m[2][1] = d1.w; Dm.m[2][2] = d2.x;
+        const v3 g1 = row(Dm, 0), g2 = row(Dm, 1), g3 = row(Dm, 2);
+        const v3 gN = local == 1 ? g1 : (local == 2 ? g2 : (local == 3 ? g3 : sub(sub(neg(g1), g2), g3)));
+        return neohookean_energy_change(p.p0, p.p1, p.p2, p.p3, Dm, d2.y, d2.z, d2.w, gN, dx);
+    }
+    const float4 d0 = x.a, d1 = x.b;
+    const float Dm[4] = {d0.x, d0.y, d0.z, d0.w};
+    float gn0, gn1;
+    if (local == 1) { gn0 = Dm[0]; gn1 = Dm[1]; }
+    else if (local == 2) { gn0 = Dm[2]; gn1 = Dm[3]; }
+    else { gn0 = -Dm[0] - Dm[2]; gn1 = -Dm[1] - Dm[3]; }
+    return stvk_energy_change(p.p0, p.p1, p.p2, Dm, d1.x, d1.y, d1.z, local, gn0, gn1, dx);
+}
+
 // One primal block update per body of colour `c` (solver.cpp:392-448 and
 // 540-595).  G lanes cooperate on one body: each lane accumulates a strided
 // share of the body's incidence entries, the partial systems are summed with
@@ -1436,6 +1467,45 @@ AVBD_DEV void fem_vertex_solve(const DevWorld& w, int b, v3 p, float L[6], float
     fem_vertex_solve(w, b, p, L, r, w.rot[b], w.last_rot[b], w.last_pos[b]);
 }
 
+// The trust-region terms of one FEM-only block update (scene option
+// AVBD_SCENE_TRUST_REGION under ADAPTIVE projection): every lane of the
+// group solves the block (L, r: the group's sums) as lane 0 does, sums the
+// energy change of its share of the elements for that move, and lane 0 adds
+// the inertia term's change and the decrease r . dx / 2 the block model
+// predicts to the scene's accumulators (k_rho_update turns them into rho).
+template <int G>
+AVBD_DEV void fem_trust_terms(const DevWorld& w, int s, int lane, v3 p, v3 ipos, float mass, float dt2, const float L[6],
+                              const float r[3], int e0, int e1, const float4* nbp) {
+    float dxa[3];
+    ldlt3_solve(L, r, dxa);
+    const v3 d{dxa[0], dxa[1], dxa[2]};
+    float dE = 0.f, dS = 0.f;  // the elements' change and its terms' summed magnitude
+    for (int k = e0 + lane; k < e1; k += G) {
+        ElemIn x;
+        elem_load(w, k, x);
+        ElemPos q;
+        if (nbp) elem_pos_lds(nbp, w.ev_l[k], p, q);
+        else elem_gather(w, x.v, q);
+        const float de = elem_energy_change(x, q, d);
+        dE = dE + de;
+        dS = dS + fabsf(de);
+    }
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        dE = dE + __shfl_xor(dE, off, G);
+        dS = dS + __shfl_xor(dS, off, G);
+    }
+    if (lane == 0) {
+        const float ml = mass / dt2;
+        const v3 y = sub(p, ipos);
+        const float dEin = (0.5f * ml) * (dot(d, d) - 2.f * dot(d, y));
+        const double pred = 0.5 * (((double)r[0] * (double)dxa[0] + (double)r[1] * (double)dxa[1]) + (double)r[2] * (double)dxa[2]);
+        atomicAdd(&w.rho_acc[3 * s], -(double)(dEin + dE));
+        atomicAdd(&w.rho_acc[3 * s + 1], pred);
+        atomicAdd(&w.rho_acc[3 * s + 2], (double)(fabsf(dEin) + dS));
+    }
+}
+
 template <int G>
 AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, int poststab, float4* nbp = nullptr) {
     // What the sweep needs of b is loaded at once, before the scene's run
@@ -1462,10 +1532,13 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
     // the scene's run test from words loaded together (no load waits on
     // the outcome of another)
     const int st_em = w.st[s * ST_WORDS + ST_EMERGENCY], st_oob = w.st[s * ST_WORDS + ST_OOB];
-    const int post = sc.post_stabilization, iters = sc.iterations;
+    const int post = sc.post_stabilization, iters = sc.iterations, pmode = sc.projection_mode;
+    const uint32_t opts = sc.options;
     const float dt = sc.dt;
     const bool run = (st_em == 0) & (poststab ? (post != 0) & (st_oob == 0) : iter < iters);
     if (!run) return;
+    const int pm = proj_mode(w, s, pmode);
+    const bool trust = !poststab && pmode == AVBD_PROJ_ADAPTIVE && (opts & AVBD_SCENE_TRUST_REGION);
     const float dt2 = dt * dt;
     float L[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float r[3] = {0.f, 0.f, 0.f};
@@ -1525,7 +1598,7 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
                 elem_pos_lds(nbp, l0, p, p0);
                 v3 g;
                 float H[6];
-                elem_terms(in0, p0, sc.projection_mode, g, H);
+                elem_terms(in0, p0, pm, g, H);
                 r[0] = r[0] + g.x;
                 r[1] = r[1] + g.y;
                 r[2] = r[2] + g.z;
@@ -1562,7 +1635,7 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
             elem_gather(w, ids1, p1);
             v3 g;
             float H[6];
-            elem_terms(in0, p0, sc.projection_mode, g, H);
+            elem_terms(in0, p0, pm, g, H);
             r[0] = r[0] + g.x;
             r[1] = r[1] + g.y;
             r[2] = r[2] + g.z;
@@ -1585,6 +1658,7 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
             for (int i = 0; i < 3; ++i) r[i] = r[i] + __shfl_xor(r[i], off, G);
         }
     }
+    if (trust) fem_trust_terms<G>(w, s, lane, p, f3(ipos), vel.w, dt2, L, r, e0, e1, (G >= 4 && nn >= 0) ? nbp : nullptr);
     if (lane == 0) fem_vertex_solve(w, b, p, L, r);
 }
 
@@ -1632,6 +1706,7 @@ AVBD_DEV void primal_fem_vertex_canon(const DevWorld& w, int b, int lane, int it
     if (w.st[s * ST_WORDS + ST_EMERGENCY]) return;
     if (poststab ? (!sc.post_stabilization || w.st[s * ST_WORDS + ST_OOB]) : iter >= sc.iterations) return;
     const float dt2 = sc.dt * sc.dt;
+    const int pm = proj_mode(w, s, sc.projection_mode);
     const v3 p = f3(w.pos[b]);
     const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
     const int last = max(e1 - 1, 0);
@@ -1663,7 +1738,7 @@ AVBD_DEV void primal_fem_vertex_canon(const DevWorld& w, int b, int lane, int it
         if (k < e1) {
             v3 g;
             float H[6];
-            elem_terms(in0, p0, sc.projection_mode, g, H);
+            elem_terms(in0, p0, pm, g, H);
             cur.r[0] = cur.r[0] + g.x;
             cur.r[1] = cur.r[1] + g.y;
             cur.r[2] = cur.r[2] + g.z;
@@ -1675,7 +1750,7 @@ AVBD_DEV void primal_fem_vertex_canon(const DevWorld& w, int b, int lane, int it
                 elem_load(w, k2, x);
                 ElemPos q;
                 elem_gather(w, x.v, q);
-                elem_terms(x, q, sc.projection_mode, g, H);
+                elem_terms(x, q, pm, g, H);
                 cur.r[0] = cur.r[0] + g.x;
                 cur.r[1] = cur.r[1] + g.y;
                 cur.r[2] = cur.r[2] + g.z;
@@ -1710,6 +1785,8 @@ AVBD_DEV void primal_fem_vertex_canon(const DevWorld& w, int b, int lane, int it
 #pragma unroll
         for (int i = 0; i < 3; ++i) cur.r[i] = cur.r[i] + __shfl_xor(cur.r[i], off, G);
     }
+    if (!poststab && sc.projection_mode == AVBD_PROJ_ADAPTIVE && (sc.options & AVBD_SCENE_TRUST_REGION))
+        fem_trust_terms<G>(w, s, lane, p, f3(w.inert_pos[b]), w.vel[b].w, dt2, cur.L, cur.r, e0, e1, nullptr);
     if (lane == 0) fem_vertex_solve(w, b, p, cur.L, cur.r);
 }
 
@@ -1802,7 +1879,8 @@ AVBD_DEV void primal_rigid(const DevWorld& w, int b, bool valid, int iter, int p
             for (int k = f0 + lane; k < f1; k += G) accumulate_force(w, w.f_ent[k], alpha, dSelf, A);
         }
         const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
-        for (int k = e0 + lane; k < e1; k += G) accumulate_energy(w, k, sc.projection_mode, A);
+        const int pm = proj_mode(w, s, sc.projection_mode);
+        for (int k = e0 + lane; k < e1; k += G) accumulate_energy(w, k, pm, A);
         if (G > 1) {
 #pragma unroll
             for (int off = 1; off < G; off <<= 1) {
@@ -2009,6 +2087,23 @@ __global__ void k_dual(DevWorld w, int iter) {
 
 // Clear lambda and penalty of every manifold touching a fractured force's
 // bodies (Force::ResetValues via solver.cpp:473-478), then clear the queue.
+// Step 5.2 (solver.cpp:451-455, empty in the reference) for the scenes
+// with AVBD_SCENE_TRUST_REGION: rho = actual / predicted decrease of the
+// sweep that just ended, then the sums restart.  Both sums cancel terms of
+// the summed magnitude acc[2]; a sweep whose predicted decrease is within
+// 3e-5 of it (a nearly converged solve: its ratio is FP32 rounding noise)
+// and a scene with no FEM-only update keep their rho.
+__global__ void k_rho_update(DevWorld w) {
+    GRID_STRIDE(s, w.S) {
+        double* acc = w.rho_acc + 3 * (size_t)s;
+        const double pred = acc[1];
+        if (pred > 3e-5 * acc[2]) w.rho[s] = (float)(acc[0] / pred);
+        acc[0] = 0.0;
+        acc[1] = 0.0;
+        acc[2] = 0.0;
+    }
+}
+
 __global__ void k_fracture_reset(DevWorld w) {
     const int nf = min(w.g[G_FRACTURES], 2 * (w.J + w.SP) + 2);
     if (nf == 0) return;

@@ -97,6 +97,8 @@ struct DevWorld {
     int S;
     const avbd_scene* sc;  // params (device copy)
     int* st;               // [S * ST_WORDS]
+    float* rho;            // [S] trustRegionRho (solver.hpp:80)
+    double* rho_acc;       // [3S] the current sweep's actual and predicted decrease and their terms' magnitude (AVBD_SCENE_TRUST_REGION)
     const int* sc_off;     // [S+1] body offsets
     // bodies
     int N;

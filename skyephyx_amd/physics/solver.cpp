@@ -255,6 +255,8 @@ void Solver::SceneRecord(std::vector<unsigned char>& out) const {
     put3(s.gravity, gravity);
     put3(s.bounds_min, boundsMin);
     put3(s.bounds_max, boundsMax);
+    s.options = (trustRegionUpdate ? AVBD_SCENE_TRUST_REGION : 0u) | (sphereCollision ? AVBD_SCENE_SPHERE_COLLISION : 0u);
+    s.trust_region_rho = trustRegionRho;
     out.assign(reinterpret_cast<const unsigned char*>(&s), reinterpret_cast<const unsigned char*>(&s) + sizeof(s));
 }
 
@@ -275,7 +277,8 @@ void Solver::BodyRecords(std::vector<avbd_body>& out) const {
         for (int k = 0; k < 3; ++k) b.inertia[k] = m.isStatic ? 0.f : m.inertiaTensorBody(k, k);
         b.friction = m.friction;
         b.radius = m.detectionRadius;
-        b.flags = (m.isStatic ? AVBD_BODY_STATIC : 0u) | (m.isParticle ? AVBD_BODY_PARTICLE : 0u);
+        b.flags = (m.isStatic ? AVBD_BODY_STATIC : 0u) | (m.isParticle ? AVBD_BODY_PARTICLE : 0u) |
+                  (m.modelType == ModelType_Sphere ? AVBD_BODY_SPHERE : 0u);
         b.scene = 0;
     }
 }
@@ -394,12 +397,15 @@ void Solver::LoadSnapshot(const std::string& path) {
     boundsMin = get3(sc.bounds_min);
     boundsMax = get3(sc.bounds_max);
     emergencyStop = sc.emergency_stop != 0;
+    trustRegionRho = sc.trust_region_rho;
+    trustRegionUpdate = (sc.options & AVBD_SCENE_TRUST_REGION) != 0;
+    sphereCollision = (sc.options & AVBD_SCENE_SPHERE_COLLISION) != 0;
 
     // The objects take the records' serials (the constructors' serial
     // requests are answered with -1 while mirroring_ is set).
     mirroring_ = true;
     for (const avbd_body& b : r.bodies) {
-        auto mesh = std::make_unique<Mesh>(this, ModelType_Cube);
+        auto mesh = std::make_unique<Mesh>(this, (b.flags & AVBD_BODY_SPHERE) ? ModelType_Sphere : ModelType_Cube);
         Mesh& m = *mesh;
         m.transform.SetPosition(get3(b.pos));
         m.transform.SetRotation(Quaternionf(b.rot[0], b.rot[1], b.rot[2], b.rot[3]));
@@ -730,6 +736,8 @@ void Solver::Step() {
     }
     int32_t em = 0, removed = 0;
     check(avbd_get_status(ctx_, &em, &removed, 1), "avbd_get_status");
+    if (trustRegionUpdate && projectionMode == EigenProjectionMode::ADAPTIVE)
+        check(avbd_get_trust_region_rho(ctx_, &trustRegionRho, 1), "avbd_get_trust_region_rho");
     emergencyStop = em != 0;
     ApplyRemovals();
     ReadBack();

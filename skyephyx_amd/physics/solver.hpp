@@ -105,6 +105,14 @@ public:
     float stepValue = 1.f / 60.f;
     EigenProjectionMode projectionMode = EigenProjectionMode::ABSOLUTE;
     float trustRegionRho = 0.f;
+    // The two features the reference declares but leaves unfinished, off by
+    // default (the reference's behaviour; not reference members):
+    // trustRegionUpdate updates trustRegionRho after every primal sweep under
+    // ADAPTIVE projection (the empty step 5.2, solver.cpp:451-455);
+    // sphereCollision makes ModelType_Sphere bodies collide as spheres
+    // (the reference collides every pair as boxes, manifold.cpp:57).
+    bool trustRegionUpdate = false;
+    bool sphereCollision = false;
     float prevTotalEnergy = 0.f;
     float trustRegionThreshold = 0.01f;
 
@@ -133,7 +141,7 @@ public:
     // throws std::runtime_error with the library's message on failure.
     void SaveSnapshot(const std::string& path);
     // Replaces the world with a one-scene snapshot file (avbd_snapshot_read):
-    // scene parameters, bodies (Mesh::modelType is not recorded: cubes),
+    // scene parameters, bodies (Mesh::modelType recorded as sphere or cube),
     // joints, springs, contact manifolds with their row state, and energies,
     // each keeping its creation serial, so that ExportWorld gives the file's
     // records back.  The next Step uploads the world.  Throws

@@ -17,6 +17,8 @@ from .world import (
     F32,
     INF,
     MODEL_CUBE,
+    MODEL_SPHERE,
+    SCENE_SPHERE_COLLISION,
     World,
     WorldBuilder,
 )
@@ -193,6 +195,43 @@ def box_lattice(shape=(40, 25, 20), spacing=3.5, jitter=0.25, edge=(0.5, 2.0), s
                 continue
             b.add_spring(int(lattice[i, j, k]), (0.0, 0.0, 0.0), int(lattice[i2, j2, k2]), (0.0, 0.0, 0.0), spring_k)
             made += 1
+    return b.build()
+
+
+def sphere_pile(n=24, seed=5, boxes=4, **scene):
+    """Spheres of seeded radii (0.3-0.6 m) dropped on a lattice over the
+    ground among a few boxes, with sphere collision switched on (scene option
+    SCENE_SPHERE_COLLISION: sphere-sphere, sphere-box and sphere-ground
+    contacts).  Without the option the same bodies collide as boxes, as in
+    the reference (manifold.cpp:57)."""
+    scene.setdefault("options", SCENE_SPHERE_COLLISION)
+    rng = np.random.default_rng(seed)
+    b = WorldBuilder(**scene)
+    add_ground(b)
+    total = n + boxes
+    cols = max(2, int(math.ceil(math.sqrt(total / 2.0))))
+    box_slots = set(range(0, total, max(1, total // boxes))[:boxes]) if boxes > 0 else set()
+    for i in range(total):
+        layer, r = divmod(i, cols * cols)
+        ix, iz = r % cols, r // cols
+        pos = ((ix - (cols - 1) * 0.5) * 1.4 + rng.uniform(-0.05, 0.05), -9.5 + 0.7 + layer * 1.4,
+               (iz - (cols - 1) * 0.5) * 1.4 + rng.uniform(-0.05, 0.05))
+        if i in box_slots:
+            b.add_body(MODEL_CUBE, 1.0, 0.5, pos, tuple(rng.uniform(0.5, 1.0, 3)), rotation=tuple(random_quaternion(rng)))
+        else:
+            d = 2.0 * rng.uniform(0.3, 0.6)
+            b.add_body(MODEL_SPHERE, 1.0, 0.5, pos, (d, d, d))
+    return b.build()
+
+
+def sliding_sphere(v0=4.0, radius=0.5, friction=0.5, **scene):
+    """A solid sphere launched along the ground with no spin (sphere
+    collision on): friction turns sliding into rolling at 5/7 v0."""
+    scene.setdefault("options", SCENE_SPHERE_COLLISION)
+    b = WorldBuilder(**scene)
+    b.add_body(MODEL_CUBE, 1.0, friction, (0.0, -10.0, 0.0), (200.0, 1.0, 200.0), is_static=True)
+    d = 2.0 * radius
+    b.add_body(MODEL_SPHERE, 1.0, friction, (-40.0, -9.5 + radius - 1e-3, 0.0), (d, d, d), velocity=(v0, 0.0, 0.0))
     return b.build()
 
 

@@ -22,6 +22,9 @@ F32 = np.float32
 
 BODY_STATIC = 1
 BODY_PARTICLE = 2
+BODY_SPHERE = 4
+SCENE_TRUST_REGION = 1
+SCENE_SPHERE_COLLISION = 2
 FORCE_DISABLED = 1
 PROJ_CLAMP, PROJ_ABSOLUTE, PROJ_ADAPTIVE = 0, 1, 2
 
@@ -42,6 +45,8 @@ SCENE_DTYPE = np.dtype(
         ("bounds_max", "<f4", (3,)),
         ("emergency_stop", "<i4"),
         ("n_removed", "<i4"),
+        ("options", "<u4"),
+        ("trust_region_rho", "<f4"),
     ],
     align=True,
 )
@@ -373,6 +378,8 @@ class WorldBuilder:
                 inertia = np.array([k * (sy2 + sz2), k * (sx2 + sz2), k * (sx2 + sy2)], F32)
             b["mass"] = m
             b["inertia"] = inertia
+        if model_type == MODEL_SPHERE:
+            b["flags"] = b["flags"] | BODY_SPHERE
         self._bodies.append(b)
         return len(self._bodies) - 1
 

@@ -37,9 +37,22 @@ def save(path, w: World):
     np.savez_compressed(path, **{k: getattr(w, k) for k in FIELDS})
 
 
+def _current(a, dtype):
+    """Records of an earlier layout in the current dtype, field by field
+    (fields added since are zero: the reference's behaviour)."""
+    if a.dtype == dtype:
+        return a
+    out = np.zeros(a.shape, dtype)
+    for name in a.dtype.names:
+        out[name] = a[name]
+    return out
+
+
 def load(path) -> World:
+    from skyephyx_amd.abi import _DTYPES
+
     with np.load(path, allow_pickle=False) as z:
-        return World(*(z[k] for k in FIELDS))
+        return World(*(_current(z[k], dt) for k, dt in zip(FIELDS, _DTYPES)))
 
 
 def main():

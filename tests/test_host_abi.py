@@ -14,7 +14,7 @@ import ctypes as C
 
 def test_library_exports_every_declared_symbol():
     L = capi.lib()
-    assert L.avbd_abi_version() == 1
+    assert L.avbd_abi_version() == 2
     out = subprocess.run(["nm", "-D", "--defined-only", capi.LIB_PATH], capture_output=True, text=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
     declared = capi.exported_symbols()
