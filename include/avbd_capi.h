@@ -463,6 +463,14 @@ int avbd_get_colours(avbd_ctx* ctx, int32_t* out, int32_t n);
  *                    its own stream beside the bucket launches) when "lanes"
  *                    equals "fem_lanes"; 0 (default) = every scene on the
  *                    per-colour bucket launches.  Results are identical.
+ *   "fused_single_scene" 1 = a world of one such scene runs it in the
+ *                    per-scene kernel as well; 0 (default) = by
+ *                    "fused_scenes" alone.
+ *   "scene_lds_bytes" 0 to 65536 (default 61440): LDS per workgroup of the
+ *                    per-scene kernel; a scene whose body state and incidence
+ *                    lists fit (256 bytes per body plus 4 per incidence
+ *                    entry) runs on an LDS copy of them, others on global
+ *                    memory; 0 = every scene on global memory.
  *   "spare_bodies"   k >= 0 (default 0): the next avbd_upload gives every
  *                    scene k free body slots after its bodies, for
  *                    avbd_add_bodies (bodies added between steps without a

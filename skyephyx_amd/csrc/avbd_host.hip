@@ -146,6 +146,8 @@ struct avbd_ctx {
     int max_scene_bodies = 0;
     std::vector<char> scene_breakable;  // per scene: a joint or spring with a finite fracture
     std::vector<char> fused_on;         // per scene: ST_FUSED as last written
+    bool fuse_single = false;           // option fused_single_scene
+    int scene_lds = 60 * 1024;          // option scene_lds_bytes
     std::vector<char> scene_trust;      // per scene: ADAPTIVE with AVBD_SCENE_TRUST_REGION
     bool any_trust = false;
     std::vector<int> fusable_ids;
@@ -530,7 +532,9 @@ int fused_lanes(const avbd_ctx* c) {
 int sync_fused(avbd_ctx* c) {
     DevWorld& w = c->w;
     const int S = w.S;
-    const bool on = c->use_fused && fused_lanes(c) > 0;
+    // option fused_single_scene: a world of one small scene (the plain
+    // per-scene step) in the per-scene kernel as well
+    const bool on = (c->use_fused || (S == 1 && c->fuse_single)) && fused_lanes(c) > 0;
     std::vector<char> want(S, 0);
     // scenes updating a trust-region ratio need its kernel between sweeps
     for (int s = 0; s < S; ++s)
@@ -560,13 +564,16 @@ void launch_scene_step(avbd_ctx* c, hipStream_t st, int mode) {
     if (n == 0) return;
     const DevWorld& w = c->w;
     const int* list = c->fused_list;
+    // dynamic LDS for a scene's body arrays and incidence lists (scenes
+    // that need more run on global memory; 0: every scene on global memory)
+    const int lds = c->scene_lds;
     switch (fused_lanes(c)) {
-        case 1: hipLaunchKernelGGL((k_scene_step<1>), dim3(n), dim3(256), 0, st, w, list, mode); break;
-        case 2: hipLaunchKernelGGL((k_scene_step<2>), dim3(n), dim3(256), 0, st, w, list, mode); break;
-        case 4: hipLaunchKernelGGL((k_scene_step<4>), dim3(n), dim3(256), 0, st, w, list, mode); break;
-        case 8: hipLaunchKernelGGL((k_scene_step<8>), dim3(n), dim3(256), 0, st, w, list, mode); break;
-        case 16: hipLaunchKernelGGL((k_scene_step<16>), dim3(n), dim3(256), 0, st, w, list, mode); break;
-        case 32: hipLaunchKernelGGL((k_scene_step<32>), dim3(n), dim3(256), 0, st, w, list, mode); break;
+        case 1: hipLaunchKernelGGL((k_scene_step<1>), dim3(n), dim3(256), lds, st, w, list, mode, lds); break;
+        case 2: hipLaunchKernelGGL((k_scene_step<2>), dim3(n), dim3(256), lds, st, w, list, mode, lds); break;
+        case 4: hipLaunchKernelGGL((k_scene_step<4>), dim3(n), dim3(256), lds, st, w, list, mode, lds); break;
+        case 8: hipLaunchKernelGGL((k_scene_step<8>), dim3(n), dim3(256), lds, st, w, list, mode, lds); break;
+        case 16: hipLaunchKernelGGL((k_scene_step<16>), dim3(n), dim3(256), lds, st, w, list, mode, lds); break;
+        case 32: hipLaunchKernelGGL((k_scene_step<32>), dim3(n), dim3(256), lds, st, w, list, mode, lds); break;
         default: break;
     }
 }
@@ -1237,7 +1244,7 @@ int remove_bodies(avbd_ctx* c, const std::vector<char>& aff, std::vector<int>& s
                 const int4 v = d.ev[x];
                 const bool tri = (ent >> 31) != 0;
                 const int m0 = new_index(g, v.x), m1 = new_index(g, v.y), m2 = new_index(g, v.z);
-                const int m3 = tri ? 0 : new_index(g, v.w);
+                const int m3 = tri ? m0 : new_index(g, v.w);  // slot 3 of a triangle repeats slot 0
                 if (m0 < 0 || m1 < 0 || m2 < 0 || m3 < 0) {
                     const uint32_t idx = ent & 0x1FFFFFFFu;
                     if (tri) c->tri_dead[idx] = 1;
@@ -1419,6 +1426,17 @@ int sweep_phases(avbd_ctx* c, const int* off, int ncol, int max_iters, bool any_
     const int N = w.N;
     // the small scenes' whole phases 4-7 on their own stream, beside the buckets
     const bool fused = !c->fusable_ids.empty();
+    if (fused && (int)c->fusable_ids.size() == w.S) {
+        // every scene in the per-scene kernel: the bucket, prediction and
+        // velocity launches would find nothing to do (the phases are not
+        // timed apart: all of it counts as the iterations)
+        HIP_TRY(hipEventRecord(c->ev[3], st));
+        launch_scene_step(c, st, 0);
+        HIP_TRY(hipEventRecord(c->ev[4], st));
+        HIP_TRY(hipEventRecord(c->ev[5], st));
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     if (fused) {
         HIP_TRY(hipEventRecord(c->fork2, st));
         HIP_TRY(hipStreamWaitEvent(c->side2, c->fork2, 0));
@@ -1763,6 +1781,16 @@ int avbd_set_option(avbd_ctx* c, const char* name, int64_t value) {
     }
     if (n == "fused_scenes") {
         c->use_fused = value != 0;
+        return 0;
+    }
+    if (n == "fused_single_scene") {
+        c->fuse_single = value != 0;
+        return 0;
+    }
+    if (n == "scene_lds_bytes") {
+        if (value < 0 || value > 65536) return fail(AVBD_E_ARG, "scene_lds_bytes out of range (0 to 65536)");
+        c->scene_lds = int(value);
+        c->drop_graphs();  // captured per-scene launches carry the size
         return 0;
     }
     if (n == "fused_max_bodies") {

@@ -564,12 +564,46 @@ AVBD_DEV unsigned long long ent_key(const DevWorld& w, uint32_t e) {
     return (s << 3) | c;
 }
 
+// Creation key of entry e with one load through a selected array (no
+// branch around it, so the keys of a list load together).
+AVBD_DEV unsigned long long ent_key_flat(const DevWorld& w, uint32_t e) {
+    const uint32_t idx = ENT_INDEX(e), kind = ENT_KIND(e);
+    const bool contact = kind == ENT_CONTACT;
+    const long long* arr = contact ? w.m.serial : (kind == ENT_JOINT ? w.j_serial : w.s_serial);
+    const unsigned long long sr = (unsigned long long)arr[contact ? (idx >> 3) : idx];
+    return (sr << 3) | (contact ? (idx & 7u) : 0u);
+}
+
 // Order each dynamic body's list by creation (Mesh::forces is append-ordered
-// with stable erase, force.cpp:5-19): insertion sort per body.
+// with stable erase, force.cpp:5-19): a rank sort in registers for short
+// lists, insertion sort for longer ones.
 __global__ void k_inc_sort(DevWorld w) {
+    constexpr int KMAX = 16;
     GRID_STRIDE(b, w.N) {
         if (is_static(w, b)) continue;
         const int lo = w.f_off[b], hi = w.f_off[b + 1];
+        const int n = hi - lo;
+        if (n <= 1) continue;
+        if (n <= KMAX) {
+            // short lists: all entries and keys load at once (indices clamped
+            // into the list), each entry is stored at its rank; equal keys
+            // (serials are the records' own) rank by position, the order the
+            // insertion sort keeps
+            uint32_t e[KMAX];
+            unsigned long long key[KMAX];
+#pragma unroll
+            for (int i = 0; i < KMAX; ++i) e[i] = w.f_ent[lo + min(i, n - 1)];
+#pragma unroll
+            for (int i = 0; i < KMAX; ++i) key[i] = i < n ? ent_key_flat(w, e[i]) : ~0ull;
+#pragma unroll
+            for (int i = 0; i < KMAX; ++i) {
+                int r = 0;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j) r += (key[j] < key[i] || (key[j] == key[i] && j < i)) ? 1 : 0;
+                if (i < n) w.f_ent[lo + r] = e[i];
+            }
+            continue;
+        }
         for (int i = lo + 1; i < hi; ++i) {
             uint32_t e = w.f_ent[i];
             unsigned long long k = ent_key(w, e);
@@ -1348,7 +1382,8 @@ AVBD_DEV void elem_pos(const DevWorld& w, const ElemIn& x, ElemPos& p) {
 }
 
 // All four vertex positions of an element, unconditionally (a triangle's
-// slot 3 holds id 0 and is not used).
+// slot 3 repeats its slot 0 id, so every gather stays inside the element's
+// scene, and is not used).
 AVBD_DEV void elem_gather(const DevWorld& w, int4 v, ElemPos& p) {
     p.p0 = f3(w.pos[v.x]);
     p.p1 = f3(w.pos[v.y]);
@@ -1553,7 +1588,7 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
     // element k is evaluated, the next element's stream record and vertex
     // positions and the one after's vertex ids are in flight.  Loads are
     // unconditional (indices clamped to the list; a triangle's slot 3 id is
-    // 0) so that none sits under a branch, which would make every wait
+    // its slot 0 id) so that none sits under a branch, which would make every wait
     // drain all loads in flight.
     // With an LDS slot (bucket sweeps, G >= 4): the group loads the vertex's
     // distinct neighbours once and each element reads its positions from
@@ -2219,10 +2254,9 @@ AVBD_DEV int sweep_kind(const DevWorld& w, int b, int col) {
 // entry, a joint or spring from its body B's.  Eligible scenes have no breakable joint or spring (a
 // fracture resets other bodies' contacts through the global queue).
 // mode 1: post-stabilisation only (after the bounds removal).
+// Phases 4-7 of one scene in one workgroup (see k_scene_step).
 template <int G>
-__global__ __launch_bounds__(256) void k_scene_step(DevWorld w, const int* scenes, int mode) {
-    const int s = scenes[blockIdx.x];
-    if (!scene_runs(w, s) || !scene_fused(w, s)) return;
+AVBD_DEV void scene_step_body(const DevWorld& w, int s, int mode) {
     const avbd_scene& sc = w.sc[s];
     const int a = w.sc_off[s], e = w.sc_off[s + 1];
     const int ncol = w.st[s * ST_WORDS + ST_NCOL];
@@ -2276,12 +2310,86 @@ __global__ __launch_bounds__(256) void k_scene_step(DevWorld w, const int* scene
     });
 }
 
-template __global__ void k_scene_step<1>(DevWorld, const int*, int);
-template __global__ void k_scene_step<2>(DevWorld, const int*, int);
-template __global__ void k_scene_step<4>(DevWorld, const int*, int);
-template __global__ void k_scene_step<8>(DevWorld, const int*, int);
-template __global__ void k_scene_step<16>(DevWorld, const int*, int);
-template __global__ void k_scene_step<32>(DevWorld, const int*, int);
+// `first` rebased so that p[first + i] is lds[i]: the address arithmetic is
+// done on integers, and loads through the result are flat accesses that the
+// hardware routes to LDS.
+template <class T>
+AVBD_DEV T* lds_alias(T* lds, int first) {
+    return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(lds) - (uintptr_t)first * sizeof(T));
+}
+
+// The per-scene kernel: one workgroup runs a scene's phases 4-7 with
+// workgroup barriers between colours instead of launch boundaries.  A
+// scene whose bodies and incidence lists fit in lds_bytes works on an LDS
+// copy of them: the per-body and per-force functions run unchanged on a
+// world whose body arrays and incidence range point into LDS (contact,
+// joint and spring records stay in global memory), so each colour's
+// dependent loads are LDS latencies; the bodies' state is written back at
+// the end.  Every force of a scene joins bodies of that scene, so no access
+// leaves the copied ranges.
+template <int G>
+__global__ __launch_bounds__(256) void k_scene_step(DevWorld w, const int* scenes, int mode, int lds_bytes) {
+    extern __shared__ float4 scene_lds[];
+    const int s = scenes[blockIdx.x];
+    if (!scene_runs(w, s) || !scene_fused(w, s)) return;
+    const int a = w.sc_off[s], e = w.sc_off[s + 1];
+    const int nb = e - a;
+    const int q0 = w.f_off[a], nf = w.f_off[e] - q0;
+    constexpr int kF4 = 15;  // float4 body arrays copied
+    const size_t need = (size_t)nb * (kF4 * sizeof(float4) + 3 * sizeof(int)) + (size_t)(nb + 1) * sizeof(int) +
+                        (size_t)nf * sizeof(uint32_t);
+    if (need > (size_t)lds_bytes) {
+        scene_step_body<G>(w, s, mode);
+        return;
+    }
+    DevWorld l = w;
+    float4* const* src4[kF4] = {&w.pos, &w.rot, &w.diff0, &w.diff1, &w.vel, &w.angvel, &w.pvel, &w.pangvel,
+                                &w.last_pos, &w.last_rot, &w.inert_pos, &w.inert_rot, &w.drag, &w.scale, &w.inertia};
+    float4** dst4[kF4] = {&l.pos, &l.rot, &l.diff0, &l.diff1, &l.vel, &l.angvel, &l.pvel, &l.pangvel,
+                          &l.last_pos, &l.last_rot, &l.inert_pos, &l.inert_rot, &l.drag, &l.scale, &l.inertia};
+    const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+#pragma unroll
+    for (int f = 0; f < kF4; ++f) {
+        float4* dst = scene_lds + (size_t)f * nb;
+        const float4* src = *src4[f];
+        for (int i = tid; i < nb; i += nt) dst[i] = src[a + i];
+        *dst4[f] = lds_alias(dst, a);
+    }
+    uint32_t* lflags = reinterpret_cast<uint32_t*>(scene_lds + (size_t)kF4 * nb);
+    int* lscene = reinterpret_cast<int*>(lflags + nb);
+    int* lcolour = lscene + nb;
+    int* lfoff = lcolour + nb;
+    uint32_t* lfent = reinterpret_cast<uint32_t*>(lfoff + nb + 1);
+    for (int i = tid; i < nb; i += nt) {
+        lflags[i] = w.flags[a + i];
+        lscene[i] = w.scene[a + i];
+        lcolour[i] = w.colour[a + i];
+    }
+    for (int i = tid; i <= nb; i += nt) lfoff[i] = w.f_off[a + i];
+    for (int i = tid; i < nf; i += nt) lfent[i] = w.f_ent[q0 + i];
+    l.flags = lds_alias(lflags, a);
+    l.scene = lds_alias(lscene, a);
+    l.colour = lds_alias(lcolour, a);
+    l.f_off = lds_alias(lfoff, a);
+    l.f_ent = lds_alias(lfent, q0);
+    __syncthreads();
+    scene_step_body<G>(l, s, mode);
+    __syncthreads();
+    // the state the phases write (scale and inertia are read only)
+#pragma unroll
+    for (int f = 0; f < kF4 - 2; ++f) {
+        const float4* src = scene_lds + (size_t)f * nb;
+        float4* dst = *src4[f];
+        for (int i = tid; i < nb; i += nt) dst[a + i] = src[i];
+    }
+}
+
+template __global__ void k_scene_step<1>(DevWorld, const int*, int, int);
+template __global__ void k_scene_step<2>(DevWorld, const int*, int, int);
+template __global__ void k_scene_step<4>(DevWorld, const int*, int, int);
+template __global__ void k_scene_step<8>(DevWorld, const int*, int, int);
+template __global__ void k_scene_step<16>(DevWorld, const int*, int, int);
+template __global__ void k_scene_step<32>(DevWorld, const int*, int, int);
 
 // ------------------------------------------------ out-of-bounds removal
 // Solver::RemoveBody for the bodies that left the world bounds
@@ -2361,7 +2469,9 @@ __global__ void k_element_stream(DevWorld w, int n) {
             w.ev_a[k] = w.r_d[(size_t)idx * 2 + 0];
             w.ev_b[k] = w.r_d[(size_t)idx * 2 + 1];
             w.ev_c[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            w.ev_v[k] = w.r_v[idx];
+            int4 v = w.r_v[idx];
+            v.w = v.x;  // the unused slot 3 repeats a vertex of the element
+            w.ev_v[k] = v;
         } else {
             w.ev_a[k] = w.t_d[(size_t)idx * 3 + 0];
             w.ev_b[k] = w.t_d[(size_t)idx * 3 + 1];

@@ -2,15 +2,18 @@
 iterations, velocities, post-stabilisation) in one workgroup each instead of
 the per-colour bucket launches.  Same per-body and per-force functions in
 the same order, so every step must be bit-identical to the bucket path
-(option fused_scenes 1 against 0), also for bounds removal (post-stabilisation after
-the removal runs in the same kernel) and in worlds that mix small scenes,
-large scenes and scenes with breakable forces (which stay on the buckets).
+(option fused_scenes 1 against 0, with the single-scene default off), also
+for bounds removal (post-stabilisation after the removal runs in the same
+kernel) and in worlds that mix small scenes, large scenes and scenes with
+breakable forces (which stay on the buckets).  Scenes whose bodies and
+incidence lists fit run on an LDS copy of them; the 300-box pile does not
+fit and runs on global memory.
 """
 import numpy as np
 import pytest
 
 import oracle_py as O
-from skyephyx_amd import scenes
+from skyephyx_amd import scenes, scenes_fast
 from skyephyx_amd.capi import Context
 from skyephyx_amd.world import World
 
@@ -47,6 +50,7 @@ CASES = {
     "cloth": lambda: scenes.cloth_sheet(n=6, spacing=0.25),
     "fem_block": lambda: scenes.fem_block(n=5, cell=0.25),
     "out_of_bounds": lambda: scenes.out_of_bounds(),
+    "pile_over_lds": lambda: _warm(scenes_fast.box_pile(300, np.random.default_rng(2)), 30),
     "mixed_world": lambda: World.concat([
         _warm(scenes.pyramid_2d(rows=6), 20),
         scenes.spring_bridge(6),          # breakable springs: bucket path
@@ -65,6 +69,7 @@ def test_per_scene_kernel_matches_buckets(case, lanes):
     with Context(lanes=lanes, fem_lanes=lanes) as f, Context(lanes=lanes, fem_lanes=lanes) as b:
         f.set_option("fused_scenes", 1)
         b.set_option("fused_scenes", 0)
+        b.set_option("fused_single_scene", 0)
         f.upload(world)
         b.upload(world)
         for k in range(8):
@@ -81,9 +86,29 @@ def test_per_scene_kernel_with_scene_subsets():
     with Context(lanes=4, fem_lanes=4) as f, Context(lanes=4, fem_lanes=4) as b:
         f.set_option("fused_scenes", 1)
         b.set_option("fused_scenes", 0)
+        b.set_option("fused_single_scene", 0)
         f.upload(world)
         b.upload(world)
         for k, ids in enumerate(subsets):
             f.step_scenes(ids, 1)
             b.step_scenes(ids, 1)
             _equal(f.download(), b.download(), k)
+
+
+def test_scene_lds_copy_matches_global_memory():
+    """The per-scene kernel on LDS copies (default) and on global memory
+    (option scene_lds_bytes 0) produce the same bits; the option is range
+    checked."""
+    world = CASES["mixed_world"]()
+    with Context(lanes=4, fem_lanes=4) as l, Context(lanes=4, fem_lanes=4) as g:
+        l.set_option("fused_scenes", 1)
+        g.set_option("fused_scenes", 1)
+        g.set_option("scene_lds_bytes", 0)
+        with pytest.raises(Exception, match="scene_lds_bytes"):
+            g.set_option("scene_lds_bytes", 65537)
+        l.upload(world)
+        g.upload(world)
+        for k in range(6):
+            l.step(1)
+            g.step(1)
+            _equal(l.download(), g.download(), k)
