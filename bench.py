@@ -65,7 +65,7 @@ def make_workload(name):
 
 KERNEL_CLASSES = ("primal_forces", "primal_fem", "dual")
 KERNEL_NAMES = {
-    "primal_fem": "k_primal<G, true> (FEM-only bucket of a primal colour sweep)",
+    "primal_fem": "k_primal_fem<G> (FEM-only bucket of a primal colour sweep)",
     "primal_forces": "k_primal<G, false> (bucket of bodies with forces)",
     "dual": "k_dual (dual and penalty update of every contact, joint and spring row)",
 }
@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--no-boxes20k", action="store_true",
                     help="skip the plain GPU step vs the CPU step on the landed 20k-box world")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel of a step one by one")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="avbd_set_option on the benchmark context (repeatable)")
     ap.add_argument("--fused", action="store_true", help="small scenes in the per-scene kernel (option fused_scenes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene comparison")
@@ -361,6 +363,9 @@ def main():
         ctx.set_option("fused_scenes", 1)
     if args.debug_output:
         ctx.set_option("debug_output", 1)
+    for opt in args.option:
+        name, value = opt.split("=", 1)
+        ctx.set_option(name, int(value))
     t0 = time.perf_counter()
     ctx.upload(world)
     ctx.synchronize()
@@ -496,6 +501,7 @@ def main():
             "tuned_lanes_workgroup": {f"{'fem' if k[0] else 'forces'}/2^{k[1]}": list(v)
                                       for k, v in sorted(tune.items())},
             "colours_last_step": int(timings["n_colours"]),
+            "options": {o.split("=", 1)[0]: int(o.split("=", 1)[1]) for o in args.option},
         },
         "roofline": roof,
         "p50_step_ms": round(float(np.percentile(step_ms, 50)), 4),

@@ -87,14 +87,15 @@ AVBD_DEV void jacobi_cs(float app, float aqq, float apq, float& c, float& s) {
 
 // Eigenpairs of the symmetric 3x3 [a00 a01 a02; a01 a11 a12; a02 a12 a22]
 // after D. Eberly, "A Robust Eigensolver for 3x3 Symmetric Matrices"
-// (Geometric Tools, 2014).  The eigenvalue farther from the other two comes
-// from the trigonometric solution of the characteristic cubic (the largest
-// when det(A - qI) >= 0, else the smallest); its vector is the longest cross
-// product of two rows of A - e I, and the value is then refined to the
-// vector's Rayleigh quotient.  The other pair diagonalises the 2x2
-// restriction of A to the orthogonal complement with one Jacobi rotation.
-// No iteration and no sorting: the pairs come in no particular order and
-// the vectors are orthonormal to rounding, also for (near-)equal eigenvalues.
+// (Geometric Tools, 2014).  The eigenvalue farther from the other two is a
+// root of the normalised characteristic cubic (the largest when
+// det(A - qI) >= 0, else the smallest), found by two Newton steps from a
+// chord; its vector is the longest cross product of two rows of A - e I,
+// and the value is then refined to the vector's Rayleigh quotient.  The
+// other pair diagonalises the 2x2 restriction of A to the orthogonal
+// complement with one Jacobi rotation.  A fixed number of steps and no
+// sorting: the pairs come in no particular order and the vectors are
+// orthonormal to rounding, also for (near-)equal eigenvalues.
 struct SymEig3 {
     float e[3];
     v3 v[3];
@@ -117,8 +118,23 @@ AVBD_DEV void sym_eig3_closed(float a00, float a01, float a02, float a11, float 
     const float c00 = n11 * n22 - n12 * n12, c01 = n01 * n22 - n12 * n02, c02 = n01 * n12 - n11 * n02;
     float hd = 0.5f * (n00 * c00 - n01 * c01 + n02 * c02);
     hd = fminf(fmaxf(hd, -1.f), 1.f);
-    const float ang = acosf(hd) * (1.f / 3.f);
-    float es = q + 2.f * p * __cosf(hd >= 0.f ? ang : ang + 2.09439510f);
+    // The eigenvalues of B / p are the roots of beta^3 - 3 beta - 2 hd = 0
+    // (beta = 2 cos t turns it into cos 3t = hd).  The one farthest from
+    // the others, |beta| in [sqrt 3, 2] with the sign of hd, is at least
+    // sqrt 3 from them, so it is well conditioned: Newton's method from the
+    // chord through (0, sqrt 3) and (1, 2) in a = |hd| (start error below
+    // 0.014) reaches it to within one rounding in two steps, where
+    // 2 cos(acos(hd) / 3) would cost an arc cosine and a cosine.
+    const float ah = fabsf(hd);
+    float beta = fmaf(0.26794919f, ah, 1.73205081f);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const float b2 = beta * beta;
+        const float f = fmaf(beta, b2 - 3.f, -2.f * ah);
+        const float fp = fmaf(3.f, b2, -3.f);
+        beta = fmaf(-f, frcp(fp), beta);
+    }
+    float es = q + p * (hd >= 0.f ? beta : -beta);
     // rows of A - es I scaled by their largest entry: for a nearly scalar A
     // they are tiny and their cross products' squared norms would fall
     // below the normal range, where the hardware reciprocal square root

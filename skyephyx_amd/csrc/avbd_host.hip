@@ -427,6 +427,17 @@ void launch_primal_g(avbd_ctx* c, int G, int grid, int block, int bucket, int it
         }
         return;
     }
+    if (FEM) {
+        switch (G) {
+            case 1: hipLaunchKernelGGL(k_primal_fem<1>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+            case 2: hipLaunchKernelGGL(k_primal_fem<2>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+            case 4: hipLaunchKernelGGL(k_primal_fem<4>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+            case 8: hipLaunchKernelGGL(k_primal_fem<8>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+            case 16: hipLaunchKernelGGL(k_primal_fem<16>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+            default: hipLaunchKernelGGL(k_primal_fem<32>, dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
+        }
+        return;
+    }
     switch (G) {
         case 1: hipLaunchKernelGGL((k_primal<1, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;
         case 2: hipLaunchKernelGGL((k_primal<2, FEM>), dim3(grid), dim3(block), 0, c->stream, w, bucket, iter, poststab); break;

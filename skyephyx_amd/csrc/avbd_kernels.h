@@ -31,6 +31,8 @@ __global__ void k_primal_pair(DevWorld w, int colour, int iter, int poststab, in
 template <int G>
 __global__ void k_primal_canon(DevWorld w, int bucket, int iter, int poststab);
 template <int G>
+__global__ void k_primal_fem(DevWorld w, int bucket, int iter, int poststab);
+template <int G>
 __global__ void k_scene_step(DevWorld w, const int* scenes, int mode, int lds_bytes);
 __global__ void k_dual(DevWorld w, int iter);
 __global__ void k_rho_update(DevWorld w);

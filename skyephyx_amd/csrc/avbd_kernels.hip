@@ -1960,6 +1960,20 @@ __global__ __launch_bounds__(256) void k_primal(DevWorld w, int bucket, int iter
         primal_rigid_body<G>(w, lo, hi, iter, poststab, blockIdx.x, gridDim.x);
 }
 
+// The FEM-only bucket sweep with its own occupancy target: at least
+// AVBD_FEM_MIN_WAVES waves per SIMD, i.e. a budget of 512 / waves VGPRs
+// that the compiler keeps (4 waves: 128 VGPRs, met without spills; left
+// free the kernel takes 140 and runs 3 waves).  Measured on the 4096-scene
+// mix: 3 waves 392.7 us per launch, 4 waves 378.9 us, 5 waves (96 VGPRs,
+// 39 spilled) 469.5 us.  Its arithmetic is k_primal<G, true>'s.
+#ifndef AVBD_FEM_MIN_WAVES
+#define AVBD_FEM_MIN_WAVES 4
+#endif
+template <int G>
+__global__ __launch_bounds__(256, AVBD_FEM_MIN_WAVES) void k_primal_fem(DevWorld w, int bucket, int iter, int poststab) {
+    primal_fem_body<G>(w, w.col_off[bucket], w.col_off[bucket + 1], iter, poststab, blockIdx.x, gridDim.x);
+}
+
 // Both buckets of a colour in one launch: blocks [0, rigid_blocks) take the
 // bodies with forces (GR lanes each), the rest the FEM-only particles (GF
 // lanes).  The two sets of bodies are disjoint and neither reads what the
@@ -2001,6 +2015,12 @@ AVBD_PRIMAL_INST(4)
 AVBD_PRIMAL_INST(8)
 AVBD_PRIMAL_INST(16)
 AVBD_PRIMAL_INST(32)
+template __global__ void k_primal_fem<1>(DevWorld, int, int, int);
+template __global__ void k_primal_fem<2>(DevWorld, int, int, int);
+template __global__ void k_primal_fem<4>(DevWorld, int, int, int);
+template __global__ void k_primal_fem<8>(DevWorld, int, int, int);
+template __global__ void k_primal_fem<16>(DevWorld, int, int, int);
+template __global__ void k_primal_fem<32>(DevWorld, int, int, int);
 
 // Dual / penalty update (solver.cpp:457-489), one thread per contact, joint
 // or spring.  A fracture disables the force and queues its bodies so that
