@@ -175,7 +175,7 @@ def cpu_all_cores(world, ids, weights, seconds=8.0):
     return threads / (t_w / w_sum), threads, sum(len(t) for t in times)
 
 
-def plain_gpu_rate(world, steps=3, configs=((8, 16), (4, 4))):
+def plain_gpu_rate(world, steps=10, configs=((8, 16), (4, 4))):
     """Plain per-scene GPU step: each sampled scene alone in its own context,
     at the faster of two lane settings (8/16 lanes suit large scenes, 4/4
     small ones)."""

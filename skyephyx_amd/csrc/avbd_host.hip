@@ -167,6 +167,9 @@ struct avbd_ctx {
     std::vector<int> h_item_off, h_tile_off;
     int4* sub_items = nullptr;
     int* sub_tiles = nullptr;
+    int* run_dev = nullptr;    // the running scene ids of a subset step (device, S)
+    int* run_pin = nullptr;    // their pinned staging (S)
+    hipEvent_t run_ev = nullptr;  // recorded after the copy out of run_pin
     int4* pin_items = nullptr;
     int* pin_tiles = nullptr;
     std::vector<int> run_ids;  // the running scenes of a subset step, ascending
@@ -316,8 +319,11 @@ struct avbd_ctx {
         sub_tiles = nullptr;
         if (pin_items) hipHostFree(pin_items);
         if (pin_tiles) hipHostFree(pin_tiles);
+        if (run_pin) hipHostFree(run_pin);
         pin_items = nullptr;
         pin_tiles = nullptr;
+        run_pin = nullptr;
+        run_dev = nullptr;
         cub_tmp = nullptr;
         have_world = false;
         if (dbg.lines) hipFree(dbg.lines);
@@ -671,6 +677,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     A(&c->items, std::max(d.n_items, 1));
     A(&c->sub_items, std::max(d.n_items, 1));
     A(&c->sub_tiles, std::max(d.n_tiles, 1));
+    A(&c->run_dev, std::max(d.S, 1));
     A(&w.alive, Mcap + 1); A(&w.alive_off, Mcap + 1);
     A(&w.fracture_bodies, (size_t)2 * (J + SP) + 2);
     A(&w.g, G_WORDS);
@@ -783,10 +790,13 @@ int upload_tiles(avbd_ctx* c, const std::vector<int>& off) {
     c->n_items = (int)items.size();
     if (c->pin_items) hipHostFree(c->pin_items);
     if (c->pin_tiles) hipHostFree(c->pin_tiles);
+    if (c->run_pin) hipHostFree(c->run_pin);
     c->pin_items = nullptr;
     c->pin_tiles = nullptr;
+    c->run_pin = nullptr;
     HIP_TRY(hipHostMalloc((void**)&c->pin_items, std::max<size_t>(items.size(), 1) * sizeof(int4), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&c->pin_tiles, std::max<size_t>(tscene.size(), 1) * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&c->run_pin, std::max<size_t>(S, 1) * sizeof(int), hipHostMallocDefault));
     c->h_items = std::move(items);
     c->h_item_off = std::move(ioff);
     c->h_tile_off = std::move(toff);
@@ -1732,6 +1742,7 @@ int avbd_create(int32_t device, avbd_ctx** out) {
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
         hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->run_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork2, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->join2, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&c->host_word, sizeof(int), hipHostMallocDefault) != hipSuccess) {
@@ -1761,6 +1772,7 @@ int avbd_destroy(avbd_ctx* c) {
     if (c->join) hipEventDestroy(c->join);
     if (c->fork2) hipEventDestroy(c->fork2);
     if (c->join2) hipEventDestroy(c->join2);
+    if (c->run_ev) hipEventDestroy(c->run_ev);
     if (c->side) hipStreamDestroy(c->side);
     if (c->side2) hipStreamDestroy(c->side2);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -2076,12 +2088,9 @@ static int step_impl(avbd_ctx* c, int32_t n_steps) {
 int avbd_step(avbd_ctx* c, int32_t n_steps) {
     if (!c) return fail(AVBD_E_ARG, "null context");
     if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
-    std::vector<int> st((size_t)c->w.S * ST_WORDS);
-    HIP_TRY(hipMemcpy(st.data(), c->w.st, st.size() * sizeof(int), hipMemcpyDeviceToHost));
-    bool changed = false;
-    for (int s = 0; s < c->w.S; ++s)
-        if (!st[s * ST_WORDS + ST_ACTIVE]) { st[s * ST_WORDS + ST_ACTIVE] = 1; changed = true; }
-    if (changed) HIP_TRY(hipMemcpy(c->w.st, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    // every scene runs: the flags are set on the device, in stream order
+    // (no read-back of the status words)
+    hipLaunchKernelGGL(k_set_active, dim3(grid_for(c->w.S)), dim3(256), 0, c->stream, c->w, (const int*)nullptr, 0, 1);
     c->subset = false;
     return step_impl(c, n_steps);
 }
@@ -2089,18 +2098,28 @@ int avbd_step(avbd_ctx* c, int32_t n_steps) {
 int avbd_step_scenes(avbd_ctx* c, const int32_t* ids, int32_t n_ids, int32_t n_steps) {
     if (!c || (!ids && n_ids > 0)) return fail(AVBD_E_ARG, "null argument");
     if (!c->have_world) return fail(AVBD_E_STATE, "no world uploaded");
-    std::vector<int> st((size_t)c->w.S * ST_WORDS);
-    HIP_TRY(hipMemcpy(st.data(), c->w.st, st.size() * sizeof(int), hipMemcpyDeviceToHost));
-    for (int s = 0; s < c->w.S; ++s) st[s * ST_WORDS + ST_ACTIVE] = 0;
+    const int S = c->w.S;
+    std::vector<char> on(S, 0);
     for (int i = 0; i < n_ids; ++i) {
-        if (ids[i] < 0 || ids[i] >= c->w.S) return fail(AVBD_E_ARG, "scene id out of range");
-        st[ids[i] * ST_WORDS + ST_ACTIVE] = 1;
+        if (ids[i] < 0 || ids[i] >= S) return fail(AVBD_E_ARG, "scene id out of range");
+        on[ids[i]] = 1;
     }
-    HIP_TRY(hipMemcpy(c->w.st, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
     c->run_ids.clear();
-    for (int s = 0; s < c->w.S; ++s)
-        if (st[s * ST_WORDS + ST_ACTIVE]) c->run_ids.push_back(s);
-    c->subset = (int)c->run_ids.size() < c->w.S;
+    for (int s = 0; s < S; ++s)
+        if (on[s]) c->run_ids.push_back(s);
+    // the run flags are set on the device, in stream order (no read-back of
+    // the status words): clear every scene's, then mark the listed ones from
+    // the pinned staging (which the previous call's copy has left by now)
+    HIP_TRY(hipEventSynchronize(c->run_ev));
+    const int n = (int)c->run_ids.size();
+    std::copy(c->run_ids.begin(), c->run_ids.end(), c->run_pin);
+    hipLaunchKernelGGL(k_set_active, dim3(grid_for(S)), dim3(256), 0, c->stream, c->w, (const int*)nullptr, 0, 0);
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(c->run_dev, c->run_pin, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipEventRecord(c->run_ev, c->stream));
+        hipLaunchKernelGGL(k_set_active, dim3(grid_for(n)), dim3(256), 0, c->stream, c->w, (const int*)c->run_dev, n, 0);
+    }
+    c->subset = n < S;
     const int rc = step_impl(c, n_steps);
     c->subset = false;
     return rc;

@@ -5,6 +5,7 @@
 
 __global__ void k_fill_int(int* p, int n, int v);
 __global__ void k_set_word(int* p, int v);
+__global__ void k_set_active(DevWorld w, const int* ids, int n, int all);
 __global__ void k_tile_bounds(DevWorld w, const int* tiles);
 __global__ void k_broad(DevWorld w, const int4* items);
 __global__ void k_pair_filter(DevWorld w);

@@ -46,6 +46,16 @@ __global__ void k_set_word(int* p, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+// Scene run flags of a step: every scene (all = 1), or the n listed scenes
+// (the others cleared first by a launch with n = 0, all = 0).
+__global__ void k_set_active(DevWorld w, const int* ids, int n, int all) {
+    if (n == 0) {
+        GRID_STRIDE(s, w.S) w.st[s * ST_WORDS + ST_ACTIVE] = all;
+    } else {
+        GRID_STRIDE(i, n) w.st[ids[i] * ST_WORDS + ST_ACTIVE] = 1;
+    }
+}
+
 // ------------------------------------------------------------------ phase 1
 // Is there already a force linking bodies i and j?  (isConstrainedTo,
 // force.hpp:81-89): scan the shorter incidence list.
@@ -1461,14 +1471,14 @@ AVBD_DEV float elem_energy_change(const ElemIn& x, const ElemPos& p, v3 dx) {
 // performs on that block; the rotation increment is exactly zero.
 // One FEM-only particle: its G lanes (a group within one wave) share the
 // particle's energy list.
-// Positions of an element from the vertex's neighbour slots in LDS: slot
-// byte 0xFF is the vertex itself (or a triangle's unused slot 3).
-AVBD_DEV void elem_pos_lds(const float4* nbp, uint32_t l, v3 own, ElemPos& p) {
-    const uint32_t i0 = l & 0xFFu, i1 = (l >> 8) & 0xFFu, i2 = (l >> 16) & 0xFFu, i3 = l >> 24;
-    p.p0 = i0 == 0xFFu ? own : f3(nbp[i0]);
-    p.p1 = i1 == 0xFFu ? own : f3(nbp[i1]);
-    p.p2 = i2 == 0xFFu ? own : f3(nbp[i2]);
-    p.p3 = i3 == 0xFFu ? own : f3(nbp[i3]);
+// Positions of an element from the vertex's neighbour slots in LDS (slot
+// NB_MAX holds the vertex itself, which is also a triangle's unused slot 3):
+// four unconditional LDS reads, no selects.
+AVBD_DEV void elem_pos_lds(const float4* nbp, uint32_t l, ElemPos& p) {
+    p.p0 = f3(nbp[l & 0xFFu]);
+    p.p1 = f3(nbp[(l >> 8) & 0xFFu]);
+    p.p2 = f3(nbp[(l >> 16) & 0xFFu]);
+    p.p3 = f3(nbp[l >> 24]);
 }
 
 // Order LDS writes of a wave's lanes before other lanes of the same wave
@@ -1482,13 +1492,28 @@ AVBD_DEV void wave_lds_sync() {
 // The block solve of a FEM-only particle and its writes (the system is
 // blockdiag(A, 0): the rotation increment is exactly zero).
 // rot, last_rot and last_pos are the particle's, loaded by the caller.
+AVBD_DEV bool identity_bits(float4 q) {
+    return __float_as_uint(q.x) == 0x3f800000u && (__float_as_uint(q.y) | __float_as_uint(q.z) | __float_as_uint(q.w)) == 0u;
+}
 AVBD_DEV void fem_vertex_solve(const DevWorld& w, int b, v3 p, float L[6], float r[3], float4 rot, float4 last_rot,
                                float4 last_pos) {
     float dx[3];
     ldlt3_solve(L, r, dx);
+    v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
+    if (identity_bits(rot) && identity_bits(last_rot)) {
+        // A particle no contact has turned: the update below gives
+        // normalize(QuaternionFromDifference(0, -1) * (1, +0, +0, +0)) =
+        // (1, +0, +0, +0) again and a rotation difference of +0 from the
+        // step start (also the identity), so the same bits are written
+        // without that arithmetic (and the unchanged rotation is not stored).
+        w.pos[b] = v4(np, 0.f);
+        const v3 dl = sub(np, f3(last_pos));
+        w.diff0[b] = make_float4(dl.x, dl.y, dl.z, 0.f);
+        w.diff1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
     const qt q = fq(rot);
     const qt lr = fq(last_rot);
-    v3 np = sub(p, v3{dx[0], dx[1], dx[2]});
     qt dq = quaternion_from_difference(v3{0.f, 0.f, 0.f}, -1.f);
     qt nq = qnormalized(qmul(dq, q));
     w.pos[b] = v4(np, 0.f);
@@ -1519,7 +1544,7 @@ AVBD_DEV void fem_trust_terms(const DevWorld& w, int s, int lane, v3 p, v3 ipos,
         ElemIn x;
         elem_load(w, k, x);
         ElemPos q;
-        if (nbp) elem_pos_lds(nbp, w.ev_l[k], p, q);
+        if (nbp) elem_pos_lds(nbp, w.ev_l[k], q);
         else elem_gather(w, x.v, q);
         const float de = elem_energy_change(x, q, d);
         dE = dE + de;
@@ -1617,6 +1642,7 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
         for (int t = 0; t < PER; ++t) nv[t] = w.pos[lane + t * G < nn ? nid[t] : b];
 #pragma unroll
         for (int t = 0; t < PER; ++t) nbp[lane + t * G] = nv[t];
+        if (lane == 0) nbp[NB_MAX] = p4;  // the vertex itself
         wave_lds_sync();
         if (k < e1) {
             bool h0 = true;
@@ -1630,7 +1656,7 @@ AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, in
                 in1.c = w.ev_c[k1];
                 const uint32_t l1 = w.ev_l[k1];
                 ElemPos p0;
-                elem_pos_lds(nbp, l0, p, p0);
+                elem_pos_lds(nbp, l0, p0);
                 v3 g;
                 float H[6];
                 elem_terms(in0, p0, pm, g, H);
@@ -1843,11 +1869,12 @@ template <int G>
 AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int poststab, int bid, int nblocks) {
     const int lane = threadIdx.x % G;
     const int gpb = blockDim.x / G;
-    // neighbour position slots: NB_MAX per lane group (32 KiB at G = 4 and
-    // 256 threads); narrower groups would need too much LDS and gather
+    // neighbour position slots: NB_MAX + 1 per lane group (the vertex's
+    // neighbours, then the vertex itself; 33 KiB at G = 4 and 256 threads);
+    // narrower groups would need too much LDS and gather
     constexpr int kGroups = G >= 4 ? 256 / G : 1;
-    __shared__ float4 nb_lds[kGroups * NB_MAX];
-    float4* slot = (G >= 4 && w.use_nb) ? nb_lds + (threadIdx.x / G) * NB_MAX : nullptr;
+    __shared__ float4 nb_lds[kGroups * (NB_MAX + 1)];
+    float4* slot = (G >= 4 && w.use_nb) ? nb_lds + (threadIdx.x / G) * (NB_MAX + 1) : nullptr;
     for (int base = bid * gpb; base < hi - lo; base += nblocks * gpb) {
         const int item = base + threadIdx.x / G;
         if (item >= hi - lo) continue;  // lane groups never straddle a wave: the group leaves together
@@ -2505,7 +2532,8 @@ __global__ void k_element_stream(DevWorld w, int n) {
 // after the stream is built or rebuilt): one thread per vertex walks its
 // entries in order, keeps the distinct other vertices in first-appearance
 // order at nb_id[2 e_off[b] ...] (room for two per entry, at most NB_MAX),
-// and writes each entry's slot indices into ev_l.  A vertex whose
+// and writes each entry's slot indices into ev_l (NB_MAX for the vertex
+// itself, whose position the sweep keeps in the slot after the list).  A vertex whose
 // neighbours do not fit gets nb_n = -1 and is swept by element ids.
 __global__ void k_nb_build(DevWorld w, int b0, int b1) {
     GRID_STRIDE(t, b1 - b0) {
@@ -2521,7 +2549,8 @@ __global__ void k_nb_build(DevWorld w, int b0, int b1) {
             const int local = (int)EENT_LOCAL(e);
             const int slots = EENT_TRI(e) ? 3 : 4;
             const int ids[4] = {v.x, v.y, v.z, v.w};
-            uint32_t packed = 0xFFFFFFFFu;
+            // slot NB_MAX: the vertex itself (and a triangle's unused slot 3)
+            uint32_t packed = (uint32_t)NB_MAX * 0x01010101u;
             for (int q = 0; q < slots; ++q) {
                 if (q == local) continue;
                 int j = 0;

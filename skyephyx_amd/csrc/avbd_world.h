@@ -170,8 +170,8 @@ struct DevWorld {
     // neighbours (first appearance in its element list) at
     // nb_id[2 e_off[b] ...], nb_n[b] of them (-1: more than fit, the
     // vertex gathers by element ids instead); ev_l[k] packs, per slot of
-    // entry k's element, the slot vertex's index in that list (byte s; 0xFF
-    // for the vertex itself and a triangle's unused slot 3).
+    // entry k's element, the slot vertex's index in that list (byte s;
+    // NB_MAX for the vertex itself and a triangle's unused slot 3).
     int* nb_id;         // [2 (4T + 3TR) + 1]
     int* nb_n;          // [N]
     uint32_t* ev_l;     // [4T + 3TR + 1]
