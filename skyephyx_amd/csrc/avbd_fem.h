@@ -317,15 +317,23 @@ AVBD_DEV void neohookean_vertex_terms(v3 p0, v3 p1, v3 p2, v3 p3, const m3& DmIn
             u[k] = smul(frcp(fmaxf(fmaf(s[i], s[j], s[k]), 1e-30f)), fc::mv(G, E.v[k]));
             w[k] = fc::dot(E.v[k], gradN);
         }
-        // twist modes (the ones that fall below the floor in the usual case)
-        float cf[3];
+        // twist modes (the ones that fall below the floor in the usual case);
+        // a mode the floor leaves has a correction of exactly 0
+        float ct[3], cf[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const int i = (k + 1) % 3, j = (k + 2) % 3;
             const float et = fmaf(dPhi, s[k], mu), ef = fmaf(-dPhi, s[k], mu);
-            const float ct = project_ev(et, projMode) - et;
+            ct[k] = project_ev(et, projMode) - et;
             cf[k] = project_ev(ef, projMode) - ef;
-            add_outer(Hv, 0.5f * ct, fc::lin2(w[j], u[i], -w[i], u[j]));
+        }
+        add_outer(Hv, 0.5f * ct[0], fc::lin2(w[2], u[1], -w[1], u[2]));
+        // Pair 0 is the eigensolver's best-separated one, the stretch of a
+        // uniaxially stretched element and usually the only mode below the
+        // floor: the other two (zero outer products otherwise, added in the
+        // same order) run only where one of them is needed.
+        if (ct[1] != 0.f || ct[2] != 0.f) {
+            add_outer(Hv, 0.5f * ct[1], fc::lin2(w[0], u[2], -w[2], u[0]));
+            add_outer(Hv, 0.5f * ct[2], fc::lin2(w[1], u[0], -w[0], u[1]));
         }
         if (cf[0] > 0.f || cf[1] > 0.f || cf[2] > 0.f) {
 #pragma unroll

@@ -661,7 +661,7 @@ int allocate_world(avbd_ctx* c, const WorldDims& d, avbd_scene** d_sc, int** d_o
     A(&w.ev_a, (size_t)4 * T + 3 * TR + 1); A(&w.ev_b, (size_t)4 * T + 3 * TR + 1);
     A(&w.ev_c, (size_t)4 * T + 3 * TR + 1); A(&w.ev_v, (size_t)4 * T + 3 * TR + 1);
     A(&w.nb_id, 2 * ((size_t)4 * T + 3 * TR) + 1); A(&w.nb_n, N); A(&w.ev_l, (size_t)4 * T + 3 * TR + 1);
-    A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N);
+    A(&w.colour, N); A(&w.col_cnt, AVBD_BUCKETS + 1); A(&w.col_off, AVBD_BUCKETS + 1); A(&w.col_list, N); A(&w.col_fem, N);
     A(&w.ccount, N); A(&w.cseed, N); A(&w.queue, (size_t)2 * N + 1); A(&w.cnt_i, N + 1); A(&w.cur_i, N + 1);
     w.pair_cap = (int)d.pair_cap;
     A(&w.pairs, w.pair_cap); A(&w.pairs_tmp, w.pair_cap);
@@ -708,6 +708,9 @@ int upload_energies(avbd_ctx* c, int N, const avbd_tet* tets, int T, const avbd_
         if (r.kind == 0) for (int k = 0; k < 4; ++k) cnt[tets[r.idx].v[k]]++;
         else for (int k = 0; k < 3; ++k) cnt[tris[r.idx].v[k]]++;
     }
+    // a FEM-only bucket entry packs a vertex's element count in 23 bits
+    if (N > 0 && *std::max_element(cnt.begin(), cnt.end()) >= (1 << 23))
+        return fail(AVBD_E_CAPACITY, "a vertex with 2^23 or more elements");
     std::vector<int> eoff(N + 1, 0);
     for (int i = 0; i < N; ++i) eoff[i + 1] = eoff[i] + cnt[i];
     std::vector<uint32_t> ent(std::max(1, eoff[N]));

@@ -1043,7 +1043,13 @@ __global__ __launch_bounds__(256) void k_colour_fill(DevWorld w) {
         for (int k = threadIdx.x; k < AVBD_BUCKETS; k += blockDim.x)
             if (h[k]) base[k] = w.col_off[k] + atomicAdd(&w.col_cnt[k], h[k]);
         __syncthreads();
-        if (kb >= 0) w.col_list[base[kb] + r] = b;
+        if (kb >= 0) {
+            w.col_list[base[kb] + r] = b;
+            if (kb & 1) {
+                const int e0 = w.e_off[b];
+                w.col_fem[base[kb] + r] = make_int4(b, e0, w.scene[b], ((w.e_off[b + 1] - e0) << 8) | (w.nb_n[b] + 1));
+            }
+        }
         __syncthreads();
     }
 }
@@ -1567,14 +1573,27 @@ AVBD_DEV void fem_trust_terms(const DevWorld& w, int s, int lane, v3 p, v3 ipos,
 }
 
 template <int G>
+AVBD_DEV void primal_fem_vertex_at(const DevWorld& w, int b, int s, int e0, int e1, int nn, int lane, int iter,
+                                   int poststab, float4* nbp);
+
+// One FEM-only particle b (its scene, element range and neighbour count
+// read here).
+template <int G>
 AVBD_DEV void primal_fem_vertex(const DevWorld& w, int b, int lane, int iter, int poststab, float4* nbp = nullptr) {
+    primal_fem_vertex_at<G>(w, b, w.scene[b], w.e_off[b], w.e_off[b + 1], nbp ? w.nb_n[b] : -1, lane, iter, poststab,
+                            nbp);
+}
+
+// One FEM-only particle b of scene s with energy entries [e0, e1) and nn
+// neighbours in its list (-1: no list, or more than fit).
+template <int G>
+AVBD_DEV void primal_fem_vertex_at(const DevWorld& w, int b, int s, int e0, int e1, int nn, int lane, int iter,
+                                   int poststab, float4* nbp) {
     // What the sweep needs of b is loaded at once, before the scene's run
     // test, so a particle waits on few dependent round trips before its
-    // first element: its own records, then its scene and its neighbour ids,
-    // then the neighbours' positions.
-    const int s = w.scene[b];
-    const int e0 = w.e_off[b], e1 = w.e_off[b + 1];
-    const int nn = nbp ? w.nb_n[b] : -1;
+    // first element: its own records, scene words, neighbour ids and first
+    // stream record, then the neighbours' positions.
+    if (!nbp) nn = -1;
     const float4 p4 = w.pos[b], vel = w.vel[b], ipos = w.inert_pos[b];
     // the neighbour ids of this lane's slots (clamped into the list; a slot
     // past the list's end later gathers the particle itself, unused)
@@ -1878,7 +1897,9 @@ AVBD_DEV void primal_fem_body(const DevWorld& w, int lo, int hi, int iter, int p
     for (int base = bid * gpb; base < hi - lo; base += nblocks * gpb) {
         const int item = base + threadIdx.x / G;
         if (item >= hi - lo) continue;  // lane groups never straddle a wave: the group leaves together
-        primal_fem_vertex<G>(w, w.col_list[lo + item], lane, iter, poststab, slot);
+        const int4 ent = w.col_fem[lo + item];
+        primal_fem_vertex_at<G>(w, ent.x, ent.z, ent.y, ent.y + (ent.w >> 8), (ent.w & 0xFF) - 1, lane, iter, poststab,
+                                slot);
     }
 }
 

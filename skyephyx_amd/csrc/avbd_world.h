@@ -181,6 +181,10 @@ struct DevWorld {
     int* col_cnt;       // [AVBD_BUCKETS + 1]
     int* col_off;       // [AVBD_BUCKETS + 1]
     int* col_list;      // [N]
+    // per entry of a FEM-only bucket (at its col_list index): the particle,
+    // its first energy entry, its scene and (its element count << 8) |
+    // (nb_n + 1), so the sweep starts from one load instead of two levels
+    int4* col_fem;      // [N]
     // scratch for the colouring and the broad phase
     int* ccount;        // [N]
     int* cseed;         // [N] bodies whose set of lower-index neighbours changed
