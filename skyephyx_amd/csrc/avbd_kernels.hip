@@ -1729,6 +1729,9 @@ AVBD_DEV void primal_fem_vertex_at(const DevWorld& w, int b, int s, int e0, int 
         }
     }
     }
+    // the records the solve's writes need, loaded before the lane sums so
+    // their round trip overlaps them
+    const float4 rot4 = w.rot[b], lrot4 = w.last_rot[b], lpos4 = w.last_pos[b];
     if (G > 1) {
 #pragma unroll
         for (int off = 1; off < G; off <<= 1) {
@@ -1739,7 +1742,7 @@ AVBD_DEV void primal_fem_vertex_at(const DevWorld& w, int b, int s, int e0, int 
         }
     }
     if (trust) fem_trust_terms<G>(w, s, lane, p, f3(ipos), vel.w, dt2, L, r, e0, e1, (G >= 4 && nn >= 0) ? nbp : nullptr);
-    if (lane == 0) fem_vertex_solve(w, b, p, L, r);
+    if (lane == 0) fem_vertex_solve(w, b, p, L, r, rot4, lrot4, lpos4);
 }
 
 // Canonical lane sums: the FEM-only update with a result that does not
